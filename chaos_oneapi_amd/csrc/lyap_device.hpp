@@ -1,0 +1,402 @@
+// Device building blocks of the gfx950 Lyapunov sweep: deterministic
+// elementary functions, the counter-based generator, the DVERK 6(5) and
+// Dormand-Prince 5(4) tableaux, the model right-hand sides and Jacobian
+// products, the step-size controller and the per-lane orthogonalisation.
+//
+// Arithmetic contract: compiled with -ffp-contract=off; every fused
+// multiply-add is written as fma().  Stage sums run in increasing stage order
+// (first term a product, then fma(a_j, k_j, acc), then fma(h, acc, y)), so
+// the FP64 path is bit-reproducible against a CPU build of the same
+// operations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "chaos_lyap.h"
+
+#define LYAP_HD __host__ __device__ __forceinline__
+#define LYAP_D __device__ __forceinline__
+
+namespace lyap {
+
+// ---------------------------------------------------------------------------
+// deterministic log / exp / pow (no libm; identical bits on host and device)
+// ---------------------------------------------------------------------------
+LYAP_HD uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
+LYAP_HD double double_of(uint64_t u) { return __builtin_bit_cast(double, u); }
+
+constexpr double kLn2Hi = 6.93147180369123816490e-01;
+constexpr double kLn2Lo = 1.90821492927058770002e-10;
+constexpr double kInf = __builtin_huge_val();
+
+// log(1+f) = f - s*(f - R), s = f/(2+f), R = 2*sum_{j>=1} s^(2j)/(2j+1)
+LYAP_HD double det_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -kInf;
+    if (x == kInf) return kInf;
+    uint64_t b = bits_of(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    int k;
+    if (e == 0) {
+        x = x * 18014398509481984.0;
+        b = bits_of(x);
+        e = (int)((b >> 52) & 0x7ff);
+        k = e - 1023 - 54;
+    } else {
+        k = e - 1023;
+    }
+    double m = double_of((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        k = k + 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double r = 2.0 / 25.0;
+    r = fma(r, z, 2.0 / 23.0);
+    r = fma(r, z, 2.0 / 21.0);
+    r = fma(r, z, 2.0 / 19.0);
+    r = fma(r, z, 2.0 / 17.0);
+    r = fma(r, z, 2.0 / 15.0);
+    r = fma(r, z, 2.0 / 13.0);
+    r = fma(r, z, 2.0 / 11.0);
+    r = fma(r, z, 2.0 / 9.0);
+    r = fma(r, z, 2.0 / 7.0);
+    r = fma(r, z, 2.0 / 5.0);
+    r = fma(r, z, 2.0 / 3.0);
+    r = r * z;
+    const double lg = f - s * (f - r);
+    const double dk = (double)k;
+    return fma(dk, kLn2Hi, fma(dk, kLn2Lo, lg));
+}
+
+// exp: nearest-integer reduction by ln2, degree-13 Taylor polynomial
+LYAP_HD double det_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return kInf;
+    if (x < -745.1332191019412) return 0.0;
+    double kd = x * 1.4426950408889634;
+    kd = (kd >= 0.0) ? floor(kd + 0.5) : ceil(kd - 0.5);
+    const double r = fma(-kd, kLn2Lo, fma(-kd, kLn2Hi, x));
+    double p = 1.0 / 6227020800.0;
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    int k = (int)kd;
+    if (k > 1023) {
+        p = p * double_of((uint64_t)(1023 + 1023) << 52);
+        k -= 1023;
+    }
+    if (k < -1022) {
+        p = p * double_of((uint64_t)1 << 52);
+        k += 1022;
+    }
+    return p * double_of((uint64_t)(k + 1023) << 52);
+}
+
+LYAP_HD double det_pow(double x, double y) {
+    if (x == 0.0) return 0.0;
+    return det_exp(y * det_log(x));
+}
+
+// ---------------------------------------------------------------------------
+// counter-based generator: splitmix64 finaliser keyed on (seed, point, stream)
+// ---------------------------------------------------------------------------
+LYAP_HD uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+LYAP_HD double uniform01(uint64_t seed, int64_t point, int64_t stream) {
+    const uint64_t z = mix64(seed ^ mix64((uint64_t)point * 1024ULL + (uint64_t)stream));
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// tableaux
+// ---------------------------------------------------------------------------
+template <int INTEG>
+struct Tab;
+
+// DVERK 6(5) (Verner's pair as published by Hull, Enright & Jackson); the
+// reference's fixed-step form is cuda/tools/dverk.dp.cpp:11-42.
+template <>
+struct Tab<LYAP_INT_DVERK> {
+    static constexpr int S = 8;
+    static constexpr double expo = 1.0 / 6.0 - 0.75 * 0.04;
+    static constexpr double a(int s, int j) {
+        constexpr double A[8][8] = {
+            {0},
+            {1.0 / 6.0},
+            {4.0 / 75.0, 16.0 / 75.0},
+            {5.0 / 6.0, -8.0 / 3.0, 5.0 / 2.0},
+            {-165.0 / 64.0, 55.0 / 6.0, -425.0 / 64.0, 85.0 / 96.0},
+            {12.0 / 5.0, -8.0, 4015.0 / 612.0, -11.0 / 36.0, 88.0 / 255.0},
+            {-8263.0 / 15000.0, 124.0 / 75.0, -643.0 / 680.0, -81.0 / 250.0, 2484.0 / 10625.0, 0.0},
+            {3501.0 / 1720.0, -300.0 / 43.0, 297275.0 / 52632.0, -319.0 / 2322.0,
+             24068.0 / 84065.0, 0.0, 3850.0 / 26703.0}};
+        return A[s][j];
+    }
+    static constexpr double b(int j) {
+        constexpr double B[8] = {3.0 / 40.0,      0.0, 875.0 / 2244.0,    23.0 / 72.0,
+                                 264.0 / 1955.0, 0.0, 125.0 / 11592.0, 43.0 / 616.0};
+        return B[j];
+    }
+    static constexpr double e(int j) {
+        constexpr double E[8] = {-1.0 / 160.0,    0.0,          -125.0 / 17952.0, 1.0 / 144.0,
+                                 -12.0 / 1955.0, -3.0 / 44.0, 125.0 / 11592.0,  43.0 / 616.0};
+        return E[j];
+    }
+    // stage 6 feeds only the error estimate
+    static constexpr bool for_solution(int s) { return s != 5; }
+};
+
+// Dormand-Prince 5(4), cuda/tools/dopri.dp.cpp:14-45; stage 7 is FSAL
+template <>
+struct Tab<LYAP_INT_DOPRI5> {
+    static constexpr int S = 7;
+    static constexpr double expo = 1.0 / 5.0 - 0.75 * 0.04;
+    static constexpr double a(int s, int j) {
+        constexpr double A[7][7] = {
+            {0},
+            {1.0 / 5.0},
+            {3.0 / 40.0, 9.0 / 40.0},
+            {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0},
+            {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0},
+            {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0},
+            {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+        return A[s][j];
+    }
+    static constexpr double b(int j) {
+        constexpr double B[7] = {35.0 / 384.0,      0.0,          500.0 / 1113.0, 125.0 / 192.0,
+                                 -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+        return B[j];
+    }
+    static constexpr double e(int j) {
+        constexpr double E[7] = {71.0 / 57600.0,       0.0,           -71.0 / 16695.0, 71.0 / 1920.0,
+                                 -17253.0 / 339200.0, 22.0 / 525.0, -1.0 / 40.0};
+        return E[j];
+    }
+    static constexpr bool for_solution(int s) { return s != 6; }
+};
+
+// Stage sums.  Coefficient lookups are constant after unrolling, so the
+// zero tests and the first-term choice resolve at compile time.
+template <class TB, class T, int LD, int N>
+LYAP_D T coef_sum_a(int s, const T (&k)[LD][N], int i, bool& any) {
+    T acc = T(0);
+    any = false;
+#pragma unroll
+    for (int j = 0; j < LD; ++j) {
+        if (j >= s) break;
+        const double c = TB::a(s, j);
+        if (c == 0.0) continue;
+        const T cj = (T)c;
+        if (!any) {
+            acc = cj * k[j][i];
+            any = true;
+        } else {
+            acc = fma(cj, k[j][i], acc);
+        }
+    }
+    return acc;
+}
+
+template <class TB, class T, int LD, int N>
+LYAP_D T coef_sum_b(const T (&k)[LD][N], int i) {
+    T acc = T(0);
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < LD; ++j) {
+        const double c = TB::b(j);
+        if (c == 0.0) continue;
+        const T cj = (T)c;
+        if (!any) {
+            acc = cj * k[j][i];
+            any = true;
+        } else {
+            acc = fma(cj, k[j][i], acc);
+        }
+    }
+    return acc;
+}
+
+template <class TB, class T, int LD, int N>
+LYAP_D T coef_sum_e(const T (&k)[LD][N], int i) {
+    T acc = T(0);
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < LD; ++j) {
+        const double c = TB::e(j);
+        if (c == 0.0) continue;
+        const T cj = (T)c;
+        if (!any) {
+            acc = cj * k[j][i];
+            any = true;
+        } else {
+            acc = fma(cj, k[j][i], acc);
+        }
+    }
+    return acc;
+}
+
+// ---------------------------------------------------------------------------
+// models: f(x) and J(x) v.  T is double, or float for the mixed tangent (the
+// caller rounds the state and parameters to float first).
+// ---------------------------------------------------------------------------
+struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27
+    static constexpr int ID = LYAP_MODEL_SHIMIZU, N = 3, P = 2;
+    template <class T>
+    LYAP_D static void rhs(const T* x, const T* p, T* o) {
+        o[0] = x[1];
+        o[1] = fma(-p[0], x[1], fma(-x[0], x[2], x[0]));
+        o[2] = fma(x[0], x[0], -p[1] * x[2]);
+    }
+    template <class T>
+    LYAP_D static void jvp(const T* x, const T* p, const T* v, T* o) {
+        o[0] = v[1];
+        o[1] = fma(T(1) - x[2], v[0], fma(-p[0], v[1], -x[0] * v[2]));
+        o[2] = fma(T(2) * x[0], v[0], -p[1] * v[2]);
+    }
+};
+
+struct Lorenz {  // (sigma, rho, beta)
+    static constexpr int ID = LYAP_MODEL_LORENZ, N = 3, P = 3;
+    template <class T>
+    LYAP_D static void rhs(const T* x, const T* p, T* o) {
+        o[0] = p[0] * (x[1] - x[0]);
+        o[1] = fma(x[0], p[1] - x[2], -x[1]);
+        o[2] = fma(x[0], x[1], -p[2] * x[2]);
+    }
+    template <class T>
+    LYAP_D static void jvp(const T* x, const T* p, const T* v, T* o) {
+        o[0] = p[0] * (v[1] - v[0]);
+        o[1] = fma(p[1] - x[2], v[0], -fma(x[0], v[2], v[1]));
+        o[2] = fma(x[1], v[0], fma(x[0], v[1], -p[2] * v[2]));
+    }
+};
+
+struct Rossler {  // (a, c, b)
+    static constexpr int ID = LYAP_MODEL_ROSSLER, N = 3, P = 3;
+    template <class T>
+    LYAP_D static void rhs(const T* x, const T* p, T* o) {
+        o[0] = -(x[1] + x[2]);
+        o[1] = fma(p[0], x[1], x[0]);
+        o[2] = fma(x[2], x[0] - p[1], p[2]);
+    }
+    template <class T>
+    LYAP_D static void jvp(const T* x, const T* p, const T* v, T* o) {
+        o[0] = -(v[1] + v[2]);
+        o[1] = fma(p[0], v[1], v[0]);
+        o[2] = fma(x[2], v[0], (x[0] - p[1]) * v[2]);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// step-size control (constants and update rule of cuda/tools/dopri.dp.cpp:9-12,
+// 100-139; the exponent is 1/q - 0.75*beta with q the embedded order + 1)
+// ---------------------------------------------------------------------------
+struct Ctrl {
+    double fac_old;
+    bool rejected;
+};
+
+template <int INTEG>
+LYAP_D bool ctrl_update(Ctrl& c, double err, double h_try, double h_max, double& h_next) {
+    constexpr double kSafe = 0.9, kFacc1 = 1.0 / 0.2, kFacc2 = 1.0 / 10.0, kBeta = 0.04;
+    const double fac11 = det_pow(err, Tab<INTEG>::expo);
+    double fac = fac11 / det_pow(c.fac_old, kBeta);
+    const double q = fac / kSafe;
+    fac = kFacc1 < q ? kFacc1 : q;
+    fac = kFacc2 > fac ? kFacc2 : fac;
+    double hn = h_try / fac;
+    if (err <= 1.0) {
+        c.fac_old = err > 1e-4 ? err : 1e-4;
+        if (fabs(hn) > h_max) hn = h_max;
+        if (c.rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
+        c.rejected = false;
+        h_next = hn;
+        return true;
+    }
+    const double q2 = fac11 / kSafe;
+    hn = h_try / (kFacc1 < q2 ? kFacc1 : q2);
+    c.rejected = true;
+    h_next = hn;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// per-point set-up shared by every kernel
+// ---------------------------------------------------------------------------
+// parameters: tasklyapunov.cpp:45-63 grid (p1 outer) unless given explicitly
+template <int P>
+LYAP_D void point_params(const lyap_sweep_desc& d, int64_t idx, const double* params_in,
+                         double* p) {
+    if (params_in) {
+#pragma unroll
+        for (int j = 0; j < P; ++j) p[j] = params_in[idx * P + j];
+        return;
+    }
+    const int64_t i1 = idx / d.n2, i2 = idx % d.n2;
+    const double v1 = d.start1 + (double)i1 * d.delta1;
+    const double v2 = d.start2 + (double)i2 * d.delta2;
+#pragma unroll
+    for (int j = 0; j < P; ++j)
+        p[j] = (j == d.sweep_idx1) ? v1 : ((j == d.sweep_idx2) ? v2 : d.base_params[j]);
+}
+
+// initial state: given, or init_state (+ p[offset]) + seeded perturbation
+// (uniform on [-scale, scale], or scale * Irwin-Hall(12) - 6 normal deviate)
+LYAP_D double init_component(const lyap_sweep_desc& d, int64_t idx, int i, const double* p) {
+    double x0 = d.init_state[i];
+    if (d.init_param_offset >= 0) x0 = x0 + p[d.init_param_offset];
+    double dx = 0.0;
+    if (d.init_kind == LYAP_INIT_UNIFORM) {
+        const double u = uniform01(d.seed, idx, i);
+        dx = d.init_scale * (2.0 * u - 1.0);
+    } else if (d.init_kind == LYAP_INIT_NORMAL) {
+        double z = 0.0;
+        for (int j = 0; j < 12; ++j) z = z + uniform01(d.seed, idx, 32 + i * 12 + j);
+        dx = d.init_scale * (z - 6.0);
+    }
+    return x0 + dx;
+}
+
+LYAP_D void kahan_add(double& s, double& c, double v) {
+    const double y = v - c;
+    const double t = s + y;
+    c = (t - s) - y;
+    s = t;
+}
+
+// outputs shared by all sweep kernels
+struct SweepIO {
+    int64_t first, count;
+    const int64_t* idx_list;
+    const double* params;
+    const double* inits;
+    double* lambda;
+    uint32_t* status;
+    uint32_t* n_acc;
+    uint32_t* n_rej;
+    unsigned long long* counter;  // work counter (persistent kernels)
+};
+
+}  // namespace lyap

@@ -1,0 +1,225 @@
+"""ctypes bindings of liblyap.so (include/chaos_lyap.h).
+
+The library is the product: the HIP kernels and the C-ABI.  This module only
+marshals descriptors and device pointers; device memory comes from torch
+(plumbing).  Loading fails loudly when the library has not been built — there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "liblyap.so")
+
+MAX_DIM = 32
+MAX_PARAMS = 8
+
+SUCCESS, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOT_FOUND = 0, -1, -2, -3, -4
+MODEL_SHIMIZU, MODEL_LORENZ, MODEL_ROSSLER, MODEL_L96 = 0, 1, 2, 3
+TYPE_FLOW, TYPE_MAP = 0, 1
+METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF = 0, 1, 2, 3
+INT_DVERK, INT_DOPRI5 = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+INIT_NONE, INIT_UNIFORM, INIT_NORMAL = 0, 1, 2
+PT_OK, PT_NONFINITE, PT_STEP_UNDERFLOW, PT_MAXSTEPS, PT_ESCAPED = 0, 1, 2, 3, 4
+
+# every entry point include/chaos_lyap.h declares
+EXPORTS = (
+    "lyap_last_error", "lyap_ctx_create", "lyap_ctx_destroy", "lyap_model_count",
+    "lyap_model_at", "lyap_model_lookup", "lyap_loop_count", "lyap_desc_finalize",
+    "lyap_sweep_run", "lyap_last_kernel_ms", "lyap_cuda_lyapunov",
+)
+
+
+class SweepDesc(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32), ("dim", ctypes.c_int32),
+        ("n_exps", ctypes.c_int32), ("n_params", ctypes.c_int32),
+        ("method", ctypes.c_int32), ("integrator", ctypes.c_int32),
+        ("adaptive", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("h", ctypes.c_double), ("rtol", ctypes.c_double), ("atol", ctypes.c_double),
+        ("h_min", ctypes.c_double), ("h_max", ctypes.c_double), ("eps", ctypes.c_double),
+        ("t_transient", ctypes.c_double), ("t_slave", ctypes.c_double),
+        ("t_average", ctypes.c_double), ("t_renorm", ctypes.c_double),
+        ("ref_steps_transient", ctypes.c_int64), ("ref_steps_slave", ctypes.c_int64),
+        ("ref_steps_average", ctypes.c_int64), ("add_skip_steps", ctypes.c_double),
+        ("steps_per_renorm", ctypes.c_int64),
+        ("n_renorm_transient", ctypes.c_int64), ("n_renorm_average", ctypes.c_int64),
+        ("max_steps", ctypes.c_int64), ("escape_radius", ctypes.c_double),
+        ("sweep_idx1", ctypes.c_int32), ("sweep_idx2", ctypes.c_int32),
+        ("n1", ctypes.c_int32), ("n2", ctypes.c_int32),
+        ("start1", ctypes.c_double), ("delta1", ctypes.c_double),
+        ("start2", ctypes.c_double), ("delta2", ctypes.c_double),
+        ("base_params", ctypes.c_double * MAX_PARAMS),
+        ("init_state", ctypes.c_double * MAX_DIM),
+        ("init_kind", ctypes.c_int32), ("init_param_offset", ctypes.c_int32),
+        ("init_scale", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+    def to_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if name in ("base_params", "init_state") else v
+        return out
+
+
+class ModelInfo(ctypes.Structure):
+    _fields_ = [
+        ("model", ctypes.c_int32), ("dim", ctypes.c_int32), ("n_params", ctypes.c_int32),
+        ("type", ctypes.c_int32), ("name", ctypes.c_char_p), ("param_names", ctypes.c_char_p),
+        ("phase_names", ctypes.c_char_p), ("equations", ctypes.c_char_p),
+    ]
+
+
+class LyapError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load liblyap.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: run chaos_oneapi_amd.build.build() first "
+                          "(the sweep has no CPU fallback)")
+    L = ctypes.CDLL(path)
+    P, i32, i64, d, vp = ctypes.POINTER, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+    L.lyap_last_error.restype = ctypes.c_char_p
+    L.lyap_ctx_create.argtypes = [ctypes.c_int, P(vp)]
+    L.lyap_ctx_destroy.argtypes = [vp]
+    L.lyap_model_count.restype = ctypes.c_int
+    L.lyap_model_at.argtypes = [ctypes.c_int, P(ModelInfo)]
+    L.lyap_model_lookup.argtypes = [ctypes.c_char_p, P(ModelInfo)]
+    L.lyap_loop_count.restype = i64
+    L.lyap_loop_count.argtypes = [d, d, d]
+    L.lyap_desc_finalize.argtypes = [P(SweepDesc)]
+    L.lyap_sweep_run.argtypes = [vp, P(SweepDesc), i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]
+    L.lyap_last_kernel_ms.argtypes = [vp, P(ctypes.c_float)]
+    L.lyap_cuda_lyapunov.argtypes = [
+        vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        vp, i32, i32, vp, i32, d, d, vp, i32, d, d, d, d, vp]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != SUCCESS:
+        raise LyapError(rc, load().lyap_last_error().decode())
+
+
+def make_desc(spec: dict) -> SweepDesc:
+    d = SweepDesc()
+    for name, _ in SweepDesc._fields_:
+        if name not in spec:
+            continue
+        v = spec[name]
+        if name in ("base_params", "init_state"):
+            arr = getattr(d, name)
+            for j, x in enumerate(v):
+                arr[j] = float(x)
+        else:
+            setattr(d, name, v)
+    return d
+
+
+def finalize(desc: SweepDesc) -> SweepDesc:
+    """Validate and fill the derived fields (host only, no GPU needed)."""
+    out = SweepDesc.from_buffer_copy(desc)
+    check(load().lyap_desc_finalize(ctypes.byref(out)))
+    return out
+
+
+def loop_count(t0: float, t1: float, h: float) -> int:
+    return int(load().lyap_loop_count(float(t0), float(t1), float(h)))
+
+
+def model_lookup(name: str) -> ModelInfo:
+    info = ModelInfo()
+    check(load().lyap_model_lookup(name.encode(), ctypes.byref(info)))
+    return info
+
+
+def models() -> list:
+    L = load()
+    out = []
+    for i in range(L.lyap_model_count()):
+        info = ModelInfo()
+        check(L.lyap_model_at(i, ctypes.byref(info)))
+        out.append(info)
+    return out
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    """A lyap_ctx on one device; run() launches on torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(load().lyap_ctx_create(int(device), ctypes.byref(self._h)))
+        self.device = device
+
+    def close(self):
+        if self._h:
+            load().lyap_ctx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, desc: SweepDesc, first: int = 0, count: int | None = None, idx_list=None,
+            params=None, inits=None, out=None, stream=None):
+        """Launch a sweep; returns (lam, status, n_acc, n_rej) device tensors (not synced)."""
+        import torch
+
+        if count is None:
+            count = len(idx_list) if idx_list is not None else desc.n1 * desc.n2 - first
+        dev = torch.device("cuda", self.device)
+        if out is None:
+            lam = torch.empty((count, desc.n_exps), dtype=torch.float64, device=dev)
+            st = torch.empty(count, dtype=torch.int32, device=dev)
+            na = torch.empty(count, dtype=torch.int32, device=dev)
+            nr = torch.empty(count, dtype=torch.int32, device=dev)
+        else:
+            lam, st, na, nr = out
+        for t in (idx_list, params, inits):
+            if t is not None:
+                assert t.is_cuda and t.is_contiguous()
+        s = stream if stream is not None else torch.cuda.current_stream(dev)
+        check(load().lyap_sweep_run(self._h, ctypes.byref(desc), int(first), int(count),
+                                    _ptr(idx_list), _ptr(params), _ptr(inits), _ptr(lam),
+                                    _ptr(st), _ptr(na), _ptr(nr), ctypes.c_void_p(s.cuda_stream)))
+        return lam, st, na, nr
+
+    def kernel_ms(self) -> float:
+        ms = ctypes.c_float()
+        check(load().lyap_last_kernel_ms(self._h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def cuda_lyapunov(self, task_num, is_qr, is_var, inits, dimension, model, params, params_dim,
+                      eps, h, lyap_exps, exps_num, time_skip, time_skip_clp, calc_time,
+                      add_skip_steps, stream=None):
+        """The reference-shaped launch (cudaLyapunov's arguments)."""
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", self.device))
+        check(load().lyap_cuda_lyapunov(
+            self._h, 32, 32, int(task_num), int(is_qr), int(is_var), TYPE_FLOW, _ptr(inits),
+            int(dimension), int(model), _ptr(params), int(params_dim), float(eps), float(h),
+            _ptr(lyap_exps), int(exps_num), float(time_skip), float(time_skip_clp),
+            float(calc_time), float(add_skip_steps), ctypes.c_void_p(s.cuda_stream)))

@@ -1,0 +1,174 @@
+/*
+ * chaos_lyap.h — C-ABI of the MI355X (gfx950) Lyapunov-spectrum sweep.
+ *
+ * This is the drop-in boundary under the reference's TaskLyapunov: every entry
+ * point takes plain pointers and sizes, reports errors as status codes (the
+ * message is in lyap_last_error()), never throws, and launches on the HIP
+ * stream it is given (NULL = the null stream).  Device buffers are owned by
+ * the caller; scratch (work counters) is owned by the context.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   lyap_cuda_lyapunov    <- cudaLyapunov,            cuda/lyapunov.dp.hpp:10-17,
+ *                                                      cuda/lyapunov.dp.cpp:321-367
+ *   lyap_sweep_run        <- kernLEDverk_ launch,      cuda/lyapunov.dp.cpp:262-318
+ *                            (plus the host grid of TaskLyapunov::allocate,
+ *                             tasklyapunov.cpp:45-71, computed on the device)
+ *   lyap_model_lookup     <- ModelFactoryRegistry::Storage lookup,
+ *                            model_factory.h:28-69, tasklyapunov.cpp:15-16
+ *   lyap_loop_count       <- the `for (double t = t0; t < T; t += h)` step
+ *                            count of cuda/lyapunov.dp.cpp:11,139,158,244
+ *   lyap_desc_finalize    <- TaskLyapunov::init/allocate's derived scalars,
+ *                            tasklyapunov.cpp:17-31,45-46,65
+ */
+#ifndef CHAOS_LYAP_H
+#define CHAOS_LYAP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LYAP_MAX_DIM 32
+#define LYAP_MAX_PARAMS 8
+
+/* status codes of the entry points */
+enum {
+    LYAP_SUCCESS = 0,
+    LYAP_ERR_INVALID = -1,     /* bad argument or descriptor */
+    LYAP_ERR_HIP = -2,         /* a HIP runtime call failed */
+    LYAP_ERR_UNSUPPORTED = -3, /* valid request this build has no kernel for */
+    LYAP_ERR_NOT_FOUND = -4    /* unknown model name */
+};
+
+/* models compiled into the library (ModelFactoryRegistry replacement) */
+enum {
+    LYAP_MODEL_SHIMIZU = 0, /* reference cuda/models/flows/shimizu.dp.cpp; params (alpha, lambda) */
+    LYAP_MODEL_LORENZ = 1,  /* params (sigma, rho, beta) */
+    LYAP_MODEL_ROSSLER = 2, /* params (a, c, b) */
+    LYAP_MODEL_L96 = 3      /* Lorenz-96, N in {8,16,32}; params (F, d) */
+};
+/* ModelType of model_factory.h:22-26 */
+enum { LYAP_TYPE_FLOW = 0, LYAP_TYPE_MAP = 1 };
+
+/* how the spectrum is computed */
+enum {
+    LYAP_METHOD_VAR = 0,     /* coupled variational RK, MGS on exact time boundaries */
+    LYAP_METHOD_VAR_REF = 1, /* reference VAR mode, cuda/lyapunov.dp.cpp:133-226 */
+    LYAP_METHOD_QR_REF = 2,  /* reference QR mode,  cuda/lyapunov.dp.cpp:229-259 */
+    LYAP_METHOD_FD_REF = 3   /* reference FD mode,  cuda/lyapunov.dp.cpp:5-129 */
+};
+enum { LYAP_INT_DVERK = 0, LYAP_INT_DOPRI5 = 1 };
+enum { LYAP_PREC_FP64 = 0, LYAP_PREC_MIXED = 1 };
+enum { LYAP_INIT_NONE = 0, LYAP_INIT_UNIFORM = 1, LYAP_INIT_NORMAL = 2 };
+
+/* per-point status word */
+enum {
+    LYAP_PT_OK = 0,
+    LYAP_PT_NONFINITE = 1,
+    LYAP_PT_STEP_UNDERFLOW = 2,
+    LYAP_PT_MAXSTEPS = 3,
+    LYAP_PT_ESCAPED = 4
+};
+
+/* One sweep.  Field meanings follow LyapunovParameters (tool_params.h:29-43);
+   the derived fields are filled by lyap_desc_finalize(). */
+typedef struct lyap_sweep_desc {
+    int32_t model, dim, n_exps, n_params;
+    int32_t method, integrator, adaptive, precision;
+    double h;           /* integration_step (initial step when adaptive) */
+    double rtol, atol;  /* adaptive tolerances */
+    double h_min, h_max;
+    double eps;         /* epsilon of the reference modes */
+    double t_transient; /* skip_time_attractor */
+    double t_slave;     /* skip_time_slave_trajectory */
+    double t_average;   /* total_time */
+    double t_renorm;    /* skip_time_normalization */
+    /* derived, reference modes: step counts of the float-accumulated loops */
+    int64_t ref_steps_transient, ref_steps_slave, ref_steps_average;
+    double add_skip_steps; /* t_renorm / h, compared with an int (tasklyapunov.cpp:65) */
+    /* derived, coupled-variational method */
+    int64_t steps_per_renorm; /* fixed step */
+    int64_t n_renorm_transient, n_renorm_average;
+    int64_t max_steps;    /* per point; exceeded -> LYAP_PT_MAXSTEPS */
+    double escape_radius; /* |x_i| above it -> LYAP_PT_ESCAPED */
+    /* sweep grid, p1 outer (tasklyapunov.cpp:45-63): p = start + i*delta */
+    int32_t sweep_idx1, sweep_idx2, n1, n2;
+    double start1, delta1, start2, delta2;
+    double base_params[LYAP_MAX_PARAMS]; /* initial_model_params */
+    /* initial state: init_state (+ p[init_param_offset]) + seeded perturbation */
+    double init_state[LYAP_MAX_DIM];
+    int32_t init_kind, init_param_offset;
+    double init_scale;
+    uint64_t seed;
+} lyap_sweep_desc;
+
+typedef struct lyap_model_info {
+    int32_t model;    /* LYAP_MODEL_* */
+    int32_t dim;      /* _dimension */
+    int32_t n_params; /* _param_number */
+    int32_t type;     /* LYAP_TYPE_* */
+    const char* name;
+    const char* param_names;
+    const char* phase_names;
+    const char* equations;
+} lyap_model_info;
+
+typedef struct lyap_ctx lyap_ctx;
+
+/* message of the last failing call on this thread ("" if none) */
+const char* lyap_last_error(void);
+
+/* a context per (host thread, device); owns the work counters */
+int lyap_ctx_create(int device, lyap_ctx** out);
+int lyap_ctx_destroy(lyap_ctx* ctx);
+
+/* model table */
+int lyap_model_count(void);
+int lyap_model_at(int i, lyap_model_info* out);
+int lyap_model_lookup(const char* name, lyap_model_info* out);
+
+/* host helpers */
+int64_t lyap_loop_count(double t0, double t1, double h);
+int lyap_desc_finalize(lyap_sweep_desc* d);
+
+/*
+ * Run points [first, first+count) of the sweep (or the points listed in
+ * d_idx_list[0..count) when it is not NULL).  Every pointer is device memory:
+ *   d_params  : optional, [n_total * n_params] AoS parameters (else the grid)
+ *   d_inits   : optional, [n_total * dim] AoS initial states (else the seeded rule)
+ *   d_lambda  : [count * n_exps] output, row i belongs to work item i
+ *   d_status, d_n_acc, d_n_rej : optional [count] outputs
+ * The call is stream-ordered and returns before the kernels finish.
+ */
+int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64_t count,
+                   const int64_t* d_idx_list, const double* d_params, const double* d_inits,
+                   double* d_lambda, uint32_t* d_status, uint32_t* d_n_acc, uint32_t* d_n_rej,
+                   void* stream);
+
+/* duration of the last sweep kernel launched by lyap_sweep_run on ctx
+   (HIP events on the launch stream; valid once that stream has drained) */
+int lyap_last_kernel_ms(lyap_ctx* ctx, float* ms);
+
+/*
+ * Reference-shaped launch: the arguments of cudaLyapunov
+ * (cuda/lyapunov.dp.hpp:10-17) minus the device function pointers (the model
+ * is named by id) and minus the per-work-item scratch (the kernels keep their
+ * state in registers).  d_inits/d_params are AoS per point as in
+ * tasklyapunov.cpp:47-71; d_lyap_exps receives task_num * exps_num values.
+ * is_qr/is_var select the reference QR/VAR/FD mode; block_num/block_size are
+ * accepted and ignored, like the reference's own override
+ * (cuda/lyapunov.dp.cpp:331-332).
+ */
+int lyap_cuda_lyapunov(lyap_ctx* ctx, int block_num, int block_size, size_t task_num, int is_qr,
+                       int is_var, int model_type, const double* d_inits, int32_t dimension,
+                       int32_t model, const double* d_params, int32_t params_dim, double eps,
+                       double integration_step, double* d_lyap_exps, int32_t exps_num,
+                       double time_skip, double time_skip_clp, double calc_time,
+                       double add_skip_steps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHAOS_LYAP_H */

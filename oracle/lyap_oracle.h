@@ -1,0 +1,117 @@
+/*
+ * CPU FP64 oracle for the Lyapunov-spectrum sweep.  TEST INFRASTRUCTURE ONLY:
+ * only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and only as the checker.  The product (chaos_oneapi_amd) never calls it.
+ *
+ * It restates, in plain C, the per-point algorithms of the reference:
+ *   - reference VAR mode  : cuda/lyapunov.dp.cpp:133-226 (deviceLESpectrumVARDverk_)
+ *   - reference QR mode   : cuda/lyapunov.dp.cpp:229-259 (deviceLESpectrumQR_)
+ *   - reference FD mode   : cuda/lyapunov.dp.cpp:5-129   (deviceLESpectrumDverk_)
+ *   - DVERK 6(5) tableau  : cuda/tools/dverk.dp.cpp:7-80
+ *   - Dopri5 + controller : cuda/tools/dopri.dp.cpp:7-140
+ *   - CGS / normalise / QR: cuda/tools/linalg.dp.cpp:5-128
+ *   - shimizu RHS/var     : cuda/models/flows/shimizu.dp.cpp:15-52
+ *   - grid rule           : tasklyapunov.cpp:45-63
+ * plus the method the sweep adds on top of the reference (coupled variational
+ * RK with the Jacobian at every stage state, modified Gram-Schmidt on exact
+ * time boundaries, adaptive DVERK/Dopri5, FP32 tangent stages with FP64
+ * renormalisation and Kahan-compensated log sums).
+ *
+ * The descriptor below has the same memory layout as lyap_sweep_desc in
+ * include/chaos_lyap.h so one Python dictionary can fill both.
+ */
+#ifndef LYAP_ORACLE_H
+#define LYAP_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_MAX_DIM 32
+#define ORC_MAX_PARAMS 8
+
+enum { ORC_MODEL_SHIMIZU = 0, ORC_MODEL_LORENZ = 1, ORC_MODEL_ROSSLER = 2, ORC_MODEL_L96 = 3 };
+enum { ORC_METHOD_VAR = 0, ORC_METHOD_VAR_REF = 1, ORC_METHOD_QR_REF = 2, ORC_METHOD_FD_REF = 3 };
+enum { ORC_INT_DVERK = 0, ORC_INT_DOPRI5 = 1 };
+enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };
+enum { ORC_INIT_NONE = 0, ORC_INIT_UNIFORM = 1, ORC_INIT_NORMAL = 2 };
+enum { ORC_OK = 0, ORC_NONFINITE = 1, ORC_STEP_UNDERFLOW = 2, ORC_MAXSTEPS = 3, ORC_ESCAPED = 4 };
+
+typedef struct orc_desc {
+    int32_t model, dim, n_exps, n_params;
+    int32_t method, integrator, adaptive, precision;
+    double h, rtol, atol, h_min, h_max, eps;
+    double t_transient, t_slave, t_average, t_renorm;
+    /* reference modes: loop counts of `for (double t = t0; t < T; t += h)` */
+    int64_t ref_steps_transient, ref_steps_slave, ref_steps_average;
+    double add_skip_steps;
+    /* added method: integer schedule */
+    int64_t steps_per_renorm;    /* fixed step only */
+    int64_t n_renorm_transient, n_renorm_average;
+    int64_t max_steps;
+    double escape_radius;
+    /* grid (p1 outer): p = start + i * delta */
+    int32_t sweep_idx1, sweep_idx2, n1, n2;
+    double start1, delta1, start2, delta2;
+    double base_params[ORC_MAX_PARAMS];
+    /* initial state */
+    double init_state[ORC_MAX_DIM];
+    int32_t init_kind, init_param_offset;
+    double init_scale;
+    uint64_t seed;
+} orc_desc;
+
+/* deterministic elementary functions (same bits on every IEEE machine) */
+double orc_log(double x);
+double orc_exp(double x);
+double orc_pow(double x, double y);
+
+/* counter-based generator: uniform in [0,1) from (seed, point, stream) */
+double orc_uniform(uint64_t seed, int64_t point, int64_t stream);
+/* `for (double t = t0; t < t1; t += h) ++n;` */
+int64_t orc_loop_count(double t0, double t1, double h);
+
+/* model right-hand side and Jacobian-vector product (FP64 and FP32 forms) */
+void orc_rhs(int model, int n, const double* x, const double* p, double* out);
+void orc_jvp(int model, int n, const double* x, const double* p, const double* v, double* out);
+void orc_jvp_f32(int model, int n, const double* x, const double* p, const float* v, float* out);
+
+/* one base RK step; err_vec (may be NULL) receives h*sum(e_j k_j); returns
+   the number of RHS evaluations.  stage_x (may be NULL) receives the stage
+   states, stride n, as needed by the coupled tangent step. */
+int orc_rk_step(int model, int integ, int n, const double* x, const double* p, double h,
+                double* x_new, double* err_vec, double* stage_x);
+/* coupled tangent step (Jacobian at the stage states returned by orc_rk_step) */
+void orc_tangent_step(int model, int integ, int n, const double* stage_x, const double* p,
+                      double h, double* v);
+/* reference frozen-Jacobian tangent step (dverkStepVarMat) */
+void orc_frozen_tangent_step(int model, int n, const double* x_frozen, const double* p,
+                             double h, double* v);
+
+/* orthogonalisation kernels */
+void orc_mgs(int n, int k, int tree, double* V /* k rows of n */, double* log_norms);
+void orc_cgs_ref(int n, int k, double* V /* k rows of n */);
+void orc_normalize_ref(int n, int k, double* V, double eps);
+void orc_qr_ref(int n, double* Q /* n x n row-major, columns are vectors */, double* Rdiag);
+
+/* controller (reference dopriStep constants) */
+double orc_err_norm(int n, int tree, const double* err_vec, const double* x, const double* x_new,
+                    double rtol, double atol);
+
+/* point parameters / initial state */
+void orc_point_params(const orc_desc* d, int64_t idx, const double* params_in, double* p);
+void orc_point_init(const orc_desc* d, int64_t idx, const double* p, const double* inits_in, double* x);
+
+/* per-point and whole-sweep drivers */
+void orc_point(const orc_desc* d, int64_t idx, const double* params_in, const double* inits_in,
+               double* lam, uint32_t* status, uint32_t* n_acc, uint32_t* n_rej);
+/* idx = first + i for i in [0, count); outputs are indexed by i */
+void orc_sweep(const orc_desc* d, int64_t first, int64_t count, const int64_t* idx_list,
+               const double* params_in, const double* inits_in, double* lam,
+               uint32_t* status, uint32_t* n_acc, uint32_t* n_rej, int n_threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

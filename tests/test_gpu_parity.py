@@ -1,0 +1,81 @@
+"""GPU sweep vs the CPU FP64 oracle on the same seeded grids.
+
+Bar: |dlambda_i| <= 1e-4 for the FP64 path (the FP64 path is also expected to
+match bit for bit: same operations in the same order, no contraction); the
+mixed form is compared bit for bit with the oracle's mixed form and its error
+against the FP64 oracle is reported.  Status words and step counters must
+match exactly.
+"""
+import numpy as np
+import pytest
+import torch
+
+from chaos_oneapi_amd import lyap as L
+from chaos_oneapi_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-4
+
+
+def run_both(ctx, spec, count=None, first=0):
+    desc = L.finalize(L.make_desc(spec))
+    n = desc.n1 * desc.n2 if count is None else count
+    print(f"case model={desc.model} method={desc.method} integ={desc.integrator} "
+          f"adapt={desc.adaptive} prec={desc.precision} n={n}", flush=True)
+    lam, st, na, nr = ctx.run(desc, first=first, count=n)
+    torch.cuda.synchronize()
+    g = (lam.cpu().numpy(), st.cpu().numpy().astype(np.uint32), na.cpu().numpy().astype(np.uint32),
+         nr.cpu().numpy().astype(np.uint32))
+    od = O.make_desc(desc.to_dict())
+    o = O.sweep(od, first=first, count=n)
+    return g, o
+
+
+def assert_match(g, o, tol, exact):
+    lam_g, st_g, na_g, nr_g = g
+    lam_o, st_o, na_o, nr_o = o
+    np.testing.assert_array_equal(st_g, st_o)
+    np.testing.assert_array_equal(na_g, na_o)
+    np.testing.assert_array_equal(nr_g, nr_o)
+    ok = st_o == 0
+    diff = np.abs(lam_g[ok] - lam_o[ok])
+    print(f"max |dlambda| = {diff.max() if diff.size else 0.0:.3e}, points ok {ok.sum()}/{len(ok)}",
+          flush=True)
+    assert np.all(np.isnan(lam_g[~ok]))
+    assert diff.size == 0 or diff.max() <= tol
+    if exact:
+        np.testing.assert_array_equal(lam_g[ok], lam_o[ok])
+
+
+@pytest.mark.parametrize("integ", [L.INT_DVERK, L.INT_DOPRI5])
+@pytest.mark.parametrize("adaptive", [0, 1])
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
+@pytest.mark.parametrize("model", ["lorenz", "rossler", "shimizu"])
+def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
+    if model == "lorenz":
+        spec = W.lorenz_fixed(n=6, precision=prec, t_transient=5.0, t_average=20.0)
+    elif model == "rossler":
+        spec = W.rossler_dverk(n=6, precision=prec, t_transient=10.0, t_average=40.0)
+    else:
+        spec = W.lorenz_fixed(n=6, precision=prec, t_transient=5.0, t_average=20.0)
+        spec.update(model=L.MODEL_SHIMIZU, n_params=2, base_params=[0.0, 0.0],
+                    init_state=[0.1, 0.0, 0.0], init_scale=1e-3)
+        spec.update(W.grid_fields(0, 0.2, 2.0, 6, 1, 0.0, 1.2, 6))
+    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9)
+    g, o = run_both(gpu_ctx, spec)
+    assert_match(g, o, TOL_FP64, exact=True)
+
+
+@pytest.mark.parametrize("method", [L.METHOD_VAR_REF, L.METHOD_QR_REF, L.METHOD_FD_REF])
+@pytest.mark.parametrize("model", ["lorenz", "shimizu"])
+def test_reference_modes_match_oracle(gpu_ctx, model, method):
+    spec = W.lorenz_fixed(n=4, t_transient=2.0, t_average=5.0)
+    spec.update(method=method, integrator=L.INT_DVERK, eps=1e-7, t_slave=1.0, t_renorm=0.05)
+    if model == "shimizu":
+        spec.update(model=L.MODEL_SHIMIZU, n_params=2, base_params=[0.0, 0.0],
+                    init_state=[1e-4, 0.0, 0.0], init_kind=L.INIT_NONE)
+        spec.update(W.grid_fields(0, 0.2, 2.0, 4, 1, 0.0, 1.2, 4))
+    g, o = run_both(gpu_ctx, spec)
+    assert_match(g, o, TOL_FP64, exact=True)
