@@ -172,8 +172,8 @@ int lyap_desc_finalize(lyap_sweep_desc* d) {
     if (d->init_param_offset >= d->n_params)
         return fail(LYAP_ERR_INVALID, "init_param_offset %d outside the parameters",
                     d->init_param_offset);
-    if (d->max_steps <= 0) d->max_steps = 100000000LL;
-    if (!(d->escape_radius > 0.0)) d->escape_radius = 1e8;
+    if (d->max_steps <= 0) d->max_steps = 20000000LL;
+    if (!(d->escape_radius > 0.0)) d->escape_radius = 1e6;
     switch (d->method) {
         case LYAP_METHOD_VAR: {
             if (!(d->t_renorm > 0.0))

@@ -28,6 +28,19 @@ namespace lyap {
 LYAP_HD uint64_t bits_of(double x) { return __builtin_bit_cast(uint64_t, x); }
 LYAP_HD double double_of(uint64_t u) { return __builtin_bit_cast(double, u); }
 
+// An FP64 coefficient materialised where it is used.  gfx9 VALU encodings
+// take no 64-bit literal, so FP64 constants live in SGPR pairs; left alone,
+// the compiler hoists all of them out of the step loop, overflows the SGPR
+// budget and spills them into VGPR lanes (a v_readlane per use).  Passing the
+// value through an empty asm keeps it a fresh s_mov pair at its use site.
+LYAP_HD double sconst(double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__ volatile("" : "+s"(v));
+#endif
+    return v;
+}
+LYAP_HD float sconst(float v) { return v; }
+
 constexpr double kLn2Hi = 6.93147180369123816490e-01;
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 constexpr double kInf = __builtin_huge_val();
@@ -57,22 +70,22 @@ LYAP_HD double det_log(double x) {
     const double f = m - 1.0;
     const double s = f / (2.0 + f);
     const double z = s * s;
-    double r = 2.0 / 25.0;
-    r = fma(r, z, 2.0 / 23.0);
-    r = fma(r, z, 2.0 / 21.0);
-    r = fma(r, z, 2.0 / 19.0);
-    r = fma(r, z, 2.0 / 17.0);
-    r = fma(r, z, 2.0 / 15.0);
-    r = fma(r, z, 2.0 / 13.0);
-    r = fma(r, z, 2.0 / 11.0);
-    r = fma(r, z, 2.0 / 9.0);
-    r = fma(r, z, 2.0 / 7.0);
-    r = fma(r, z, 2.0 / 5.0);
-    r = fma(r, z, 2.0 / 3.0);
+    double r = sconst(2.0 / 25.0);
+    r = fma(r, z, sconst(2.0 / 23.0));
+    r = fma(r, z, sconst(2.0 / 21.0));
+    r = fma(r, z, sconst(2.0 / 19.0));
+    r = fma(r, z, sconst(2.0 / 17.0));
+    r = fma(r, z, sconst(2.0 / 15.0));
+    r = fma(r, z, sconst(2.0 / 13.0));
+    r = fma(r, z, sconst(2.0 / 11.0));
+    r = fma(r, z, sconst(2.0 / 9.0));
+    r = fma(r, z, sconst(2.0 / 7.0));
+    r = fma(r, z, sconst(2.0 / 5.0));
+    r = fma(r, z, sconst(2.0 / 3.0));
     r = r * z;
     const double lg = f - s * (f - r);
     const double dk = (double)k;
-    return fma(dk, kLn2Hi, fma(dk, kLn2Lo, lg));
+    return fma(dk, sconst(kLn2Hi), fma(dk, sconst(kLn2Lo), lg));
 }
 
 // exp: nearest-integer reduction by ln2, degree-13 Taylor polynomial
@@ -82,21 +95,21 @@ LYAP_HD double det_exp(double x) {
     if (x < -745.1332191019412) return 0.0;
     double kd = x * 1.4426950408889634;
     kd = (kd >= 0.0) ? floor(kd + 0.5) : ceil(kd - 0.5);
-    const double r = fma(-kd, kLn2Lo, fma(-kd, kLn2Hi, x));
-    double p = 1.0 / 6227020800.0;
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
+    const double r = fma(-kd, sconst(kLn2Lo), fma(-kd, sconst(kLn2Hi), x));
+    double p = sconst(1.0 / 6227020800.0);
+    p = fma(p, r, sconst(1.0 / 479001600.0));
+    p = fma(p, r, sconst(1.0 / 39916800.0));
+    p = fma(p, r, sconst(1.0 / 3628800.0));
+    p = fma(p, r, sconst(1.0 / 362880.0));
+    p = fma(p, r, sconst(1.0 / 40320.0));
+    p = fma(p, r, sconst(1.0 / 5040.0));
+    p = fma(p, r, sconst(1.0 / 720.0));
+    p = fma(p, r, sconst(1.0 / 120.0));
+    p = fma(p, r, sconst(1.0 / 24.0));
+    p = fma(p, r, sconst(1.0 / 6.0));
+    p = fma(p, r, sconst(0.5));
+    p = fma(p, r, sconst(1.0));
+    p = fma(p, r, sconst(1.0));
     int k = (int)kd;
     if (k > 1023) {
         p = p * double_of((uint64_t)(1023 + 1023) << 52);
@@ -197,61 +210,84 @@ struct Tab<LYAP_INT_DOPRI5> {
     static constexpr bool for_solution(int s) { return s != 6; }
 };
 
-// Stage sums.  Coefficient lookups are constant after unrolling, so the
-// zero tests and the first-term choice resolve at compile time.
+
+
+// One tableau row (stage s) of coefficients as opaque SGPR values, shared by
+// every component and vector that uses the row.
+template <class TB, class T, int LD>
+struct RowA {
+    T c[LD];
+    LYAP_D explicit RowA(int s) {
+#pragma unroll
+        for (int j = 0; j < LD; ++j) c[j] = (j < s && TB::a(s, j) != 0.0) ? sconst((T)TB::a(s, j)) : T(0);
+    }
+};
+template <class TB, class T, int LD>
+struct RowB {
+    T c[LD];
+    LYAP_D RowB() {
+#pragma unroll
+        for (int j = 0; j < LD; ++j) c[j] = TB::b(j) != 0.0 ? sconst((T)TB::b(j)) : T(0);
+    }
+};
+template <class TB, class T, int LD>
+struct RowE {
+    T c[LD];
+    LYAP_D RowE() {
+#pragma unroll
+        for (int j = 0; j < LD; ++j) c[j] = TB::e(j) != 0.0 ? sconst((T)TB::e(j)) : T(0);
+    }
+};
+
+// Stage sums: coefficients in increasing stage order, first nonzero term a
+// product, later terms fma(c_j, k_j, acc).  The zero tests use the constexpr
+// tableau, so they resolve at compile time after unrolling.
 template <class TB, class T, int LD, int N>
-LYAP_D T coef_sum_a(int s, const T (&k)[LD][N], int i, bool& any) {
+LYAP_D T coef_sum_a(int s, const RowA<TB, T, LD>& row, const T (&k)[LD][N], int i, bool& any) {
     T acc = T(0);
     any = false;
 #pragma unroll
     for (int j = 0; j < LD; ++j) {
-        if (j >= s) break;
-        const double c = TB::a(s, j);
-        if (c == 0.0) continue;
-        const T cj = (T)c;
+        if (j >= s || TB::a(s, j) == 0.0) continue;
         if (!any) {
-            acc = cj * k[j][i];
+            acc = row.c[j] * k[j][i];
             any = true;
         } else {
-            acc = fma(cj, k[j][i], acc);
+            acc = fma(row.c[j], k[j][i], acc);
         }
     }
     return acc;
 }
 
 template <class TB, class T, int LD, int N>
-LYAP_D T coef_sum_b(const T (&k)[LD][N], int i) {
+LYAP_D T coef_sum_b(const RowB<TB, T, LD>& row, const T (&k)[LD][N], int i) {
     T acc = T(0);
     bool any = false;
 #pragma unroll
     for (int j = 0; j < LD; ++j) {
-        const double c = TB::b(j);
-        if (c == 0.0) continue;
-        const T cj = (T)c;
+        if (TB::b(j) == 0.0) continue;
         if (!any) {
-            acc = cj * k[j][i];
+            acc = row.c[j] * k[j][i];
             any = true;
         } else {
-            acc = fma(cj, k[j][i], acc);
+            acc = fma(row.c[j], k[j][i], acc);
         }
     }
     return acc;
 }
 
 template <class TB, class T, int LD, int N>
-LYAP_D T coef_sum_e(const T (&k)[LD][N], int i) {
+LYAP_D T coef_sum_e(const RowE<TB, T, LD>& row, const T (&k)[LD][N], int i) {
     T acc = T(0);
     bool any = false;
 #pragma unroll
     for (int j = 0; j < LD; ++j) {
-        const double c = TB::e(j);
-        if (c == 0.0) continue;
-        const T cj = (T)c;
+        if (TB::e(j) == 0.0) continue;
         if (!any) {
-            acc = cj * k[j][i];
+            acc = row.c[j] * k[j][i];
             any = true;
         } else {
-            acc = fma(cj, k[j][i], acc);
+            acc = fma(row.c[j], k[j][i], acc);
         }
     }
     return acc;
@@ -275,8 +311,18 @@ struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27
         o[1] = fma(T(1) - x[2], v[0], fma(-p[0], v[1], -x[0] * v[2]));
         o[2] = fma(T(2) * x[0], v[0], -p[1] * v[2]);
     }
+    LYAP_D static double jnorm(const double* x, const double* p) {
+        const double r0 = 1.0;
+        const double r1 = (fabs(1.0 - x[2]) + fabs(p[0])) + fabs(x[0]);
+        const double r2 = 2.0 * fabs(x[0]) + fabs(p[1]);
+        const double m = r0 > r1 ? r0 : r1;
+        return m > r2 ? m : r2;
+    }
 };
 
+// Each model also gives jnorm(x) = max row sum of |J(x)|: adaptive steps are
+// capped at h*jnorm <= 1, which keeps the tangent integration accurate where
+// the base error estimate vanishes (stable fixed points).
 struct Lorenz {  // (sigma, rho, beta)
     static constexpr int ID = LYAP_MODEL_LORENZ, N = 3, P = 3;
     template <class T>
@@ -290,6 +336,13 @@ struct Lorenz {  // (sigma, rho, beta)
         o[0] = p[0] * (v[1] - v[0]);
         o[1] = fma(p[1] - x[2], v[0], -fma(x[0], v[2], v[1]));
         o[2] = fma(x[1], v[0], fma(x[0], v[1], -p[2] * v[2]));
+    }
+    LYAP_D static double jnorm(const double* x, const double* p) {
+        const double r0 = 2.0 * fabs(p[0]);
+        const double r1 = (fabs(p[1] - x[2]) + 1.0) + fabs(x[0]);
+        const double r2 = (fabs(x[1]) + fabs(x[0])) + fabs(p[2]);
+        const double m = r0 > r1 ? r0 : r1;
+        return m > r2 ? m : r2;
     }
 };
 
@@ -307,6 +360,13 @@ struct Rossler {  // (a, c, b)
         o[1] = fma(p[0], v[1], v[0]);
         o[2] = fma(x[2], v[0], (x[0] - p[1]) * v[2]);
     }
+    LYAP_D static double jnorm(const double* x, const double* p) {
+        const double r0 = 2.0;
+        const double r1 = 1.0 + fabs(p[0]);
+        const double r2 = fabs(x[2]) + fabs(x[0] - p[1]);
+        const double m = r0 > r1 ? r0 : r1;
+        return m > r2 ? m : r2;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -315,20 +375,28 @@ struct Rossler {  // (a, c, b)
 // ---------------------------------------------------------------------------
 struct Ctrl {
     double fac_old;
+    double log_fac_old;  // det_log(fac_old), carried from the step that set it
     bool rejected;
 };
 
+// lerr = det_log(err).  det_pow(x, y) = det_exp(y * det_log(x)), so carrying
+// the log of fac_old from the accepted step that produced it (it is err, or
+// the 1e-4 floor with log_floor = det_log(1e-4)) gives the same bits as
+// recomputing det_pow(fac_old, beta).
 template <int INTEG>
-LYAP_D bool ctrl_update(Ctrl& c, double err, double h_try, double h_max, double& h_next) {
+LYAP_D bool ctrl_update(Ctrl& c, double err, double lerr, double log_floor, double h_try,
+                        double h_max, double& h_next) {
     constexpr double kSafe = 0.9, kFacc1 = 1.0 / 0.2, kFacc2 = 1.0 / 10.0, kBeta = 0.04;
-    const double fac11 = det_pow(err, Tab<INTEG>::expo);
-    double fac = fac11 / det_pow(c.fac_old, kBeta);
+    const double fac11 = det_exp(Tab<INTEG>::expo * lerr);
+    double fac = fac11 / det_exp(kBeta * c.log_fac_old);
     const double q = fac / kSafe;
     fac = kFacc1 < q ? kFacc1 : q;
     fac = kFacc2 > fac ? kFacc2 : fac;
     double hn = h_try / fac;
     if (err <= 1.0) {
-        c.fac_old = err > 1e-4 ? err : 1e-4;
+        const bool floor = !(err > 1e-4);
+        c.fac_old = floor ? 1e-4 : err;
+        c.log_fac_old = floor ? log_floor : lerr;
         if (fabs(hn) > h_max) hn = h_max;
         if (c.rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
         c.rejected = false;

@@ -39,10 +39,11 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
             continue;
         }
         double arg[N];
+        const RowA<TB, double, S> row(s);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             bool any;
-            const double acc = coef_sum_a<TB>(s, k, i, any);
+            const double acc = coef_sum_a(s, row, k, i, any);
             arg[i] = any ? fma(h, acc, x[i]) : x[i];
             X[s][i] = (TX)arg[i];
         }
@@ -53,11 +54,15 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
             M::template rhs<double>(arg, p, k[s]);
         }
     }
+    {
+        const RowB<TB, double, S> rb;
 #pragma unroll
-    for (int i = 0; i < N; ++i) xn[i] = fma(h, coef_sum_b<TB>(k, i), x[i]);
+        for (int i = 0; i < N; ++i) xn[i] = fma(h, coef_sum_b(rb, k, i), x[i]);
+    }
     if (ERR) {
+        const RowE<TB, double, S> re;
 #pragma unroll
-        for (int i = 0; i < N; ++i) ev[i] = h * coef_sum_e<TB>(k, i);
+        for (int i = 0; i < N; ++i) ev[i] = h * coef_sum_e(re, k, i);
     }
     if (FSAL) {
 #pragma unroll
@@ -87,23 +92,25 @@ LYAP_D void tangent_step(const TX (&X)[Tab<INTEG>::S][M::N], const double* p, co
             continue;
         }
         T arg[N], xs[N];
+        const RowA<TB, T, S> row(s);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             bool any;
-            const T acc = coef_sum_a<TB>(s, kv, i, any);
+            const T acc = coef_sum_a(s, row, kv, i, any);
             arg[i] = any ? fma(hT, acc, vt[i]) : vt[i];
             xs[i] = (T)X[s][i];
         }
         if constexpr (MIXED) M::template jvp<float>(xs, pf, arg, kv[s]);
         else M::template jvp<double>(xs, p, arg, kv[s]);
     }
+    const RowB<TB, T, S> rb;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         if constexpr (MIXED) {
-            const float inc = hT * coef_sum_b<TB>(kv, i);
+            const float inc = hT * coef_sum_b(rb, kv, i);
             v[i] = v[i] + (double)inc;
         } else {
-            v[i] = fma(h, coef_sum_b<TB>(kv, i), v[i]);
+            v[i] = fma(h, coef_sum_b(rb, kv, i), v[i]);
         }
     }
 }
@@ -176,29 +183,41 @@ template <class M, int INTEG, bool ADAPT, bool MIXED>
 __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const SweepIO io) {
     constexpr int N = M::N, P = M::P, K = M::N, S = Tab<INTEG>::S;
     using TX = typename std::conditional<MIXED, float, double>::type;
-    const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
+    // scalars the step loop reads; everything else of the descriptor is
+    // re-read through an opaque pointer when a lane takes a new point, so the
+    // compiler does not keep (and spill) the whole descriptor in SGPRs
     const int64_t n_tr = d.n_renorm_transient;
+    const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
+    const int64_t max_steps = d.max_steps, spr = d.steps_per_renorm;
+    const double rtol = d.rtol, atol = d.atol, h_min = d.h_min, h_max = d.h_max;
+    const double t_renorm = d.t_renorm, esc = d.escape_radius, h_fixed = d.h;
+    const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
+                               : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
+    const int n_exps = d.n_exps;
+    const double log_floor = det_log(1e-4);
 
     double x[N], V[K][N], sum[K], comp[K], p[P], k1[N];
     float pf[P];
     double t = 0.0, h = 0.0;
-    int64_t m = 0, step_in = 0, item = -1, idx = 0;
+    int64_t m = 0, step_in = 0, item = -1;
     uint32_t nacc = 0, nrej = 0;
-    Ctrl ctrl{1e-4, false};
+    Ctrl ctrl{1e-4, log_floor, false};
     bool active = false, exhausted = false;
 
     for (;;) {
         const int64_t got = grab_items(!active && !exhausted, io.counter);
         if (!active && !exhausted) {
             if (got < io.count) {
+                const lyap_sweep_desc* dp = &d;
+                __asm__ volatile("" : "+s"(dp));
                 item = got;
-                idx = io.idx_list ? io.idx_list[item] : io.first + item;
-                point_params<P>(d, idx, io.params, p);
+                const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
+                point_params<P>(*dp, idx, io.params, p);
 #pragma unroll
                 for (int j = 0; j < P; ++j) pf[j] = (float)p[j];
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    x[i] = io.inits ? io.inits[idx * N + i] : init_component(d, idx, i, p);
+                    x[i] = io.inits ? io.inits[idx * N + i] : init_component(*dp, idx, i, p);
 #pragma unroll
                 for (int r = 0; r < K; ++r) {
                     sum[r] = 0.0;
@@ -207,11 +226,12 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
                     for (int i = 0; i < N; ++i) V[r][i] = (r == i) ? 1.0 : 0.0;
                 }
                 t = 0.0;
-                h = d.h;
+                h = dp->h;
                 m = 0;
                 step_in = 0;
                 nacc = nrej = 0;
                 ctrl.fac_old = 1e-4;
+                ctrl.log_fac_old = log_floor;
                 ctrl.rejected = false;
                 if (INTEG == LYAP_INT_DOPRI5 && ADAPT) M::template rhs<double>(x, p, k1);
                 active = true;
@@ -227,18 +247,22 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
         double xn[N], ln[K];
         TX X[S][N];
         if constexpr (ADAPT) {
-            if ((int64_t)nacc + (int64_t)nrej >= d.max_steps) {
+            if ((int64_t)nacc + (int64_t)nrej >= max_steps) {
                 st = LYAP_PT_MAXSTEPS;
                 finished = true;
             } else {
-                const double t_next = (double)(m + 1) * d.t_renorm;
+                const double t_next = (double)(m + 1) * t_renorm;
                 double h_try = h;
                 bool clipped = false;
-                if (t + h_try >= t_next) {
+                const double jn = M::jnorm(x, p);
+                if (h_try * jn > 1.0) h_try = 1.0 / jn;
+                // land on the boundary when it is within 1/16 of the step, so no
+                // round-off sliver is left before it
+                if (t_next - t <= 1.0625 * h_try) {
                     h_try = t_next - t;
                     clipped = true;
                 }
-                if (h_try < d.h_min) {
+                if (h_try < h_min) {
                     st = LYAP_PT_STEP_UNDERFLOW;
                     finished = true;
                 } else {
@@ -246,13 +270,15 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
 #pragma unroll
                     for (int i = 0; i < N; ++i) k1_next[i] = k1[i];
                     base_step<M, INTEG, true, TX>(x, p, h_try, xn, ev, X, k1_next);
-                    const double err = err_norm_seq<N>(ev, x, xn, d.rtol, d.atol);
+                    const double err = err_norm_seq<N>(ev, x, xn, rtol, atol);
                     if (!(err < kInf)) {
                         st = LYAP_PT_NONFINITE;
                         finished = true;
                     } else {
                         double h_next;
-                        const bool acc = ctrl_update<INTEG>(ctrl, err, h_try, d.h_max, h_next);
+                        const double lerr = det_log(err);
+                        const bool acc =
+                            ctrl_update<INTEG>(ctrl, err, lerr, log_floor, h_try, h_max, h_next);
                         h = h_next;
                         if (!acc) {
                             ++nrej;
@@ -267,7 +293,7 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
                             }
                             t = clipped ? t_next : t + h_try;
                             ++nacc;
-                            if (escaped<N>(x, d.escape_radius)) {
+                            if (escaped<N>(x, esc)) {
                                 st = LYAP_PT_ESCAPED;
                                 finished = true;
                             } else if (clipped) {
@@ -284,13 +310,13 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
                 }
             }
         } else {
-            base_step<M, INTEG, false, TX>(x, p, d.h, xn, nullptr, X, nullptr);
+            base_step<M, INTEG, false, TX>(x, p, h_fixed, xn, nullptr, X, nullptr);
 #pragma unroll
-            for (int r = 0; r < K; ++r) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, d.h, V[r]);
+            for (int r = 0; r < K; ++r) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_fixed, V[r]);
 #pragma unroll
             for (int i = 0; i < N; ++i) x[i] = xn[i];
             ++nacc;
-            if (++step_in == d.steps_per_renorm) {
+            if (++step_in == spr) {
                 step_in = 0;
                 bool bad = false;
 #pragma unroll
@@ -298,7 +324,7 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
                 if (bad) {
                     st = LYAP_PT_NONFINITE;
                     finished = true;
-                } else if (escaped<N>(x, d.escape_radius)) {
+                } else if (escaped<N>(x, esc)) {
                     st = LYAP_PT_ESCAPED;
                     finished = true;
                 } else {
@@ -313,16 +339,12 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
             }
         }
         if (finished) {
-            const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
-                                       : (double)d.n_renorm_average *
-                                             (double)d.steps_per_renorm * d.h;
-            for (int r = 0; r < d.n_exps; ++r) {
+            for (int r = 0; r < n_exps; ++r) {
                 double s = 0.0;
 #pragma unroll
                 for (int q = 0; q < K; ++q)
                     if (q == r) s = sum[q];
-                io.lambda[item * d.n_exps + r] =
-                    (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
+                io.lambda[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
             }
             if (io.status) io.status[item] = st;
             if (io.n_acc) io.n_acc[item] = nacc;
@@ -349,16 +371,18 @@ LYAP_D void frozen_tangent_step(const double* xf, const double* p, double h, dou
             continue;
         }
         double arg[N];
+        const RowA<TB, double, S> row(s);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             bool any;
-            const double acc = coef_sum_a<TB>(s, kv, i, any);
+            const double acc = coef_sum_a(s, row, kv, i, any);
             arg[i] = any ? fma(h, acc, v[i]) : v[i];
         }
         M::template jvp<double>(xf, p, arg, kv[s]);
     }
+    const RowB<TB, double, S> rb;
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = fma(h, coef_sum_b<TB>(kv, i), v[i]);
+    for (int i = 0; i < N; ++i) v[i] = fma(h, coef_sum_b(rb, kv, i), v[i]);
 }
 
 template <class M>

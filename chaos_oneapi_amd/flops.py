@@ -1,0 +1,76 @@
+"""Algorithmic flop count of the sweep (the numerator of the roofline figure).
+
+Convention: an FMA counts 2, every other arithmetic operation 1; negations,
+comparisons, conversions and the step-size controller's log/exp are not
+counted.  Counts follow the operations the kernels perform:
+
+  stage sums     first term a product (1), later terms fma (2), then
+                 fma(h, acc, y) (2)  ->  2*nnz + 1 per stage and component
+  solution       2*nnz(b) + 1 per component
+  error estimate 2*nnz(e) per component (the last multiply by h included),
+                 norm 7 per component + 2
+  model          f and J v costs of the written formulas (below)
+  MGS            per vector j: j*(4N-1) + (2N-1) + N + 2
+
+Per point: n_acc * (base + K * tangent) + n_rej * base + renorms * MGS.
+"""
+from __future__ import annotations
+
+from . import lyap as L
+
+# (f, J v per vector) flop counts of the written formulas
+MODEL_COST = {
+    L.MODEL_LORENZ: lambda n: (8, 12),
+    L.MODEL_ROSSLER: lambda n: (6, 7),
+    L.MODEL_SHIMIZU: lambda n: (7, 10),
+    # f_i = fma(x+ - x--, x-, fma(-d, x, F)): 5;  J v_i: 1 + 1 + 2 + 2 + 1 = 7
+    L.MODEL_L96: lambda n: (5 * n, 7 * n),
+}
+
+# nonzero counts of the tableaux
+DVERK_A_ROWS = [0, 1, 2, 3, 4, 5, 5, 6]  # stage 6 (index 5) feeds the error only
+DVERK_NNZ_B, DVERK_NNZ_E = 6, 7
+DOPRI_A_ROWS = [0, 1, 2, 3, 4, 5, 5]  # stage 7 (index 6) is the FSAL stage
+DOPRI_NNZ_B, DOPRI_NNZ_E = 5, 6
+
+
+def _stages(integ: int, adaptive: bool, for_tangent: bool):
+    rows = DVERK_A_ROWS if integ == L.INT_DVERK else DOPRI_A_ROWS
+    skip = 5 if integ == L.INT_DVERK else 6
+    out = []
+    for s, nnz in enumerate(rows):
+        if s == skip and (for_tangent or not adaptive):
+            continue
+        out.append(nnz)
+    return out
+
+
+def per_step(model: int, n: int, k: int, integ: int, adaptive: bool):
+    """(flops of an accepted step, flops of a rejected step) for one point."""
+    f_cost, jv_cost = MODEL_COST[model](n)
+    nnz_b = DVERK_NNZ_B if integ == L.INT_DVERK else DOPRI_NNZ_B
+    nnz_e = DVERK_NNZ_E if integ == L.INT_DVERK else DOPRI_NNZ_E
+    base_st = _stages(integ, adaptive, for_tangent=False)
+    # Dopri5 with error control reuses stage 7 as the next stage 1 (FSAL)
+    n_f = len(base_st) - (1 if (integ == L.INT_DOPRI5 and adaptive) else 0)
+    base = n_f * f_cost + n * sum(2 * z + 1 for z in base_st if z > 0) + n * (2 * nnz_b + 1)
+    if adaptive:
+        base += n * 2 * nnz_e + 7 * n + 2
+    tan_st = _stages(integ, adaptive, for_tangent=True)
+    tangent = len(tan_st) * jv_cost + n * sum(2 * z + 1 for z in tan_st if z > 0) + n * (2 * nnz_b + 1)
+    return base + k * tangent, base
+
+
+def mgs(n: int, k: int) -> int:
+    return sum(j * (4 * n - 1) + (2 * n - 1) + n + 2 for j in range(k))
+
+
+def sweep_flops(desc, n_acc, n_rej) -> float:
+    """Total algorithmic flops of a sweep from its per-point step counters."""
+    import numpy as np
+
+    acc, rej = per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
+    renorms = desc.n_renorm_transient + desc.n_renorm_average
+    na = np.asarray(n_acc, dtype=np.float64)
+    nr = np.asarray(n_rej, dtype=np.float64)
+    return float(acc * na.sum() + rej * nr.sum() + mgs(desc.dim, desc.dim) * renorms * len(na))

@@ -191,6 +191,42 @@ void orc_jvp(int model, int n, const double* x, const double* p, const double* v
     }
 }
 
+/* max row sum of |J(x)|, used to cap adaptive steps: h*||J||_inf <= 1 keeps the
+   tangent integration accurate where the base error estimate vanishes
+   (stable fixed points) */
+double orc_jac_norm(int model, int n, const double* x, const double* p) {
+    double r0, r1, r2, m;
+    switch (model) {
+    case ORC_MODEL_SHIMIZU:
+        r0 = 1.0;
+        r1 = (fabs(1.0 - x[2]) + fabs(p[0])) + fabs(x[0]);
+        r2 = 2.0 * fabs(x[0]) + fabs(p[1]);
+        break;
+    case ORC_MODEL_LORENZ:
+        r0 = 2.0 * fabs(p[0]);
+        r1 = (fabs(p[1] - x[2]) + 1.0) + fabs(x[0]);
+        r2 = (fabs(x[1]) + fabs(x[0])) + fabs(p[2]);
+        break;
+    case ORC_MODEL_ROSSLER:
+        r0 = 2.0;
+        r1 = 1.0 + fabs(p[0]);
+        r2 = fabs(x[2]) + fabs(x[0] - p[1]);
+        break;
+    case ORC_MODEL_L96:
+        m = 0.0;
+        for (int i = 0; i < n; ++i) {
+            int ip1 = (i + 1) % n, im1 = (i + n - 1) % n, im2 = (i + n - 2) % n;
+            double r = (2.0 * fabs(x[im1]) + fabs(x[ip1] - x[im2])) + fabs(p[1]);
+            m = r > m ? r : m;
+        }
+        return m;
+    default:
+        return NAN;
+    }
+    m = r0 > r1 ? r0 : r1;
+    return m > r2 ? m : r2;
+}
+
 /* the same products evaluated in binary32: state and parameters are rounded
    to float first, then every operation is a float operation */
 void orc_jvp_f32(int model, int n, const double* xd, const double* pd, const float* v, float* out) {
@@ -689,7 +725,11 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if ((int64_t)(n_acc + n_rej) >= d->max_steps) { st = ORC_MAXSTEPS; break; }
             double h_try = h;
             int clipped = 0;
-            if (t + h_try >= t_next) { h_try = t_next - t; clipped = 1; }
+            const double jn = orc_jac_norm(d->model, n, x, p);
+            if (h_try * jn > 1.0) h_try = 1.0 / jn;
+            /* land on the boundary when it is within 1/16 of the step, so no
+               round-off sliver is left before it */
+            if (t_next - t <= 1.0625 * h_try) { h_try = t_next - t; clipped = 1; }
             if (h_try < d->h_min) { st = ORC_STEP_UNDERFLOW; break; }
             orc_rk_step(d->model, integ, n, x, p, h_try, xn, ev, sx);
             double err = orc_err_norm(n, tree, ev, x, xn, d->rtol, d->atol);

@@ -75,6 +75,7 @@ int64_t orc_loop_count(double t0, double t1, double h);
 /* model right-hand side and Jacobian-vector product (FP64 and FP32 forms) */
 void orc_rhs(int model, int n, const double* x, const double* p, double* out);
 void orc_jvp(int model, int n, const double* x, const double* p, const double* v, double* out);
+double orc_jac_norm(int model, int n, const double* x, const double* p);
 void orc_jvp_f32(int model, int n, const double* x, const double* p, const float* v, float* out);
 
 /* one base RK step; err_vec (may be NULL) receives h*sum(e_j k_j); returns

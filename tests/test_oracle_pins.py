@@ -1,0 +1,322 @@
+"""Pins of the CPU oracle (oracle/lyap_oracle.c) against what the mathematics
+and independent libraries fix.  The reference ships no tests, golden vectors
+or fixtures for this path and cannot be built here (it needs a SYCL compiler
+and the dpct headers), so every pin below is a closed form, an invariant, a
+library routine (numpy/scipy) or a published value, chosen so that a dropped
+term, a wrong sign/index or a transposed operand fails at least one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.integrate
+import scipy.linalg
+
+from oracle import oracle as O
+
+rng = np.random.default_rng(7)
+
+
+# ---------------------------------------------------------------- elementary
+def test_log_exp_against_libm():
+    L = O.lib()
+    xs = np.concatenate([np.exp(rng.uniform(-740, 709, 4000)), [1.0, 2.0, 0.5, 1e-300, 5e-324,
+                                                                 1.4142135623730951, 0.7071067811865476]])
+    for x in xs:
+        ref = math.log(x)
+        assert abs(L.orc_log(float(x)) - ref) <= 4.5e-16 * max(1.0, abs(ref))
+    for y in rng.uniform(-700, 700, 4000):
+        ref = math.exp(y)
+        assert abs(L.orc_exp(float(y)) - ref) <= 4.5e-16 * ref
+    assert L.orc_log(0.0) == -math.inf and math.isnan(L.orc_log(-1.0)) and L.orc_log(math.inf) == math.inf
+    assert L.orc_exp(-800.0) == 0.0 and L.orc_exp(800.0) == math.inf and L.orc_pow(0.0, 0.17) == 0.0
+    for x, y in [(0.3, 0.17), (1e-4, 0.04), (7.5, 1.0 / 6.0 - 0.03)]:
+        assert abs(L.orc_pow(x, y) - x ** y) <= 2e-15 * x ** y
+
+
+def test_loop_count_matches_float_loop():
+    # cuda/lyapunov.dp.cpp iterates `for (double t = t0; t < T; t += h)`
+    def brute(t0, t1, h):
+        n, t = 0, t0
+        while t < t1:
+            n += 1
+            t += h
+        return n
+    assert O.lib().orc_loop_count(0.0, 1000.0, 0.01) == 100001
+    assert O.lib().orc_loop_count(0.0, 100.0, 0.01) == 10000
+    for t0, t1, h in [(0, 7.3, 0.01), (1000, 1000 + 37.1, 0.01), (0, 5, 1e-3), (0, 1, 0.1)]:
+        assert O.lib().orc_loop_count(float(t0), float(t1), h) == brute(float(t0), float(t1), h)
+
+
+def test_uniform_generator_statistics_and_determinism():
+    L = O.lib()
+    u = np.array([L.orc_uniform(99, i, j) for i in range(3000) for j in range(3)])
+    assert u.min() >= 0.0 and u.max() < 1.0
+    assert abs(u.mean() - 0.5) < 0.01 and abs(u.var() - 1 / 12) < 0.003
+    assert L.orc_uniform(99, 5, 1) == L.orc_uniform(99, 5, 1) != L.orc_uniform(98, 5, 1)
+    # adjacent streams and points are uncorrelated
+    a = np.array([L.orc_uniform(3, i, 0) for i in range(5000)])
+    b = np.array([L.orc_uniform(3, i, 1) for i in range(5000)])
+    assert abs(np.corrcoef(a, b)[0, 1]) < 0.05
+
+
+def lorenz_desc(**kw):
+    base = dict(model=O.MODEL_LORENZ, dim=3, n_exps=3, n_params=3, method=O.METHOD_VAR,
+                integrator=O.INT_DOPRI5, adaptive=0, precision=O.PREC_FP64, h=0.01, rtol=1e-9,
+                atol=1e-9, h_min=1e-12, h_max=1.0, eps=1e-7, t_transient=50, t_slave=0,
+                t_average=200, t_renorm=1.0, max_steps=10**9, escape_radius=1e8, sweep_idx1=1,
+                sweep_idx2=2, n1=1, n2=1, start1=28.0, delta1=0.0, start2=8 / 3, delta2=0.0,
+                base_params=[10, 28, 8 / 3], init_state=[1, 1, 1], init_kind=O.INIT_UNIFORM,
+                init_param_offset=-1, init_scale=1e-3, seed=1234)
+    base.update(kw)
+    return O.make_desc(base)
+
+
+def test_grid_rule_p1_outer():
+    # tasklyapunov.cpp:45-63: p = start + i*|end-start|/(n-1), first parameter outer
+    n1, n2 = 5, 7
+    d = lorenz_desc(n1=n1, n2=n2, start1=0.0, delta1=200.0 / (n1 - 1), start2=0.5,
+                    delta2=3.5 / (n2 - 1))
+    for idx in range(n1 * n2):
+        p = O.point_params(d, idx)
+        i1, i2 = divmod(idx, n2)
+        assert p[0] == 10.0
+        assert abs(p[1] - np.linspace(0, 200, n1)[i1]) <= 1e-12
+        assert abs(p[2] - np.linspace(0.5, 4, n2)[i2]) <= 1e-12
+
+
+def test_initial_state_rules():
+    d = lorenz_desc(init_kind=O.INIT_UNIFORM, init_scale=1e-3)
+    xs = np.array([O.point_init(d, i) for i in range(2000)])
+    assert np.all(np.abs(xs - 1.0) <= 1e-3) and abs(xs.mean() - 1.0) < 5e-5
+    d = lorenz_desc(init_kind=O.INIT_NORMAL, init_scale=0.1, init_state=[0, 0, 0],
+                    init_param_offset=0)
+    xs = np.array([O.point_init(d, i) for i in range(4000)])
+    assert abs(xs.mean() - 10.0) < 0.01 and abs(xs.std() - 0.1) < 0.005  # F + N(0, 0.1^2)
+    d = lorenz_desc(init_kind=O.INIT_NONE)
+    assert np.array_equal(O.point_init(d, 17), [1.0, 1.0, 1.0])  # tasklyapunov.cpp:67-71
+
+
+# ---------------------------------------------------------------- models
+MODELS = [
+    (O.MODEL_LORENZ, [10.0, 28.0, 8 / 3], 3,
+     lambda x, p: [p[0] * (x[1] - x[0]), x[0] * (p[1] - x[2]) - x[1], x[0] * x[1] - p[2] * x[2]]),
+    (O.MODEL_ROSSLER, [0.2, 5.7, 0.2], 3,
+     lambda x, p: [-x[1] - x[2], x[0] + p[0] * x[1], p[2] + x[2] * (x[0] - p[1])]),
+    # cuda/models/flows/shimizu.dp.cpp:15-19
+    (O.MODEL_SHIMIZU, [0.75, 0.45], 3,
+     lambda x, p: [x[1], -p[0] * x[1] - x[0] * x[2] + x[0], -p[1] * x[2] + x[0] ** 2]),
+    (O.MODEL_L96, [8.0, 1.0], 8,
+     lambda x, p: [(x[(i + 1) % 8] - x[i - 2]) * x[i - 1] - p[1] * x[i] + p[0] for i in range(8)]),
+]
+
+
+@pytest.mark.parametrize("model,p,n,f", MODELS)
+def test_rhs_and_jacobian(model, p, n, f):
+    for _ in range(5):
+        x = rng.normal(size=n) * 3
+        np.testing.assert_allclose(O.rhs(model, x, p), f(x, p), rtol=1e-14, atol=1e-13)
+        v = rng.normal(size=n)
+        d = 1e-6
+        fd = (O.rhs(model, x + d * v, p) - O.rhs(model, x - d * v, p)) / (2 * d)
+        np.testing.assert_allclose(O.jvp(model, x, p, v), fd, rtol=1e-7, atol=1e-7)
+
+
+# ---------------------------------------------------------------- RK steps
+def exact_flow(model, p, x0, h):
+    sol = scipy.integrate.solve_ivp(lambda t, y: O.rhs(model, y, p), (0, h), x0, method="DOP853",
+                                    rtol=1e-14, atol=1e-16)
+    return sol.y[:, -1]
+
+
+@pytest.mark.parametrize("integ,order", [(O.INT_DVERK, 6), (O.INT_DOPRI5, 5)])
+@pytest.mark.parametrize("model,p,n,f", MODELS[:3])
+def test_local_error_order(integ, order, model, p, n, f):
+    x0 = np.array([1.0, 2.0, 3.0]) if model != O.MODEL_ROSSLER else np.array([1.0, -2.0, 0.5])
+    # step pairs in each model's asymptotic range (above roundoff, below stiffness)
+    hs = {O.MODEL_LORENZ: [0.016, 0.008], O.MODEL_ROSSLER: [0.064, 0.032],
+          O.MODEL_SHIMIZU: [0.128, 0.064]}[model]
+    errs, ests = [], []
+    for h in hs:
+        xn, ev, _ = O.rk_step(model, integ, x0, p, h)
+        errs.append(np.linalg.norm(xn - exact_flow(model, p, x0, h)))
+        ests.append(np.linalg.norm(ev))
+    # local error of the propagated solution ~ h^(order+1), estimate ~ h^order
+    assert math.log2(errs[0] / errs[1]) > order + 1 - 0.7
+    assert math.log2(ests[0] / ests[1]) > order - 0.7
+
+
+def test_dopri5_step_matches_scipy_tableau():
+    from scipy.integrate._ivp import rk as srk
+    for model, p, n, f in MODELS:
+        x0 = rng.normal(size=n)
+        h = 0.013
+        fun = lambda t, y: O.rhs(model, y, p)
+        K = np.empty((srk.RK45.n_stages + 1, n))
+        y_new, f_new = srk.rk_step(fun, 0.0, x0, fun(0, x0), h, srk.RK45.A, srk.RK45.B, srk.RK45.C, K)
+        xn, ev, _ = O.rk_step(model, O.INT_DOPRI5, x0, p, h)
+        np.testing.assert_allclose(xn, y_new, rtol=1e-14, atol=1e-14)
+        np.testing.assert_allclose(ev, -h * (K.T @ srk.RK45.E), rtol=1e-10, atol=1e-17)
+
+
+@pytest.mark.parametrize("integ", [O.INT_DVERK, O.INT_DOPRI5])
+@pytest.mark.parametrize("model,p,n,f", MODELS)
+def test_coupled_tangent_is_derivative_of_step(integ, model, p, n, f):
+    # J at every stage state: the tangent step is D(step)(x) v exactly
+    x0 = rng.normal(size=n) + 1.0
+    v = rng.normal(size=n)
+    h = 0.02
+    _, _, sx = O.rk_step(model, integ, x0, p, h)
+    tv = O.tangent_step(model, integ, sx, p, h, v)
+    d = 1e-6
+    fd = (O.rk_step(model, integ, x0 + d * v, p, h)[0] - O.rk_step(model, integ, x0 - d * v, p, h)[0]) / (2 * d)
+    np.testing.assert_allclose(tv, fd, rtol=1e-7, atol=1e-8)
+
+
+@pytest.mark.parametrize("model,p,n,f", MODELS)
+def test_frozen_tangent_is_dverk_on_linear_system(model, p, n, f):
+    # cuda/tools/dverk.dp.cpp:46-80 applies DVERK to v' = J(x*) v: exp(hJ) v to O(h^7)
+    x0 = rng.normal(size=n)
+    J = np.column_stack([O.jvp(model, x0, p, e) for e in np.eye(n)])
+    v = rng.normal(size=n)
+    for h in (0.01, 0.005):
+        got = O.frozen_tangent_step(model, x0, p, h, v)
+        ref = scipy.linalg.expm(h * J) @ v
+        tol = (h * np.linalg.norm(J, 2)) ** 7 / 5040 * np.linalg.norm(v) * 4 + 1e-14
+        assert np.linalg.norm(got - ref) <= tol
+
+
+# ---------------------------------------------------------------- orthogonalisation
+def test_mgs_matches_numpy_qr():
+    for n, k in [(3, 3), (8, 8), (16, 16), (32, 32), (5, 3)]:
+        A = rng.normal(size=(n, k))
+        Q, R = np.linalg.qr(A)
+        for tree in (False, True):
+            if tree and n & (n - 1):
+                continue
+            V, ln = O.mgs(A.T, tree=tree)
+            np.testing.assert_allclose(ln, np.log(np.abs(np.diag(R))), rtol=0, atol=1e-12)
+            np.testing.assert_allclose(np.abs(V), np.abs(Q.T), atol=1e-11)
+            np.testing.assert_allclose(V @ V.T, np.eye(k), atol=1e-12)
+
+
+def test_reference_cgs_normalize_and_qr():
+    A = rng.normal(size=(3, 3))
+    Q, R = np.linalg.qr(A)
+    # cuda/tools/linalg.dp.cpp:36-51: row i -> R_ii q_i (unnormalised orthogonal rows)
+    V = O.cgs_ref(A.T)
+    np.testing.assert_allclose(np.abs(V), np.abs(np.diag(R))[:, None] * np.abs(Q.T), atol=1e-12)
+    # :20-28: rows scaled to eps
+    W = O.normalize_ref(V, 1e-7)
+    np.testing.assert_allclose(np.linalg.norm(W, axis=1), 1e-7, rtol=1e-14)
+    # :78-128: columns are the vectors; diag(R) up to sign, Q orthonormal
+    Qo, rd = O.qr_ref(A)
+    np.testing.assert_allclose(np.abs(rd), np.abs(np.diag(R)), rtol=1e-12)
+    np.testing.assert_allclose(Qo.T @ Qo, np.eye(3), atol=1e-12)
+
+
+# ---------------------------------------------------------------- spectra
+def lorenz_fixed_point_spectrum(sigma, rho, beta):
+    if rho < 1:
+        J = np.array([[-sigma, sigma, 0], [rho, -1, 0], [0, 0, -beta]])
+    else:
+        c = math.sqrt(beta * (rho - 1))
+        J = np.array([[-sigma, sigma, 0], [1, -1, -c], [c, c, -beta]])
+    return np.sort(np.linalg.eigvals(J).real)[::-1]
+
+
+@pytest.mark.parametrize("rho", [0.5, 10.0])
+@pytest.mark.parametrize("method", [O.METHOD_VAR, O.METHOD_VAR_REF, O.METHOD_QR_REF, O.METHOD_FD_REF])
+def test_stable_fixed_point_spectrum(rho, method):
+    # the reference QR mode has no tangent transient (cuda/lyapunov.dp.cpp:239-254): at a
+    # decoupled fixed point its second column needs ~ln(1e12)/7.9 time units to turn onto
+    # the z direction, an O(1/T) bias of ~28/T, so it is run ten times longer
+    t_avg = 4000 if method == O.METHOD_QR_REF else 400
+    d = lorenz_desc(method=method, integrator=O.INT_DVERK if method != O.METHOD_VAR else O.INT_DOPRI5,
+                    start1=rho, base_params=[10, rho, 8 / 3], t_transient=60, t_slave=20,
+                    t_average=t_avg, t_renorm=0.1 if method != O.METHOD_VAR else 1.0,
+                    init_state=[1, 1, 1] if rho < 1 else [7.9, 7.9, 8.8], eps=1e-8)
+    lam, st, _, _ = O.sweep(d, count=1)
+    assert st[0] == O.OK
+    np.testing.assert_allclose(lam[0], lorenz_fixed_point_spectrum(10, rho, 8 / 3), atol=0.03)
+
+
+@pytest.mark.parametrize("integ,adaptive", [(O.INT_DOPRI5, 0), (O.INT_DVERK, 0), (O.INT_DOPRI5, 1),
+                                             (O.INT_DVERK, 1)])
+def test_lorenz_chaotic_spectrum(integ, adaptive):
+    d = lorenz_desc(integrator=integ, adaptive=adaptive, t_transient=100, t_average=1000)
+    lam, st, na, nr = O.sweep(d, count=1)
+    assert st[0] == O.OK
+    # trace invariant: sum(lambda) = <tr J> = -(sigma + 1 + beta)
+    assert abs(lam[0].sum() + (11 + 8 / 3)) < 1e-6
+    assert abs(lam[0][1]) < 2e-3  # the flow direction
+    assert abs(lam[0][0] - 0.9056) < 0.03  # Sprott's published value, finite-time noise allowed
+    if adaptive:
+        assert na[0] > 0
+
+
+def test_rossler_and_l96_spectra():
+    d = lorenz_desc(model=O.MODEL_ROSSLER, base_params=[0.2, 5.7, 0.2], sweep_idx1=0, sweep_idx2=1,
+                    start1=0.2, start2=5.7, integrator=O.INT_DVERK, adaptive=1, rtol=1e-10, atol=1e-10,
+                    t_transient=200, t_average=2000, t_renorm=0.5, init_state=[0, 0, 0], init_scale=1.0)
+    lam, st, _, _ = O.sweep(d, count=1)
+    assert st[0] == O.OK
+    assert abs(lam[0][0] - 0.0714) < 0.01 and abs(lam[0][1]) < 2e-3 and abs(lam[0][2] + 5.3943) < 0.05
+    n = 8
+    d = lorenz_desc(model=O.MODEL_L96, dim=n, n_exps=n, n_params=2, base_params=[8.0, 1.0],
+                    sweep_idx1=0, sweep_idx2=1, start1=8.0, start2=1.0, integrator=O.INT_DOPRI5,
+                    adaptive=1, t_transient=20, t_average=100, t_renorm=0.2, init_state=[0] * n,
+                    init_kind=O.INIT_NORMAL, init_param_offset=0, init_scale=0.1)
+    lam, st, _, _ = O.sweep(d, count=1)
+    assert st[0] == O.OK
+    assert abs(lam[0].sum() + n * 1.0) < 1e-6  # tr J = -N d
+    assert lam[0][0] > 0.5 and np.all(np.diff(lam[0]) <= 1e-2)
+
+
+def test_mixed_form_close_to_fp64():
+    d = lorenz_desc(t_transient=50, t_average=300)
+    lam64, _, _, _ = O.sweep(d, count=1)
+    d.precision = O.PREC_MIXED
+    lam32, _, _, _ = O.sweep(d, count=1)
+    assert abs(lam32[0][0] - lam64[0][0]) < 1e-5
+
+
+def test_adaptive_controller_rejects_and_meets_tolerance():
+    # a deliberately huge first step must be rejected; tightening rtol moves
+    # lambda_1,2 toward the small fixed-step value; at this stable focus the
+    # step is held by the h*||J||_inf <= 1 cap, which bounds the lambda_3 error
+    fp = dict(start1=10.0, base_params=[10, 10, 8 / 3], init_state=[7.9, 7.9, 8.8],
+              t_transient=10, t_average=50)
+    _, st, na, nr = O.sweep(lorenz_desc(adaptive=1, h=0.5, **fp), count=1)
+    assert st[0] == O.OK and nr[0] >= 1
+    ref = O.sweep(lorenz_desc(h=0.001, **fp), count=1)[0][0]
+    errs = []
+    for tol in (1e-4, 1e-9):
+        lam = O.sweep(lorenz_desc(adaptive=1, rtol=tol, atol=tol, **fp), count=1)[0][0]
+        errs.append(np.max(np.abs(lam[:2] - ref[:2])))
+        assert abs(lam[2] - ref[2]) < 2e-3
+    assert errs[1] <= errs[0] and errs[1] < 2e-5
+
+
+def test_step_cap_bounds_steps_by_jacobian_norm():
+    # without the cap the base error estimate vanishes at the fixed point and
+    # steps grow to h_max; with it h*||J|| <= 1 (mean step ~ 1/||J||)
+    d = lorenz_desc(adaptive=1, start1=10.0, base_params=[10, 10, 8 / 3],
+                    init_state=[7.9, 7.9, 8.8], t_transient=10, t_average=50)
+    _, st, na, _ = O.sweep(d, count=1)
+    c = math.sqrt(8 / 3 * 9)
+    jn = max(20.0, 1 + 1 + c, c + c + 8 / 3)
+    assert st[0] == O.OK and na[0] >= 60 * jn * 0.9
+
+
+def test_status_words():
+    # escaping orbit (Rossler with a large a), step underflow, step budget
+    d = lorenz_desc(model=O.MODEL_ROSSLER, base_params=[0.45, 4.0, 0.2], sweep_idx1=0, sweep_idx2=1,
+                    start1=0.45, start2=4.0, integrator=O.INT_DVERK, adaptive=1, rtol=1e-10,
+                    atol=1e-10, t_renorm=0.5, init_state=[5, 5, 5], init_scale=0.0, escape_radius=1e3)
+    lam, st, _, _ = O.sweep(d, count=1)
+    assert st[0] in (O.ESCAPED, O.NONFINITE) and np.all(np.isnan(lam[0]))
+    d = lorenz_desc(adaptive=1, rtol=1e-30, atol=1e-30, h_min=1e-6)
+    assert O.sweep(d, count=1)[1][0] == O.STEP_UNDERFLOW
+    d = lorenz_desc(adaptive=1, max_steps=100)
+    assert O.sweep(d, count=1)[1][0] == O.MAXSTEPS
