@@ -166,14 +166,40 @@ void escape_into(std::string& out, const std::string& s) {
 
 std::string format_double(double v) {
     if (std::isnan(v) || std::isinf(v)) return "null";  // JSON has no NaN/Inf
+    if (v == 0.0) return std::signbit(v) ? "-0.0" : "0.0";
+    // shortest round-trip significand digits and decimal exponent
     char buf[40];
-    for (int prec = 1; prec <= 17; ++prec) {
-        std::snprintf(buf, sizeof(buf), "%.*g", prec, v);
+    int prec = 1;
+    for (; prec <= 17; ++prec) {
+        std::snprintf(buf, sizeof(buf), "%.*e", prec - 1, v);
         if (std::strtod(buf, nullptr) == v) break;
     }
-    std::string s(buf);
-    if (s.find_first_of(".eE") == std::string::npos) s += ".0";
-    return s;
+    std::string sci(buf);
+    const bool neg = sci[0] == '-';
+    const size_t epos = sci.find('e');
+    std::string digits;
+    for (size_t i = neg ? 1 : 0; i < epos; ++i)
+        if (sci[i] != '.') digits += sci[i];
+    while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+    const int e10 = std::atoi(sci.c_str() + epos + 1);
+    const int k = (int)digits.size();
+    const int n = e10 + 1;  // value = 0.d1..dk * 10^n
+    // layout of nlohmann/json's dtoa format_buffer (min_exp -4, max_exp 15)
+    std::string out = neg ? "-" : "";
+    if (k <= n && n <= 15) {
+        out += digits + std::string((size_t)(n - k), '0') + ".0";
+    } else if (0 < n && n <= 15) {
+        out += digits.substr(0, (size_t)n) + "." + digits.substr((size_t)n);
+    } else if (-4 < n && n <= 0) {
+        out += "0." + std::string((size_t)(-n), '0') + digits;
+    } else {
+        out += digits.substr(0, 1);
+        if (k > 1) out += "." + digits.substr(1);
+        char eb[16];
+        std::snprintf(eb, sizeof(eb), "e%c%02d", n - 1 < 0 ? '-' : '+', std::abs(n - 1));
+        out += eb;
+    }
+    return out;
 }
 
 Json::Json(const std::vector<double>& v) : type_(Type::Array) {

@@ -26,6 +26,20 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+}  // namespace
+
+int lyap::set_error(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+namespace {
+
 int hip_fail(hipError_t e, const char* what) {
     return fail(LYAP_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
@@ -63,6 +77,24 @@ void fill_info(const ModelRow& r, lyap_model_info* out) {
     out->equations = r.equations;
 }
 
+int64_t gcd64(int64_t a, int64_t b) {
+    while (b) {
+        const int64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// a stride near count/phi that is coprime to count
+int64_t spread_stride(int64_t count) {
+    if (count <= 2) return 1;
+    int64_t s = (int64_t)((double)count * 0.6180339887498949);
+    if (s < 1) s = 1;
+    while (gcd64(s, count) != 1) ++s;
+    return s;
+}
+
 bool lane_model(int32_t model) {
     return model == LYAP_MODEL_SHIMIZU || model == LYAP_MODEL_LORENZ ||
            model == LYAP_MODEL_ROSSLER;
@@ -73,6 +105,8 @@ bool lane_model(int32_t model) {
 struct lyap_ctx {
     int device = 0;
     int num_cus = 0;
+    int l96_waves[3] = {0, 0, 0};  // waves per point for N = 8, 16, 32 (0: default)
+    int lane_max_blocks = 0;
     unsigned long long* counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -238,14 +272,17 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;
-    lyap::SweepIO io{first,  count,    d_idx_list, d_params, d_inits,
-                     d_lambda, d_status, d_n_acc,   d_n_rej,  c->counter};
+    lyap::SweepIO io{first,    count,    d_idx_list, d_params, d_inits,  d_lambda,
+                     d_status, d_n_acc,  d_n_rej,    c->counter, spread_stride(count)};
     if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     if (lane_model(fd.model)) {
-        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, 0);
+        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks);
+    } else if (fd.model == LYAP_MODEL_L96) {
+        if (fd.method != LYAP_METHOD_VAR)
+            return fail(LYAP_ERR_UNSUPPORTED, "Lorenz-96 runs the coupled variational method only");
+        e = lyap::launch_l96_sweep(fd, io, s, c->num_cus, c->l96_waves[fd.dim == 8 ? 0 : (fd.dim == 16 ? 1 : 2)]);
     } else {
-        return fail(LYAP_ERR_UNSUPPORTED, "no kernel for model %d (dimension %d) yet", fd.model,
-                    fd.dim);
+        return fail(LYAP_ERR_UNSUPPORTED, "no kernel for model %d (dimension %d)", fd.model, fd.dim);
     }
     if (e != hipSuccess) return hip_fail(e, "sweep launch");
     if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
@@ -273,6 +310,25 @@ int lyap_sweep_run(lyap_ctx* c, const lyap_sweep_desc* d, int64_t first, int64_t
     if (rc != LYAP_SUCCESS) return rc;
     return run_finalized(c, fd, first, count, d_idx_list, d_params, d_inits, d_lambda, d_status,
                          d_n_acc, d_n_rej, stream);
+}
+
+int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
+    if (!c || !key) return fail(LYAP_ERR_INVALID, "lyap_ctx_set_option: NULL argument");
+    const std::string k(key);
+    const int dims[3] = {8, 16, 32};
+    for (int j = 0; j < 3; ++j)
+        if (k == "l96_waves_" + std::to_string(dims[j])) {
+            if (value != 0 && !lyap::l96_supported(dims[j], (int)value))
+                return fail(LYAP_ERR_INVALID, "%s = %lld is not a built layout", key, (long long)value);
+            c->l96_waves[j] = (int)value;
+            return LYAP_SUCCESS;
+        }
+    if (k == "lane_max_blocks") {
+        if (value < 0) return fail(LYAP_ERR_INVALID, "lane_max_blocks must be >= 0");
+        c->lane_max_blocks = (int)value;
+        return LYAP_SUCCESS;
+    }
+    return fail(LYAP_ERR_INVALID, "unknown option '%s'", key);
 }
 
 int lyap_last_kernel_ms(lyap_ctx* c, float* ms) {

@@ -414,9 +414,8 @@ LYAP_D bool ctrl_update(Ctrl& c, double err, double lerr, double log_floor, doub
 // per-point set-up shared by every kernel
 // ---------------------------------------------------------------------------
 // parameters: tasklyapunov.cpp:45-63 grid (p1 outer) unless given explicitly
-template <int P>
-LYAP_D void point_params(const lyap_sweep_desc& d, int64_t idx, const double* params_in,
-                         double* p) {
+template <int P, class D>
+LYAP_D void point_params(const D& d, int64_t idx, const double* params_in, double* p) {
     if (params_in) {
 #pragma unroll
         for (int j = 0; j < P; ++j) p[j] = params_in[idx * P + j];
@@ -432,7 +431,8 @@ LYAP_D void point_params(const lyap_sweep_desc& d, int64_t idx, const double* pa
 
 // initial state: given, or init_state (+ p[offset]) + seeded perturbation
 // (uniform on [-scale, scale], or scale * Irwin-Hall(12) - 6 normal deviate)
-LYAP_D double init_component(const lyap_sweep_desc& d, int64_t idx, int i, const double* p) {
+template <class D>
+LYAP_D double init_component(const D& d, int64_t idx, int i, const double* p) {
     double x0 = d.init_state[i];
     if (d.init_param_offset >= 0) x0 = x0 + p[d.init_param_offset];
     double dx = 0.0;
@@ -454,6 +454,19 @@ LYAP_D void kahan_add(double& s, double& c, double v) {
     s = t;
 }
 
+// The descriptor as it sits in the kernarg segment (the first kernel argument,
+// offset 0), read through an opaque constant-address-space pointer: fields
+// needed only when a lane or block takes a new point are then scalar loads at
+// that moment, instead of values the compiler keeps live (and spills) across
+// the whole step loop.  Taking the address of the by-value parameter instead
+// would copy the descriptor to scratch.
+using KernargDesc = const __attribute__((address_space(4))) lyap_sweep_desc;
+LYAP_D KernargDesc* kernarg_desc() {
+    KernargDesc* p = (KernargDesc*)__builtin_amdgcn_kernarg_segment_ptr();
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
+
 // outputs shared by all sweep kernels
 struct SweepIO {
     int64_t first, count;
@@ -465,6 +478,14 @@ struct SweepIO {
     uint32_t* n_acc;
     uint32_t* n_rej;
     unsigned long long* counter;  // work counter (persistent kernels)
+    int64_t stride;               // work item i takes position (i * stride) mod count
 };
+
+// position of work item `item`: a multiplicative permutation of [0, count)
+// (stride coprime to count) that spreads consecutive items over the grid, so
+// the last items taken are not one expensive region of the parameter plane
+LYAP_D int64_t item_position(const SweepIO& io, int64_t item) {
+    return (int64_t)(((unsigned __int128)item * (unsigned __int128)io.stride) % (unsigned __int128)io.count);
+}
 
 }  // namespace lyap

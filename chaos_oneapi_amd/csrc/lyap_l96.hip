@@ -1,0 +1,391 @@
+// Lorenz-96 sweep kernels, N in {8, 16, 32}, full N-exponent spectrum.
+//
+// Layout: one point per block of W wave64s.  Lane l of wave w owns component
+// i = l % N of the base state and of E = N*N/(64*W) deviation vectors, the
+// vectors r = gid + (W*64/N)*m (gid = w*64/N + l/N, m < E).  The base state is
+// replicated in every N-lane group (the replicas cost no extra instructions),
+// so the Jacobian at each stage state is available to every vector's lanes.
+//   - neighbour couplings x_{i+1}, x_{i-1}, x_{i-2} (and of each vector) are
+//     lane shuffles inside the N-lane group;
+//   - the error norm and the Jacobian-norm step cap are xor-butterfly
+//     reductions over the group (offsets N/2 .. 1), identical in every lane;
+//   - renormalisation stages the N x N deviation matrix in LDS, transposed so
+//     lane r owns vector r, and runs right-looking MGS with one barrier per
+//     vector (double-buffered q_j).
+// Every lane computes the same control decisions from the same bits, so
+// all branches are uniform; they are also made wave-uniform explicitly.
+// The launch is persistent: a block takes its next point from the counter.
+#include "lyap_device.hpp"
+#include "lyap_launch.hpp"
+
+namespace lyap {
+
+__global__ void k_reset_counter_l96(unsigned long long* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
+}
+
+namespace {
+
+template <int N>
+LYAP_D double group_sum(double v) {
+#pragma unroll
+    for (int o = N / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
+    return v;
+}
+
+template <int N>
+LYAP_D double group_max(double v) {
+#pragma unroll
+    for (int o = N / 2; o >= 1; o >>= 1) {
+        const double w = __shfl_xor(v, o);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
+LYAP_D bool wave_uniform(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
+
+}  // namespace
+
+template <int N, int W, int INTEG, bool ADAPT, bool MIXED>
+__global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const SweepIO io) {
+    using TB = Tab<INTEG>;
+    using T = typename std::conditional<MIXED, float, double>::type;
+    constexpr int G = 64 / N;   // vector groups per wave
+    constexpr int NG = W * G;   // vector groups per block
+    constexpr int K = N;        // exponents
+    constexpr int E = K / NG;   // vectors per lane
+    constexpr int S = TB::S;
+    static_assert(K % NG == 0 && E >= 1, "layout");
+
+    __shared__ double sV[K * N];
+    __shared__ double sQ[2][N];
+    __shared__ unsigned long long s_item;
+
+    const int tid = (int)threadIdx.x;
+    const int lane = tid & 63;
+    const int i = lane % N;
+    const int gid = (tid >> 6) * G + lane / N;
+    const int base_lane = lane - i;
+    const int src_p1 = base_lane + (i + 1) % N;
+    const int src_m1 = base_lane + (i + N - 1) % N;
+    const int src_m2 = base_lane + (i + N - 2) % N;
+
+    const int64_t n_tr = d.n_renorm_transient;
+    const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
+    const int64_t max_steps = d.max_steps, spr = d.steps_per_renorm;
+    const double rtol = d.rtol, atol = d.atol, h_min = d.h_min, h_max = d.h_max;
+    const double t_renorm = d.t_renorm, esc = d.escape_radius, h_fixed = d.h;
+    const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
+                               : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
+    const int n_exps = d.n_exps;
+    const double log_floor = det_log(1e-4);
+
+    for (;;) {
+        if (tid == 0) s_item = atomicAdd(io.counter, 1ULL);
+        __syncthreads();
+        const int64_t got = (int64_t)s_item;
+        __syncthreads();
+        if (got >= io.count) break;
+        const int64_t pos = item_position(io, got);
+        KernargDesc* dp = kernarg_desc();
+        const int64_t idx = io.idx_list ? io.idx_list[pos] : io.first + pos;
+        double p[2];
+        point_params<2>(*dp, idx, io.params, p);
+        const double F = p[0], dd = p[1];
+        const T FT = (T)F, dT = (T)dd;
+        double x = io.inits ? io.inits[idx * N + i] : init_component(*dp, idx, i, p);
+        double V[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) V[m] = (gid + NG * m == i) ? 1.0 : 0.0;
+        double sum = 0.0, comp = 0.0;
+        double t = 0.0, h = dp->h;
+        int64_t mr = 0, step_in = 0;
+        uint32_t nacc = 0, nrej = 0;
+        Ctrl ctrl{1e-4, log_floor, false};
+        uint32_t st = LYAP_PT_OK;
+        double k1 = 0.0;
+        if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
+            const double xp1 = __shfl(x, src_p1), xm1 = __shfl(x, src_m1), xm2 = __shfl(x, src_m2);
+            k1 = fma(xp1 - xm2, xm1, fma(-dd, x, F));
+        }
+
+        for (;;) {
+            double h_try = h;
+            bool clipped = false;
+            double t_next = 0.0;
+            if constexpr (ADAPT) {
+                if (wave_uniform((int64_t)nacc + (int64_t)nrej >= max_steps)) {
+                    st = LYAP_PT_MAXSTEPS;
+                    break;
+                }
+                t_next = (double)(mr + 1) * t_renorm;
+                {
+                    const double xp1 = __shfl(x, src_p1), xm1 = __shfl(x, src_m1),
+                                 xm2 = __shfl(x, src_m2);
+                    const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
+                    const double jn = group_max<N>(row);
+                    if (h_try * jn > 1.0) h_try = 1.0 / jn;
+                }
+                if (t_next - t <= 1.0625 * h_try) {
+                    h_try = t_next - t;
+                    clipped = true;
+                }
+                if (wave_uniform(h_try < h_min)) {
+                    st = LYAP_PT_STEP_UNDERFLOW;
+                    break;
+                }
+            } else {
+                h_try = h_fixed;
+            }
+
+            // base step, own component; J data of every stage kept for the tangent
+            double k[S];
+            T jm1[S], jd[S];
+            {
+                const RowB<TB, double, S> rb;
+#pragma unroll
+                for (int s = 0; s < S; ++s) {
+                    if (!ADAPT && !TB::for_solution(s)) {
+                        k[s] = 0.0;
+                        jm1[s] = jd[s] = T(0);
+                        continue;
+                    }
+                    const RowA<TB, double, S> row(s);
+                    double acc = 0.0;
+                    bool any = false;
+#pragma unroll
+                    for (int j = 0; j < S; ++j) {
+                        if (j >= s || TB::a(s, j) == 0.0) continue;
+                        acc = any ? fma(row.c[j], k[j], acc) : row.c[j] * k[j];
+                        any = true;
+                    }
+                    const double arg = any ? fma(h_try, acc, x) : x;
+                    const double xp1 = __shfl(arg, src_p1), xm1 = __shfl(arg, src_m1),
+                                 xm2 = __shfl(arg, src_m2);
+                    jm1[s] = (T)xm1;
+                    jd[s] = (T)xp1 - (T)xm2;
+                    if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
+                    else k[s] = fma(xp1 - xm2, xm1, fma(-dd, arg, F));
+                }
+                double accb = 0.0;
+                bool anyb = false;
+#pragma unroll
+                for (int j = 0; j < S; ++j) {
+                    if (TB::b(j) == 0.0) continue;
+                    accb = anyb ? fma(rb.c[j], k[j], accb) : rb.c[j] * k[j];
+                    anyb = true;
+                }
+                const double xn = fma(h_try, accb, x);
+                bool accept = true;
+                if constexpr (ADAPT) {
+                    const RowE<TB, double, S> re;
+                    double acce = 0.0;
+                    bool anye = false;
+#pragma unroll
+                    for (int j = 0; j < S; ++j) {
+                        if (TB::e(j) == 0.0) continue;
+                        acce = anye ? fma(re.c[j], k[j], acce) : re.c[j] * k[j];
+                        anye = true;
+                    }
+                    const double ev = h_try * acce;
+                    const double ax = fabs(x), an = fabs(xn);
+                    const double sk = fma(rtol, ax > an ? ax : an, atol);
+                    const double q = ev / sk;
+                    const double err = sqrt(group_sum<N>(q * q) / (double)N);
+                    if (wave_uniform(!(err < kInf))) {
+                        st = LYAP_PT_NONFINITE;
+                        break;
+                    }
+                    double h_next;
+                    accept = ctrl_update<INTEG>(ctrl, err, det_log(err), log_floor, h_try, h_max,
+                                                h_next);
+                    accept = wave_uniform(accept);
+                    h = h_next;
+                    if (!accept) {
+                        ++nrej;
+                        continue;
+                    }
+                }
+                // tangent step of the lane's E vectors, J at every stage state
+                const T hT = (T)h_try;
+                const RowB<TB, T, S> rbt;
+#pragma unroll
+                for (int m = 0; m < E; ++m) {
+                    const T v0 = (T)V[m];
+                    T kv[S];
+#pragma unroll
+                    for (int s = 0; s < S; ++s) {
+                        if (!TB::for_solution(s)) {
+                            kv[s] = T(0);
+                            continue;
+                        }
+                        const RowA<TB, T, S> row(s);
+                        T acc = T(0);
+                        bool any = false;
+#pragma unroll
+                        for (int j = 0; j < S; ++j) {
+                            if (j >= s || TB::a(s, j) == 0.0) continue;
+                            acc = any ? fma(row.c[j], kv[j], acc) : row.c[j] * kv[j];
+                            any = true;
+                        }
+                        const T varg = any ? fma(hT, acc, v0) : v0;
+                        const T vp1 = __shfl(varg, src_p1), vm1 = __shfl(varg, src_m1),
+                                vm2 = __shfl(varg, src_m2);
+                        kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
+                    }
+                    T accv = T(0);
+                    bool anyv = false;
+#pragma unroll
+                    for (int j = 0; j < S; ++j) {
+                        if (TB::b(j) == 0.0) continue;
+                        accv = anyv ? fma(rbt.c[j], kv[j], accv) : rbt.c[j] * kv[j];
+                        anyv = true;
+                    }
+                    if constexpr (MIXED) V[m] = V[m] + (double)(hT * accv);
+                    else V[m] = fma(h_try, accv, V[m]);
+                }
+                x = xn;
+                if (INTEG == LYAP_INT_DOPRI5 && ADAPT) k1 = k[S - 1];
+            }
+            ++nacc;
+            bool boundary;
+            if constexpr (ADAPT) {
+                t = clipped ? t_next : t + h_try;
+                boundary = clipped;
+            } else {
+                boundary = (++step_in == spr);
+                if (boundary) step_in = 0;
+            }
+            if constexpr (!ADAPT) {
+                if (boundary && wave_uniform(__any(!isfinite(x)))) {
+                    st = LYAP_PT_NONFINITE;
+                    break;
+                }
+            }
+            if (ADAPT || boundary) {
+                if (wave_uniform(__any(!(fabs(x) <= esc)))) {
+                    st = LYAP_PT_ESCAPED;
+                    break;
+                }
+            }
+            if (!wave_uniform(boundary)) continue;
+
+            // renormalisation: MGS over the deviation matrix staged in LDS
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) sV[(gid + NG * m) * N + i] = V[m];
+            __syncthreads();
+            double vr[N];
+            double ln = 0.0;
+            if (tid < K) {
+#pragma unroll
+                for (int c = 0; c < N; ++c) vr[c] = sV[tid * N + c];
+            }
+            for (int j = 0; j < K; ++j) {
+                if (tid == j) {
+                    double s2 = vr[0] * vr[0];
+#pragma unroll
+                    for (int c = 1; c < N; ++c) s2 = fma(vr[c], vr[c], s2);
+                    const double nrm = sqrt(s2);
+                    ln = det_log(nrm);
+                    const double inv = 1.0 / nrm;
+#pragma unroll
+                    for (int c = 0; c < N; ++c) {
+                        vr[c] = vr[c] * inv;
+                        sQ[j & 1][c] = vr[c];
+                    }
+                }
+                __syncthreads();
+                if (tid > j && tid < K) {
+                    const double* q = sQ[j & 1];
+                    double cdot = q[0] * vr[0];
+#pragma unroll
+                    for (int c = 1; c < N; ++c) cdot = fma(q[c], vr[c], cdot);
+#pragma unroll
+                    for (int c = 0; c < N; ++c) vr[c] = fma(-cdot, q[c], vr[c]);
+                }
+            }
+            if (tid < K) {
+#pragma unroll
+                for (int c = 0; c < N; ++c) sV[tid * N + c] = vr[c];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < E; ++m) V[m] = sV[(gid + NG * m) * N + i];
+            ++mr;
+            if (mr > n_tr && tid < K) kahan_add(sum, comp, ln);
+            if (mr == m_total) break;
+        }
+
+        if (tid < n_exps) io.lambda[pos * n_exps + tid] = (st == LYAP_PT_OK) ? sum / t_avg : __builtin_nan("");
+        if (tid == 0) {
+            if (io.status) io.status[pos] = st;
+            if (io.n_acc) io.n_acc[pos] = nacc;
+            if (io.n_rej) io.n_rej[pos] = nrej;
+        }
+    }
+}
+
+namespace {
+
+template <int N, int W, int INTEG, bool ADAPT, bool MIXED>
+hipError_t launch_one(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus) {
+    auto kern = k_l96<N, W, INTEG, ADAPT, MIXED>;
+    int per_cu = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    int64_t blocks = (int64_t)per_cu * num_cus;
+    if (blocks > io.count) blocks = io.count;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_reset_counter_l96, dim3(1), dim3(64), 0, s, io.counter);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), 0, s, d, io);
+    return hipGetLastError();
+}
+
+template <int N, int W>
+hipError_t launch_nw(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus) {
+    const int key = (d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0);
+    switch (key) {
+#define LYAP_L96_CASE(I, A, X) \
+    case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0): return launch_one<N, W, I, A, X>(d, io, s, num_cus);
+        LYAP_L96_CASE(LYAP_INT_DVERK, false, false)
+        LYAP_L96_CASE(LYAP_INT_DVERK, false, true)
+        LYAP_L96_CASE(LYAP_INT_DVERK, true, false)
+        LYAP_L96_CASE(LYAP_INT_DVERK, true, true)
+        LYAP_L96_CASE(LYAP_INT_DOPRI5, false, false)
+        LYAP_L96_CASE(LYAP_INT_DOPRI5, false, true)
+        LYAP_L96_CASE(LYAP_INT_DOPRI5, true, false)
+        LYAP_L96_CASE(LYAP_INT_DOPRI5, true, true)
+#undef LYAP_L96_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+int l96_default_waves(int n) { return n == 8 ? 1 : (n == 16 ? 2 : 4); }
+
+bool l96_supported(int n, int waves) {
+    return (n == 8 && waves == 1) || (n == 16 && (waves == 1 || waves == 2 || waves == 4)) ||
+           (n == 32 && (waves == 2 || waves == 4));
+}
+
+hipError_t launch_l96_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
+                            int waves) {
+    if (waves <= 0) waves = l96_default_waves(d.dim);
+    if (d.method != LYAP_METHOD_VAR || !l96_supported(d.dim, waves)) return hipErrorInvalidValue;
+    switch (d.dim * 16 + waves) {
+        case 8 * 16 + 1: return launch_nw<8, 1>(d, io, s, num_cus);
+        case 16 * 16 + 1: return launch_nw<16, 1>(d, io, s, num_cus);
+        case 16 * 16 + 2: return launch_nw<16, 2>(d, io, s, num_cus);
+        case 16 * 16 + 4: return launch_nw<16, 4>(d, io, s, num_cus);
+        case 32 * 16 + 2: return launch_nw<32, 2>(d, io, s, num_cus);
+        case 32 * 16 + 4: return launch_nw<32, 4>(d, io, s, num_cus);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace lyap

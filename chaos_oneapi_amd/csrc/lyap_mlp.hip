@@ -1,0 +1,542 @@
+// Surrogate network of the sweep: (p1, p2) -> 2*F Fourier features -> three
+// hidden layers of H GELU units -> [mu1, mu2, mu3, log-variance, escape logit].
+//
+// Weights are kept as an FP32 master copy plus two BF16 copies (W^T for the
+// forward products, W for the input-gradient products); every matrix product
+// runs on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation in one NT GEMM
+// kernel, C[M,N] = A[M,K] * B[N,K]^T (+ bias), both operands row-major so each
+// lane fetches four contiguous bf16 per operand.  Activations are stored in
+// both row- and column-major BF16 so weight gradients are NT products too.
+// Adam runs on the FP32 master weights and re-issues the BF16 copies.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "chaos_lyap.h"
+#include "lyap_device.hpp"
+#include "lyap_launch.hpp"
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace lyap {
+namespace mlp {
+
+constexpr int kOut = 5;       // mu1, mu2, mu3, log-variance, escape logit
+constexpr int kOutPad = 16;   // head rows padded to one MFMA tile
+constexpr int kLayers = 4;
+
+LYAP_HD uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                     // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+LYAP_HD float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
+
+// ---------------------------------------------------------------------------
+// NT GEMM on MFMA.  One wave computes a 16 x (16*NB) tile of C; lane l holds
+// A[m0 + l%16][k0 + 4(l/16) .. +3] and B[n0 + 16j + l%16][k0 + 4(l/16) .. +3];
+// the accumulator element i of lane l is C[m0 + 4(l/16) + i][n0 + 16j + l%16].
+// M, N multiples of 16 (of 16*NB for N), K a multiple of 16.
+// ---------------------------------------------------------------------------
+template <int NB>
+__global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
+                                                 const uint16_t* __restrict__ B, float* __restrict__ C,
+                                                 const float* __restrict__ bias, int M, int N, int K,
+                                                 float beta) {
+    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const int tiles_n = N / (16 * NB);
+    const int m0 = (wave / tiles_n) * 16;
+    const int n0 = (wave % tiles_n) * (16 * NB);
+    if (m0 >= M) return;
+    const int r = lane & 15, q = lane >> 4;
+    v4f acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* a_row = A + (size_t)(m0 + r) * K + 4 * q;
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        const v4s a = *reinterpret_cast<const v4s*>(a_row + k0);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const v4s b = *reinterpret_cast<const v4s*>(B + (size_t)(n0 + 16 * j + r) * K + 4 * q + k0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int col = n0 + 16 * j + r;
+        const float bv = bias ? bias[col] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float* c = C + (size_t)(m0 + 4 * q + i) * N + col;
+            *c = (beta != 0.f ? beta * *c : 0.f) + acc[j][i] + bv;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+__device__ __forceinline__ float gelu_grad(float x) {
+    return 0.5f * (1.f + erff(x * 0.70710678118654752f)) +
+           x * 0.39894228040143268f * expf(-0.5f * x * x);
+}
+
+// features[b][j] = sin(2 pi (B_j . u)), features[b][F+j] = cos(...); row- and column-major bf16
+__global__ void k_fourier(const double* __restrict__ p, int n, int n_valid, const float* __restrict__ fb,
+                          int F, double lo1, double inv1, double lo2, double inv2,
+                          uint16_t* __restrict__ X, uint16_t* __restrict__ Xt) {
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= n) return;
+    float u0 = 0.f, u1 = 0.f;
+    if (b < n_valid) {
+        u0 = (float)((p[2 * b] - lo1) * inv1);
+        u1 = (float)((p[2 * b + 1] - lo2) * inv2);
+    }
+    for (int j = 0; j < F; ++j) {
+        const float f = 6.283185307179586f * (fb[2 * j] * u0 + fb[2 * j + 1] * u1);
+        float s, c;
+        sincosf(f, &s, &c);
+        const uint16_t sb = f2bf(s), cb = f2bf(c);
+        X[(size_t)b * 2 * F + j] = sb;
+        X[(size_t)b * 2 * F + F + j] = cb;
+        Xt[(size_t)j * n + b] = sb;
+        Xt[(size_t)(F + j) * n + b] = cb;
+    }
+}
+
+// H = gelu(Z) in row- and column-major bf16
+__global__ void k_gelu_fwd(const float* __restrict__ Z, int n, int H, uint16_t* __restrict__ Hr,
+                           uint16_t* __restrict__ Ht) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * H) return;
+    const int b = (int)(idx / H), j = (int)(idx % H);
+    const uint16_t h = f2bf(gelu(Z[idx]));
+    Hr[idx] = h;
+    Ht[(size_t)j * n + b] = h;
+}
+
+// dZ = dH * gelu'(Z) in row- and column-major bf16
+__global__ void k_gelu_bwd(const float* __restrict__ dH, const float* __restrict__ Z, int n, int H,
+                           uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * H) return;
+    const int b = (int)(idx / H), j = (int)(idx % H);
+    const uint16_t g = f2bf(dH[idx] * gelu_grad(Z[idx]));
+    dZr[idx] = g;
+    dZt[(size_t)j * n + b] = g;
+}
+
+// loss and head gradient: Gaussian NLL on mu1 with the learned log-variance,
+// squared error on mu2, mu3 (targets normalised), logistic loss on escape.
+// Rows b >= n_valid are padding (zero gradient).
+__global__ void k_head_loss(const float* __restrict__ Z, const double* __restrict__ lam,
+                            const uint32_t* __restrict__ status, int n, int n_valid,
+                            const double* __restrict__ ym, const double* __restrict__ ys, float inv_n,
+                            uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                            float* __restrict__ dZf, float* __restrict__ loss) {
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= n) return;
+    float g[kOutPad];
+#pragma unroll
+    for (int j = 0; j < kOutPad; ++j) g[j] = 0.f;
+    float l = 0.f;
+    if (b < n_valid) {
+        const float* z = Z + (size_t)b * kOutPad;
+        const float bad = status[b] != 0 ? 1.f : 0.f;
+        const float zl = z[4];
+        const float sp = zl > 0.f ? zl + log1pf(expf(-zl)) : log1pf(expf(zl));
+        const float sig = 1.f / (1.f + expf(-zl));
+        l += sp - bad * zl;
+        g[4] = (sig - bad) * inv_n;
+        if (bad == 0.f) {
+            const float y1 = (float)((lam[3 * b] - ym[0]) / ys[0]);
+            const float y2 = (float)((lam[3 * b + 1] - ym[1]) / ys[1]);
+            const float y3 = (float)((lam[3 * b + 2] - ym[2]) / ys[2]);
+            const float s = z[3], e = expf(-s);
+            const float d1 = z[0] - y1, d2 = z[1] - y2, d3 = z[2] - y3;
+            l += 0.5f * (d1 * d1 * e + s) + 0.5f * (d2 * d2 + d3 * d3);
+            g[0] = d1 * e * inv_n;
+            g[1] = d2 * inv_n;
+            g[2] = d3 * inv_n;
+            g[3] = 0.5f * (1.f - d1 * d1 * e) * inv_n;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kOutPad; ++j) {
+        const uint16_t gb = f2bf(g[j]);
+        dZr[(size_t)b * kOutPad + j] = gb;
+        dZt[(size_t)j * n + b] = gb;
+        dZf[(size_t)b * kOutPad + j] = g[j];
+    }
+    if (loss && b < n_valid) atomicAdd(loss, l * inv_n);
+}
+
+// db[j] = sum_b dZ[b][j]
+__global__ void k_colsum(const float* __restrict__ dZ, int n, int H, float* __restrict__ db) {
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= H) return;
+    float s = 0.f;
+    for (int b = 0; b < n; ++b) s += dZ[(size_t)b * H + j];
+    db[j] = s;
+}
+
+// dZ (bf16, row-major) -> fp32 for the bias reduction of hidden layers
+__global__ void k_bf_to_f(const uint16_t* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = bf2f(a[i]);
+}
+
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, const uint8_t* __restrict__ trainable, int64_t n, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !trainable[i]) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// master [O][I] fp32 -> Wt [O][I] bf16 and Wn [I][O] bf16
+__global__ void k_issue_bf16(const float* __restrict__ w, int O, int I, uint16_t* __restrict__ wt,
+                             uint16_t* __restrict__ wn) {
+    const int idx = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (idx >= O * I) return;
+    const int o = idx / I, i = idx % I;
+    const uint16_t b = f2bf(w[idx]);
+    wt[idx] = b;
+    wn[(size_t)i * O + o] = b;
+}
+
+// inference output: de-normalised means, std of lambda1, escape probability
+__global__ void k_head_out(const float* __restrict__ Z, int n_valid, const double* __restrict__ ym,
+                           const double* __restrict__ ys, float* __restrict__ out) {
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= n_valid) return;
+    const float* z = Z + (size_t)b * kOutPad;
+    float* o = out + (size_t)b * kOut;
+    o[0] = (float)(z[0] * ys[0] + ym[0]);
+    o[1] = (float)(z[1] * ys[1] + ym[1]);
+    o[2] = (float)(z[2] * ys[2] + ym[2]);
+    o[3] = (float)(expf(0.5f * z[3]) * ys[0]);
+    o[4] = 1.f / (1.f + expf(-z[4]));
+}
+
+}  // namespace mlp
+}  // namespace lyap
+
+// ---------------------------------------------------------------------------
+// C-ABI object
+// ---------------------------------------------------------------------------
+using namespace lyap::mlp;
+
+namespace {
+
+template <class... A>
+int mfail(int code, const char* fmt, A... args) {
+    return lyap::set_error(code, fmt, args...);
+}
+
+#define MLP_HIP(call)                                                               \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) return mfail(LYAP_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+struct lyap_mlp {
+    int device = 0;
+    lyap_mlp_config cfg{};
+    int dims[kLayers + 1];  // in, hidden, hidden, hidden, out_pad
+    int64_t off_w[kLayers], off_b[kLayers], n_params = 0;
+    float* master = nullptr;
+    float* grad = nullptr;
+    float* adam_m = nullptr;
+    float* adam_v = nullptr;
+    uint8_t* trainable = nullptr;
+    uint16_t* wt[kLayers] = {};
+    uint16_t* wn[kLayers] = {};
+    float* fourier = nullptr;
+    double* norm = nullptr;  // y_mean[3], y_std[3]
+    // workspace for max_batch rows
+    uint16_t *act[kLayers] = {}, *act_t[kLayers] = {};  // inputs of layer l (row, column major)
+    float* z[kLayers] = {};                              // pre-activations
+    uint16_t *dz = nullptr, *dz_t = nullptr;
+    float* dh = nullptr;
+    float* dzf = nullptr;
+    float* loss = nullptr;
+    int64_t step = 0;
+    std::vector<void*> allocs;
+};
+
+extern "C" {
+
+static int mlp_alloc(lyap_mlp* m, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    m->allocs.push_back(*p);
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_destroy(lyap_mlp* m) {
+    if (!m) return LYAP_SUCCESS;
+    hipSetDevice(m->device);
+    for (void* p : m->allocs) hipFree(p);
+    delete m;
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
+    if (!cfg || !out) return mfail(LYAP_ERR_INVALID, "lyap_mlp_create: NULL argument");
+    *out = nullptr;
+    if (cfg->max_batch <= 0 || cfg->max_batch % 64 != 0)
+        return mfail(LYAP_ERR_INVALID, "max_batch must be a positive multiple of 64");
+    if (cfg->hidden <= 0 || cfg->hidden % 64 != 0)
+        return mfail(LYAP_ERR_INVALID, "hidden must be a positive multiple of 64");
+    if (cfg->n_fourier <= 0 || (2 * cfg->n_fourier) % 16 != 0)
+        return mfail(LYAP_ERR_INVALID, "2*n_fourier must be a multiple of 16");
+    if (!(cfg->hi1 > cfg->lo1) || !(cfg->hi2 > cfg->lo2))
+        return mfail(LYAP_ERR_INVALID, "input ranges must be non-empty");
+    MLP_HIP(hipSetDevice(device));
+    lyap_mlp* m = new lyap_mlp();
+    m->device = device;
+    m->cfg = *cfg;
+    const int F2 = 2 * cfg->n_fourier, H = cfg->hidden, B = cfg->max_batch;
+    int dims[kLayers + 1] = {F2, H, H, H, kOutPad};
+    std::memcpy(m->dims, dims, sizeof(dims));
+    for (int l = 0; l < kLayers; ++l) {
+        m->off_w[l] = m->n_params;
+        m->n_params += (int64_t)dims[l + 1] * dims[l];
+        m->off_b[l] = m->n_params;
+        m->n_params += dims[l + 1];
+    }
+    int rc = LYAP_SUCCESS;
+    const size_t P = (size_t)m->n_params;
+    rc |= mlp_alloc(m, (void**)&m->master, P * 4);
+    rc |= mlp_alloc(m, (void**)&m->grad, P * 4);
+    rc |= mlp_alloc(m, (void**)&m->adam_m, P * 4);
+    rc |= mlp_alloc(m, (void**)&m->adam_v, P * 4);
+    rc |= mlp_alloc(m, (void**)&m->trainable, P);
+    rc |= mlp_alloc(m, (void**)&m->fourier, (size_t)cfg->n_fourier * 2 * 4);
+    rc |= mlp_alloc(m, (void**)&m->norm, 6 * 8);
+    for (int l = 0; l < kLayers; ++l) {
+        const size_t w = (size_t)dims[l + 1] * dims[l];
+        rc |= mlp_alloc(m, (void**)&m->wt[l], w * 2);
+        rc |= mlp_alloc(m, (void**)&m->wn[l], w * 2);
+        rc |= mlp_alloc(m, (void**)&m->act[l], (size_t)B * dims[l] * 2);
+        rc |= mlp_alloc(m, (void**)&m->act_t[l], (size_t)B * dims[l] * 2);
+        rc |= mlp_alloc(m, (void**)&m->z[l], (size_t)B * dims[l + 1] * 4);
+    }
+    rc |= mlp_alloc(m, (void**)&m->dz, (size_t)B * H * 2);
+    rc |= mlp_alloc(m, (void**)&m->dz_t, (size_t)B * H * 2);
+    rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
+    rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * H * 4);
+    rc |= mlp_alloc(m, (void**)&m->loss, 4);
+    if (rc != LYAP_SUCCESS) {
+        lyap_mlp_destroy(m);
+        return LYAP_ERR_HIP;
+    }
+    // seeded initialisation on the host: U(-sqrt(3/fan_in), sqrt(3/fan_in)) weights,
+    // zero biases, head rows beyond the 5 outputs frozen at zero
+    std::vector<float> w(P, 0.f);
+    std::vector<uint8_t> tr(P, 1);
+    for (int l = 0; l < kLayers; ++l) {
+        const float a = std::sqrt(3.0f / (float)dims[l]) * (l == kLayers - 1 ? 0.5f : 1.0f);
+        for (int o = 0; o < dims[l + 1]; ++o)
+            for (int i = 0; i < dims[l]; ++i) {
+                const int64_t k = m->off_w[l] + (int64_t)o * dims[l] + i;
+                const double u = lyap::uniform01(cfg->seed, k, 7);
+                w[k] = (float)((2.0 * u - 1.0) * a);
+                if (l == kLayers - 1 && o >= kOut) {
+                    w[k] = 0.f;
+                    tr[k] = 0;
+                }
+            }
+        if (l == kLayers - 1)
+            for (int o = kOut; o < kOutPad; ++o) tr[m->off_b[l] + o] = 0;
+    }
+    std::vector<float> fb((size_t)cfg->n_fourier * 2);
+    for (size_t j = 0; j < fb.size(); ++j) {
+        double z = 0.0;
+        for (int r = 0; r < 12; ++r) z += lyap::uniform01(cfg->seed, (int64_t)j, 100 + r);
+        fb[j] = (float)((z - 6.0) * cfg->fourier_sigma);
+    }
+    double norm[6] = {cfg->y_mean[0], cfg->y_mean[1], cfg->y_mean[2],
+                      cfg->y_std[0] > 0 ? cfg->y_std[0] : 1.0, cfg->y_std[1] > 0 ? cfg->y_std[1] : 1.0,
+                      cfg->y_std[2] > 0 ? cfg->y_std[2] : 1.0};
+    hipError_t e = hipMemcpy(m->master, w.data(), P * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->trainable, tr.data(), P, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->fourier, fb.data(), fb.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(m->norm, norm, sizeof(norm), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m->adam_m, 0, P * 4);
+    if (e == hipSuccess) e = hipMemset(m->adam_v, 0, P * 4);
+    if (e == hipSuccess) e = hipMemset(m->grad, 0, P * 4);
+    if (e != hipSuccess) {
+        lyap_mlp_destroy(m);
+        return mfail(LYAP_ERR_HIP, "mlp init copy: %s", hipGetErrorString(e));
+    }
+    for (int l = 0; l < kLayers; ++l) {
+        const int n = dims[l + 1] * dims[l];
+        hipLaunchKernelGGL(k_issue_bf16, dim3((n + 255) / 256), dim3(256), 0, 0, m->master + m->off_w[l],
+                           dims[l + 1], dims[l], m->wt[l], m->wn[l]);
+    }
+    e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        lyap_mlp_destroy(m);
+        return mfail(LYAP_ERR_HIP, "mlp init: %s", hipGetErrorString(e));
+    }
+    *out = m;
+    return LYAP_SUCCESS;
+}
+
+static void gemm(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, const float* bias, int M,
+                 int N, int K, float beta) {
+    const int nb = (N % 64 == 0) ? 4 : 1;
+    const int waves = (M / 16) * (N / (16 * nb));
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    if (nb == 4) hipLaunchKernelGGL(k_gemm_nt<4>, grid, block, 0, s, A, B, C, bias, M, N, K, beta);
+    else hipLaunchKernelGGL(k_gemm_nt<1>, grid, block, 0, s, A, B, C, bias, M, N, K, beta);
+}
+
+static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s) {
+    const auto& c = m->cfg;
+    const int* D = m->dims;
+    hipLaunchKernelGGL(k_fourier, dim3((n_pad + 255) / 256), dim3(256), 0, s, d_p, n_pad, n_valid, m->fourier,
+                       c.n_fourier, c.lo1, 1.0 / (c.hi1 - c.lo1), c.lo2, 1.0 / (c.hi2 - c.lo2), m->act[0],
+                       m->act_t[0]);
+    for (int l = 0; l < kLayers; ++l) {
+        gemm(s, m->act[l], m->wt[l], m->z[l], m->master + m->off_b[l], n_pad, D[l + 1], D[l], 0.f);
+        if (l + 1 < kLayers) {
+            const int64_t tot = (int64_t)n_pad * D[l + 1];
+            hipLaunchKernelGGL(k_gelu_fwd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m->z[l], n_pad,
+                               D[l + 1], m->act[l + 1], m->act_t[l + 1]);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "mlp forward: %s", hipGetErrorString(e));
+    return LYAP_SUCCESS;
+}
+
+static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, int n_pad, int n_valid,
+                    hipStream_t s) {
+    const int* D = m->dims;
+    hipLaunchKernelGGL(k_head_loss, dim3((n_pad + 255) / 256), dim3(256), 0, s, m->z[kLayers - 1], d_lam,
+                       d_status, n_pad, n_valid, m->norm, m->norm + 3, 1.0f / (float)n_valid, m->dz, m->dz_t,
+                       m->dzf, m->loss);
+    for (int l = kLayers - 1; l >= 0; --l) {
+        // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T
+        gemm(s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], nullptr, D[l + 1], D[l], n_pad, 0.f);
+        hipLaunchKernelGGL(k_colsum, dim3((D[l + 1] + 63) / 64), dim3(64), 0, s, m->dzf, n_pad, D[l + 1],
+                           m->grad + m->off_b[l]);
+        if (l == 0) break;
+        // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU
+        gemm(s, m->dz, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);
+        const int64_t tot = (int64_t)n_pad * D[l];
+        hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m->dh, m->z[l - 1],
+                           n_pad, D[l], m->dz, m->dz_t);
+        hipLaunchKernelGGL(k_bf_to_f, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m->dz, m->dzf, tot);
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "mlp backward: %s", hipGetErrorString(e));
+    return LYAP_SUCCESS;
+}
+
+static int check_batch(lyap_mlp* m, int64_t n) {
+    if (!m) return mfail(LYAP_ERR_INVALID, "NULL network");
+    if (n <= 0 || n > m->cfg.max_batch)
+        return mfail(LYAP_ERR_INVALID, "batch %lld outside (0, %d]", (long long)n, m->cfg.max_batch);
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_backward(lyap_mlp* m, const double* d_p, const double* d_lambda, const uint32_t* d_status,
+                      int64_t n, float* d_loss, void* stream) {
+    int rc = check_batch(m, n);
+    if (rc) return rc;
+    if (!d_p || !d_lambda || !d_status) return mfail(LYAP_ERR_INVALID, "NULL batch pointer");
+    hipStream_t s = (hipStream_t)stream;
+    MLP_HIP(hipSetDevice(m->device));
+    const int n_pad = (int)((n + 63) / 64 * 64);
+    MLP_HIP(hipMemsetAsync(m->loss, 0, 4, s));
+    if ((rc = forward(m, d_p, n_pad, (int)n, s))) return rc;
+    if ((rc = backward(m, d_lambda, d_status, n_pad, (int)n, s))) return rc;
+    if (d_loss) MLP_HIP(hipMemcpyAsync(d_loss, m->loss, 4, hipMemcpyDeviceToDevice, s));
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_train_step(lyap_mlp* m, const double* d_p, const double* d_lambda, const uint32_t* d_status,
+                        int64_t n, float* d_loss, void* stream) {
+    int rc = lyap_mlp_backward(m, d_p, d_lambda, d_status, n, d_loss, stream);
+    if (rc) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const auto& c = m->cfg;
+    m->step += 1;
+    const float c1 = 1.f - std::pow(c.beta1, (float)m->step), c2 = 1.f - std::pow(c.beta2, (float)m->step);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->n_params + 255) / 256)), dim3(256), 0, s, m->master, m->grad,
+                       m->adam_m, m->adam_v, m->trainable, m->n_params, c.lr, c.beta1, c.beta2, c.eps, c1, c2);
+    for (int l = 0; l < kLayers; ++l) {
+        const int nw = m->dims[l + 1] * m->dims[l];
+        hipLaunchKernelGGL(k_issue_bf16, dim3((nw + 255) / 256), dim3(256), 0, s, m->master + m->off_w[l],
+                           m->dims[l + 1], m->dims[l], m->wt[l], m->wn[l]);
+    }
+    MLP_HIP(hipGetLastError());
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_forward(lyap_mlp* m, const double* d_p, int64_t n, float* d_out, void* stream) {
+    int rc = check_batch(m, n);
+    if (rc) return rc;
+    if (!d_p || !d_out) return mfail(LYAP_ERR_INVALID, "NULL pointer");
+    hipStream_t s = (hipStream_t)stream;
+    MLP_HIP(hipSetDevice(m->device));
+    const int n_pad = (int)((n + 63) / 64 * 64);
+    if ((rc = forward(m, d_p, n_pad, (int)n, s))) return rc;
+    hipLaunchKernelGGL(k_head_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->z[kLayers - 1], (int)n,
+                       m->norm, m->norm + 3, d_out);
+    MLP_HIP(hipGetLastError());
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_param_count(lyap_mlp* m, int64_t* n) {
+    if (!m || !n) return mfail(LYAP_ERR_INVALID, "NULL argument");
+    *n = m->n_params;
+    return LYAP_SUCCESS;
+}
+
+// which = 0: parameters, 1: gradients, 2: Fourier matrix [n_fourier][2]; host copies
+int lyap_mlp_get(lyap_mlp* m, int which, float* host, int64_t n) {
+    if (!m || !host) return mfail(LYAP_ERR_INVALID, "NULL argument");
+    MLP_HIP(hipSetDevice(m->device));
+    MLP_HIP(hipDeviceSynchronize());
+    const float* src = which == 0 ? m->master : (which == 1 ? m->grad : m->fourier);
+    const int64_t cnt = which == 2 ? (int64_t)m->cfg.n_fourier * 2 : m->n_params;
+    if (which < 0 || which > 2 || n != cnt) return mfail(LYAP_ERR_INVALID, "lyap_mlp_get: size %lld", (long long)cnt);
+    MLP_HIP(hipMemcpy(host, src, (size_t)cnt * 4, hipMemcpyDeviceToHost));
+    return LYAP_SUCCESS;
+}
+
+int lyap_mlp_set_params(lyap_mlp* m, const float* host, int64_t n) {
+    if (!m || !host || n != m->n_params) return mfail(LYAP_ERR_INVALID, "lyap_mlp_set_params: bad size");
+    MLP_HIP(hipSetDevice(m->device));
+    MLP_HIP(hipMemcpy(m->master, host, (size_t)n * 4, hipMemcpyHostToDevice));
+    for (int l = 0; l < kLayers; ++l) {
+        const int nw = m->dims[l + 1] * m->dims[l];
+        hipLaunchKernelGGL(k_issue_bf16, dim3((nw + 255) / 256), dim3(256), 0, 0, m->master + m->off_w[l],
+                           m->dims[l + 1], m->dims[l], m->wt[l], m->wn[l]);
+    }
+    MLP_HIP(hipDeviceSynchronize());
+    return LYAP_SUCCESS;
+}
+
+}  // extern "C"

@@ -208,9 +208,8 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
         const int64_t got = grab_items(!active && !exhausted, io.counter);
         if (!active && !exhausted) {
             if (got < io.count) {
-                const lyap_sweep_desc* dp = &d;
-                __asm__ volatile("" : "+s"(dp));
-                item = got;
+                KernargDesc* dp = kernarg_desc();
+                item = item_position(io, got);
                 const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
                 point_params<P>(*dp, idx, io.params, p);
 #pragma unroll

@@ -29,7 +29,9 @@ PT_OK, PT_NONFINITE, PT_STEP_UNDERFLOW, PT_MAXSTEPS, PT_ESCAPED = 0, 1, 2, 3, 4
 EXPORTS = (
     "lyap_last_error", "lyap_ctx_create", "lyap_ctx_destroy", "lyap_model_count",
     "lyap_model_at", "lyap_model_lookup", "lyap_loop_count", "lyap_desc_finalize",
-    "lyap_sweep_run", "lyap_last_kernel_ms", "lyap_cuda_lyapunov",
+    "lyap_sweep_run", "lyap_last_kernel_ms", "lyap_cuda_lyapunov", "lyap_ctx_set_option",
+    "lyap_mlp_create", "lyap_mlp_destroy", "lyap_mlp_backward", "lyap_mlp_train_step",
+    "lyap_mlp_forward", "lyap_mlp_param_count", "lyap_mlp_get", "lyap_mlp_set_params",
 )
 
 
@@ -75,6 +77,17 @@ class ModelInfo(ctypes.Structure):
     ]
 
 
+class MlpConfig(ctypes.Structure):
+    _fields_ = [
+        ("max_batch", ctypes.c_int32), ("hidden", ctypes.c_int32), ("n_fourier", ctypes.c_int32),
+        ("reserved0", ctypes.c_int32), ("fourier_sigma", ctypes.c_float), ("lr", ctypes.c_float),
+        ("beta1", ctypes.c_float), ("beta2", ctypes.c_float), ("eps", ctypes.c_float),
+        ("reserved1", ctypes.c_float), ("seed", ctypes.c_uint64),
+        ("lo1", ctypes.c_double), ("hi1", ctypes.c_double), ("lo2", ctypes.c_double),
+        ("hi2", ctypes.c_double), ("y_mean", ctypes.c_double * 3), ("y_std", ctypes.c_double * 3),
+    ]
+
+
 class LyapError(RuntimeError):
     def __init__(self, code: int, msg: str):
         super().__init__(f"[{code}] {msg}")
@@ -105,6 +118,15 @@ def load(path: str = LIB_PATH):
     L.lyap_desc_finalize.argtypes = [P(SweepDesc)]
     L.lyap_sweep_run.argtypes = [vp, P(SweepDesc), i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     L.lyap_last_kernel_ms.argtypes = [vp, P(ctypes.c_float)]
+    L.lyap_ctx_set_option.argtypes = [vp, ctypes.c_char_p, i64]
+    L.lyap_mlp_create.argtypes = [ctypes.c_int, P(MlpConfig), P(vp)]
+    L.lyap_mlp_destroy.argtypes = [vp]
+    L.lyap_mlp_backward.argtypes = [vp, vp, vp, vp, i64, vp, vp]
+    L.lyap_mlp_train_step.argtypes = [vp, vp, vp, vp, i64, vp, vp]
+    L.lyap_mlp_forward.argtypes = [vp, vp, i64, vp, vp]
+    L.lyap_mlp_param_count.argtypes = [vp, P(i64)]
+    L.lyap_mlp_get.argtypes = [vp, ctypes.c_int, vp, i64]
+    L.lyap_mlp_set_params.argtypes = [vp, vp, i64]
     L.lyap_cuda_lyapunov.argtypes = [
         vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         vp, i32, i32, vp, i32, d, d, vp, i32, d, d, d, d, vp]
@@ -206,6 +228,9 @@ class Context:
                                     _ptr(st), _ptr(na), _ptr(nr), ctypes.c_void_p(s.cuda_stream)))
         return lam, st, na, nr
 
+    def set_option(self, key: str, value: int) -> None:
+        check(load().lyap_ctx_set_option(self._h, key.encode(), int(value)))
+
     def kernel_ms(self) -> float:
         ms = ctypes.c_float()
         check(load().lyap_last_kernel_ms(self._h, ctypes.byref(ms)))
@@ -223,3 +248,65 @@ class Context:
             int(dimension), int(model), _ptr(params), int(params_dim), float(eps), float(h),
             _ptr(lyap_exps), int(exps_num), float(time_skip), float(time_skip_clp),
             float(calc_time), float(add_skip_steps), ctypes.c_void_p(s.cuda_stream)))
+
+
+class Mlp:
+    """The surrogate network object of liblyap.so (lyap_mlp_*)."""
+
+    def __init__(self, device: int = 0, max_batch: int = 4096, hidden: int = 256,
+                 n_fourier: int = 32, fourier_sigma: float = 6.0, lr: float = 1e-3,
+                 beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8, seed: int = 1,
+                 ranges=(0.0, 1.0, 0.0, 1.0), y_mean=(0.0, 0.0, 0.0), y_std=(1.0, 1.0, 1.0)):
+        c = MlpConfig()
+        c.max_batch, c.hidden, c.n_fourier = max_batch, hidden, n_fourier
+        c.fourier_sigma, c.lr, c.beta1, c.beta2, c.eps = fourier_sigma, lr, beta1, beta2, eps
+        c.seed = seed
+        c.lo1, c.hi1, c.lo2, c.hi2 = ranges
+        for j in range(3):
+            c.y_mean[j], c.y_std[j] = y_mean[j], y_std[j]
+        self.cfg = c
+        self.device = device
+        self._h = ctypes.c_void_p()
+        check(load().lyap_mlp_create(int(device), ctypes.byref(c), ctypes.byref(self._h)))
+        n = ctypes.c_int64()
+        check(load().lyap_mlp_param_count(self._h, ctypes.byref(n)))
+        self.n_params = int(n.value)
+
+    def close(self):
+        if self._h:
+            load().lyap_mlp_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _stream(self, stream):
+        import torch
+        return stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", self.device))
+
+    def backward(self, p, lam, status, loss=None, stream=None):
+        check(load().lyap_mlp_backward(self._h, _ptr(p), _ptr(lam), _ptr(status), int(p.shape[0]),
+                                       _ptr(loss), ctypes.c_void_p(self._stream(stream).cuda_stream)))
+
+    def train_step(self, p, lam, status, loss=None, stream=None):
+        check(load().lyap_mlp_train_step(self._h, _ptr(p), _ptr(lam), _ptr(status), int(p.shape[0]),
+                                         _ptr(loss), ctypes.c_void_p(self._stream(stream).cuda_stream)))
+
+    def forward(self, p, out, stream=None):
+        check(load().lyap_mlp_forward(self._h, _ptr(p), int(p.shape[0]), _ptr(out),
+                                      ctypes.c_void_p(self._stream(stream).cuda_stream)))
+
+    def get(self, which: int):
+        import numpy as np
+        n = self.cfg.n_fourier * 2 if which == 2 else self.n_params
+        out = np.empty(n, dtype=np.float32)
+        check(load().lyap_mlp_get(self._h, int(which), out.ctypes.data_as(ctypes.c_void_p), n))
+        return out
+
+    def set_params(self, host):
+        import numpy as np
+        host = np.ascontiguousarray(host, dtype=np.float32)
+        check(load().lyap_mlp_set_params(self._h, host.ctypes.data_as(ctypes.c_void_p), host.size))

@@ -147,6 +147,16 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
                    double* d_lambda, uint32_t* d_status, uint32_t* d_n_acc, uint32_t* d_n_rej,
                    void* stream);
 
+/*
+ * Launch-shape options of a context (no counterpart in the reference, whose
+ * launch shape was fixed at cuda/lyapunov.dp.cpp:331-332); set by callers or
+ * by the autotuner.  Keys:
+ *   "l96_waves_8", "l96_waves_16", "l96_waves_32": wave64s per point of the
+ *       Lorenz-96 kernel (N=8: 1; N=16: 1, 2, 4; N=32: 2, 4; 0 = default)
+ *   "lane_max_blocks": cap on the persistent grid of the 3-D kernels (0 = none)
+ */
+int lyap_ctx_set_option(lyap_ctx* ctx, const char* key, int64_t value);
+
 /* duration of the last sweep kernel launched by lyap_sweep_run on ctx
    (HIP events on the launch stream; valid once that stream has drained) */
 int lyap_last_kernel_ms(lyap_ctx* ctx, float* ms);
@@ -167,6 +177,44 @@ int lyap_cuda_lyapunov(lyap_ctx* ctx, int block_num, int block_size, size_t task
                        double integration_step, double* d_lyap_exps, int32_t exps_num,
                        double time_skip, double time_skip_clp, double calc_time,
                        double add_skip_steps, void* stream);
+
+/*
+ * Surrogate network (no counterpart in the reference): (p1, p2) -> 2*n_fourier
+ * Fourier features -> three hidden layers of `hidden` GELU units ->
+ * [mu1, mu2, mu3, log-variance of lambda1, escape logit].  BF16 weights with
+ * an FP32 master copy, FP32 accumulation on MFMA, Adam.  Inputs are device
+ * arrays: p as [n][2] doubles, lambda as [n][3] doubles, status as [n] words
+ * (non-zero: the point escaped or failed; its lambdas are not used).
+ */
+typedef struct lyap_mlp_config {
+    int32_t max_batch;    /* rows per call, a multiple of 64 */
+    int32_t hidden;       /* a multiple of 64 */
+    int32_t n_fourier;    /* 2*n_fourier a multiple of 16 */
+    int32_t reserved0;
+    float fourier_sigma;  /* std of the Fourier frequencies (inputs scaled to [0,1]) */
+    float lr, beta1, beta2, eps;
+    float reserved1;
+    uint64_t seed;
+    double lo1, hi1, lo2, hi2;       /* input ranges */
+    double y_mean[3], y_std[3];      /* target normalisation */
+} lyap_mlp_config;
+
+typedef struct lyap_mlp lyap_mlp;
+
+int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out);
+int lyap_mlp_destroy(lyap_mlp* m);
+/* forward + backward into the gradient buffer (no update); mean loss to *d_loss */
+int lyap_mlp_backward(lyap_mlp* m, const double* d_p, const double* d_lambda,
+                      const uint32_t* d_status, int64_t n, float* d_loss, void* stream);
+/* lyap_mlp_backward followed by one Adam step */
+int lyap_mlp_train_step(lyap_mlp* m, const double* d_p, const double* d_lambda,
+                        const uint32_t* d_status, int64_t n, float* d_loss, void* stream);
+/* d_out [n][5]: mu1, mu2, mu3 (lambda units), std of lambda1, escape probability */
+int lyap_mlp_forward(lyap_mlp* m, const double* d_p, int64_t n, float* d_out, void* stream);
+int lyap_mlp_param_count(lyap_mlp* m, int64_t* n);
+/* which: 0 parameters, 1 gradients, 2 Fourier frequencies [n_fourier][2] (host copy) */
+int lyap_mlp_get(lyap_mlp* m, int which, float* host, int64_t n);
+int lyap_mlp_set_params(lyap_mlp* m, const float* host, int64_t n);
 
 #ifdef __cplusplus
 }

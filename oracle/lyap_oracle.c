@@ -6,8 +6,10 @@
  *   - built with -ffp-contract=off; every fused multiply-add is an explicit fma();
  *   - RK stage sums are accumulated in increasing stage order, first term a
  *     product, later terms fma(a_j, k_j, acc), then fma(h, acc, y);
- *   - sums over components are sequential fma chains for the 3-D models and a
- *     xor-butterfly (offsets n/2, n/4, ..., 1) over plain products for Lorenz-96;
+ *   - dot products are sequential fma chains; the error norm of a step is a
+ *     sequential fma chain for the 3-D models and a xor-butterfly (offsets
+ *     n/2, n/4, ..., 1) over plain products for Lorenz-96, whose components
+ *     live in different GPU lanes;
  *   - log/exp/pow are the deterministic routines below (no libm), so every
  *     branch the adaptive controller takes is reproducible bit for bit.
  *
@@ -691,7 +693,10 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
     uint64_t n_acc = 0, n_rej = 0;
     double t_avg_exact;
 
-#define ORC_RENORM() orc_mgs(n, k, tree, V, ln)
+/* MGS dot products are sequential for every model (the GPU stages the
+   deviation matrix in LDS and gives each vector to one lane); the step's
+   error norm uses the butterfly for Lorenz-96 */
+#define ORC_RENORM() orc_mgs(n, k, 0, V, ln)
 
     if (!d->adaptive) {
         const int64_t R = llround(d->t_renorm / d->h);

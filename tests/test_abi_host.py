@@ -112,6 +112,11 @@ def test_descriptor_layout_matches_oracle():
         assert a == b and getattr(L.SweepDesc, a).offset == getattr(O.OrcDesc, b).offset
 
 
+def test_set_option_validation():
+    lib = L.load()
+    assert lib.lyap_ctx_set_option(None, b"l96_waves_16", 2) == L.ERR_INVALID
+
+
 def test_null_arguments_return_codes():
     lib = L.load()
     assert lib.lyap_ctx_create(0, None) == L.ERR_INVALID

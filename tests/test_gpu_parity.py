@@ -79,3 +79,23 @@ def test_reference_modes_match_oracle(gpu_ctx, model, method):
         spec.update(W.grid_fields(0, 0.2, 2.0, 4, 1, 0.0, 1.2, 4))
     g, o = run_both(gpu_ctx, spec)
     assert_match(g, o, TOL_FP64, exact=True)
+
+
+L96_LAYOUTS = [(8, 1), (16, 1), (16, 2), (16, 4), (32, 2), (32, 4)]
+
+
+@pytest.mark.parametrize("n,waves", L96_LAYOUTS)
+@pytest.mark.parametrize("integ,adaptive,prec", [(L.INT_DOPRI5, 1, L.PREC_FP64),
+                                                 (L.INT_DOPRI5, 1, L.PREC_MIXED),
+                                                 (L.INT_DVERK, 1, L.PREC_FP64),
+                                                 (L.INT_DOPRI5, 0, L.PREC_FP64),
+                                                 (L.INT_DVERK, 0, L.PREC_MIXED)])
+def test_l96_matches_oracle(gpu_ctx, n, waves, integ, adaptive, prec):
+    spec = W.lorenz96_dopri(n=3, dim=n, precision=prec, t_transient=2.0, t_average=6.0)
+    spec.update(integrator=integ, adaptive=adaptive)
+    gpu_ctx.set_option(f"l96_waves_{n}", waves)
+    try:
+        g, o = run_both(gpu_ctx, spec)
+    finally:
+        gpu_ctx.set_option(f"l96_waves_{n}", 0)
+    assert_match(g, o, TOL_FP64, exact=True)
