@@ -2,22 +2,29 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
 
-A step is one full sweep of the chosen form over the workload's grid, with
-every input generated on the device (parameters from the grid rule, initial
-states from the seeded generator), so nothing crosses PCIe in the timed
-region.  The same grid is also swept once in plain FP64 (the form that
-matches the CPU FP64 oracle bit for bit) for the speed-up and the |dlambda1|
-check.  Prints one JSON line.
+Workloads (every input generated on the device from the grid rule and the
+seeded initial-state rule, so nothing crosses PCIe in the timed region):
 
-Roofline: the sweep is bound by the vector ALU (FP64 for the plain form);
-`achieved` is the algorithmic flop count of the kernel launch (from the
-per-point accepted/rejected step counters the kernel writes, see
-chaos_oneapi_amd/flops.py) divided by the launch's duration measured with HIP
-events on the launch stream.
+  rossler-2048-surrogate (default)  Rossler (b=0.2) over a 2048x2048 (a, c) plane,
+      adaptive DVERK 6(5) rtol=atol=1e-10, transient 200, averaging 2000, MGS every
+      0.5.  A step is the chosen form end to end: a 256x256 training map in the
+      mixed form, surrogate training (BF16 MFMA, Adam), inference over the plane,
+      mixed-form reruns where the surrogate is uncertain or |mu1| < 0.02, and the
+      FP64 judge on a seeded 1 % verification subsample.
+  rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
+  lorenz-64     Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01; one mixed-form sweep.
 
-cpu_baseline: the CPU FP64 oracle (oracle/lyap_oracle.c, OpenMP, one point
-per iteration) timed on a bounded seeded subsample of the same grid; its
-values double as the parity check of that subsample.
+The plain FP64 sweep of the same grid is timed once for the speed-up.  Prints
+one JSON line.
+
+Roofline: the sweep is bound by the vector ALU (FP64, and FP32 without packed
+instructions, both 78.6 TFLOP/s on MI355X); `achieved` is the algorithmic flop
+count of the dominant kernel launch (per-point accepted/rejected step counters
+x per-step flops, chaos_oneapi_amd/flops.py) over that launch's duration from
+HIP events on its stream.
+
+cpu_baseline: the CPU FP64 oracle (oracle/lyap_oracle.c, OpenMP) timed on a
+bounded seeded subsample of the same grid; its values double as a parity check.
 """
 from __future__ import annotations
 
@@ -32,22 +39,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("param points/sec on 1 GPU at |Δλ1|≤1e-3 vs FP64 oracle; "
           "chosen form vs plain FP64")
-PEAK_FP64_VALU_TFLOPS = 78.6  # MI355X vector FP64 (also the FP32 rate without packed ops)
-CHAOS_THRESHOLD = 0.01  # lambda1 above it: chaotic
+PEAK_FP64_VALU_TFLOPS = 78.6
+CHAOS_THRESHOLD = 0.01
 
 
-def workload(name: str):
-    from chaos_oneapi_amd import lyap as L
-    from chaos_oneapi_amd import workloads as W
-
-    if name == "rossler-512":
-        return W.rossler_dverk(512, precision=L.PREC_MIXED), "rossler-512x512-dverk65-adaptive"
-    if name == "lorenz-64":
-        return W.lorenz_fixed(64, precision=L.PREC_MIXED), "lorenz-64x64-dopri5-fixed"
-    raise SystemExit(f"unknown workload {name}")
-
-
-def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int):
+def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int, idx_pool=None):
     """Time the CPU oracle on a seeded subsample; returns (record, parity dict)."""
     import numpy as np
 
@@ -57,13 +53,13 @@ def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int):
     od.precision = 0
     n = desc.n1 * desc.n2
     rng = np.random.default_rng(12345)
-    order = rng.permutation(n)
+    order = rng.permutation(n) if idx_pool is None else rng.permutation(idx_pool)
     done, t_cpu, chunk = 0, 0.0, max(threads, 16)
     idx_all, lam_all = [], []
-    while t_cpu < budget_s and done < n:
+    while t_cpu < budget_s and done < len(order):
         idx = order[done:done + chunk]
         t0 = time.perf_counter()
-        lam_o, st_o, _, _ = O.sweep(od, idx_list=idx, n_threads=threads)
+        lam_o, _, _, _ = O.sweep(od, idx_list=idx, n_threads=threads)
         t_cpu += time.perf_counter() - t0
         idx_all.append(idx)
         lam_all.append(lam_o)
@@ -71,52 +67,53 @@ def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int):
         chunk = min(4 * chunk, 4096)
     idx = np.concatenate(idx_all)
     lam_o = np.concatenate(lam_all)
-    ok = np.isfinite(lam_o[:, 0]) & np.isfinite(lam_chosen[idx, 0])
-    d_chosen = float(np.max(np.abs(lam_chosen[idx][ok, 0] - lam_o[ok, 0]))) if ok.any() else 0.0
-    d_fp64 = float(np.max(np.abs(lam_fp64[idx][ok] - lam_o[ok]))) if ok.any() else 0.0
     rec = {"value": done / t_cpu, "unit": "points/s", "cores": threads, "kind": "port",
            "sample": f"{done} seeded random points of the same grid, FP64 oracle, "
                      f"OpenMP {threads} threads, {t_cpu:.1f} s"}
-    parity = {"oracle_sample": int(done), "max_dlambda1_chosen_vs_oracle": d_chosen,
-              "max_dlambda_fp64_vs_oracle": d_fp64}
+    parity = {"oracle_sample": int(done)}
+    if lam_fp64 is not None:
+        ok = np.isfinite(lam_o[:, 0]) & np.isfinite(lam_fp64[idx, 0])
+        parity["max_dlambda_fp64_vs_oracle"] = float(np.max(np.abs(lam_fp64[idx][ok] - lam_o[ok]))) if ok.any() else 0.0
+    if lam_chosen is not None:
+        ok = np.isfinite(lam_o[:, 0]) & np.isfinite(lam_chosen[idx, 0])
+        parity["max_dlambda1_chosen_vs_oracle"] = float(np.max(np.abs(lam_chosen[idx][ok, 0] - lam_o[ok, 0]))) if ok.any() else 0.0
     return rec, parity
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="rossler-512")
-    ap.add_argument("--cpu-budget", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--no-cpu", action="store_true")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every point is independent and this build runs on a "
-              "single MI355X; multi-GPU sharding is described in DESIGN.md, not launched.",
-              file=sys.stderr)
-        sys.exit(2)
+def plain_fp64(ctx, desc):
+    """One plain FP64 sweep of the grid: (lam, status, kernel ms, flops)."""
+    import torch
 
+    from chaos_oneapi_amd import flops as F
+    from chaos_oneapi_amd import lyap as L
+
+    plain = L.SweepDesc.from_buffer_copy(desc)
+    plain.precision = L.PREC_FP64
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    lam, st, na, nr = ctx.run(plain)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    return (lam.cpu().numpy(), st.cpu().numpy(), ms,
+            F.sweep_flops(plain, na.cpu().numpy(), nr.cpu().numpy()))
+
+
+def bench_sweep(args, ctx, spec, wl_name):
     import numpy as np
     import torch
 
     from chaos_oneapi_amd import flops as F
     from chaos_oneapi_amd import lyap as L
 
-    spec, wl_name = workload(args.workload)
     desc = L.finalize(L.make_desc(spec))
-    plain = L.SweepDesc.from_buffer_copy(desc)
-    plain.precision = L.PREC_FP64
     n = desc.n1 * desc.n2
-    ctx = L.Context(0)
     stream = torch.cuda.current_stream()
     dev = torch.device("cuda", 0)
     bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
-            torch.empty(n, dtype=torch.int32, device=dev),
-            torch.empty(n, dtype=torch.int32, device=dev),
+            torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev),
             torch.empty(n, dtype=torch.int32, device=dev))
-
     for _ in range(args.warmup):
         ctx.run(desc, out=bufs)
     torch.cuda.synchronize()
@@ -129,64 +126,147 @@ def main():
         ev[i][1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kern_ms = [a.elapsed_time(b) for a, b in ev]
+    kern_avg = sum(a.elapsed_time(b) for a, b in ev) / args.steps
     ms_per_step = wall * 1e3 / args.steps
-    lam_c = bufs[0].cpu().numpy().copy()
-    st_c = bufs[1].cpu().numpy().copy()
+    lam_c, st_c = bufs[0].cpu().numpy().copy(), bufs[1].cpu().numpy().copy()
     na_c, nr_c = bufs[2].cpu().numpy().copy(), bufs[3].cpu().numpy().copy()
     flops_c = F.sweep_flops(desc, na_c, nr_c)
-
-    # plain FP64 on the same grid
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    ctx.run(plain, out=bufs)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    plain_ms = e0.elapsed_time(e1)
-    lam_p = bufs[0].cpu().numpy().copy()
-    st_p = bufs[1].cpu().numpy().copy()
-    flops_p = F.sweep_flops(plain, bufs[2].cpu().numpy(), bufs[3].cpu().numpy())
-
+    lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
     ok = (st_c == 0) & (st_p == 0)
     d1 = float(np.max(np.abs(lam_c[ok, 0] - lam_p[ok, 0]))) if ok.any() else 0.0
     cls = float(np.mean((lam_c[ok, 0] > CHAOS_THRESHOLD) == (lam_p[ok, 0] > CHAOS_THRESHOLD)))
-    kern_avg = sum(kern_ms) / len(kern_ms)
     achieved = flops_c / (kern_avg * 1e-3) / 1e12
-
-    rec = {
-        "metric": METRIC,
-        "value": n / (ms_per_step * 1e-3),
-        "unit": "points/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 stages + f64 state" if desc.precision == L.PREC_MIXED else "f64",
-        "data": "synthetic (seeded grid and initial states generated on the device)",
-        "config": {
-            "workload": wl_name,
-            "points": n,
-            "chosen_form": "mixed" if desc.precision == L.PREC_MIXED else "fp64",
-            "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
-            "speedup_vs_plain_fp64": plain_ms / kern_avg,
-            "max_dlambda1_chosen_vs_plain_fp64": d1,
-            "classification_agreement": cls,
-            "points_status_ok": int(ok.sum()),
-            "mean_steps_per_point": float(na_c.mean()),
-            "mean_rejects_per_point": float(nr_c.mean()),
-            "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
-        },
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_VALU_TFLOPS,
-                     "traffic": None},
+    rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
+    rec["config"] = {
+        "workload": wl_name, "points": n,
+        "chosen_form": "mixed" if desc.precision == L.PREC_MIXED else "fp64",
+        "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
+        "speedup_vs_plain_fp64": plain_ms / ms_per_step,
+        "max_dlambda1_chosen_vs_plain_fp64": d1,
+        "points_over_1e-3": int(np.sum(np.abs(lam_c[ok, 0] - lam_p[ok, 0]) > 1e-3)),
+        "classification_agreement": cls, "points_status_ok": int(ok.sum()),
+        "mean_steps_per_point": float(na_c.mean()), "mean_rejects_per_point": float(nr_c.mean()),
+        "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
+    rec["roofline"] = roofline(achieved, "k_var_lane (mixed) over the grid")
     if not args.no_cpu:
         cpu, parity = cpu_baseline(desc, lam_c, lam_p, args.cpu_budget, args.cpu_threads)
         rec["cpu_baseline"] = cpu
         rec["config"].update(parity)
+    return rec
+
+
+def bench_surrogate(args, ctx, spec, wl_name):
+    import numpy as np
+    import torch
+
+    from chaos_oneapi_amd import flops as F
+    from chaos_oneapi_amd import lyap as L
+    from chaos_oneapi_amd import refine as R
+
+    desc = L.finalize(L.make_desc(spec))
+    n = desc.n1 * desc.n2
+    kw = dict(train_n=args.train_n, epochs=args.epochs, fourier_sigma=args.fourier_sigma)
+    for _ in range(args.warmup):
+        R.refine(ctx, spec, **kw)
+    torch.cuda.synchronize()
+    times, res = [], None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = R.refine(ctx, spec, **kw)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms_per_step = 1e3 * sum(times) / len(times)
+    st = res.stats
+    lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
+    # the final map against the plain FP64 map over the whole plane
+    okp = st_p == 0
+    cls_final = R.classify(np.nan_to_num(res.lam[:, 0], nan=0.0), res.status)
+    cls_plain = R.classify(lam_p[:, 0], st_p)
+    computed = res.rerun & okp & (res.status == 0)
+    rerun_flops = F.per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
+    fl = (rerun_flops[0] * st["rerun_acc_steps"] + rerun_flops[1] * st["rerun_rej_steps"]
+          + F.mgs(desc.dim, desc.dim) * (desc.n_renorm_transient + desc.n_renorm_average) * st["rerun"])
+    achieved = fl / (st["rerun_kernel_ms"] * 1e-3) / 1e12
+    rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
+    rec["config"] = {
+        "workload": wl_name, "points": n, "chosen_form": "mixed + surrogate refinement",
+        "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
+        "speedup_vs_plain_fp64": plain_ms / ms_per_step,
+        "rerun_fraction": st["rerun_fraction"], "train_points": st["train_points"],
+        "verify_points": st["verify"],
+        "max_dlambda1_chosen_vs_fp64_verify": st.get("max_dlambda1_chosen_vs_fp64"),
+        "class_agreement_verify": st.get("class_agreement"),
+        "surrogate_answered_verify": st.get("surrogate_answered_verify"),
+        "surrogate_class_agreement_verify": st.get("surrogate_class_agreement"),
+        "max_dlambda1_computed_vs_plain_fp64": float(np.max(np.abs(res.lam[computed, 0] - lam_p[computed, 0]))) if computed.any() else 0.0,
+        "class_agreement_full_plane": float(np.mean(cls_final == cls_plain)),
+        "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
+        "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
+    }
+    rec["roofline"] = roofline(achieved, "k_var_lane (mixed) rerun launch")
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(desc, None, lam_p, args.cpu_budget, args.cpu_threads)
+        rec["cpu_baseline"] = cpu
+        rec["config"].update(parity)
+    return rec
+
+
+def base_record(args, value, ms_per_step, desc):
+    from chaos_oneapi_amd import lyap as L
+
+    return {
+        "metric": METRIC, "value": value, "unit": "points/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 stages + f64 state" if desc.precision == L.PREC_MIXED else "f64",
+        "data": "synthetic (seeded grid and initial states generated on the device)",
+    }
+
+
+def roofline(achieved, kernel):
+    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": None, "kernel": kernel}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="rossler-2048-surrogate")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--train-n", type=int, default=256)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--fourier-sigma", type=float, default=6.0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every point is independent and this build runs on a "
+              "single MI355X; multi-GPU sharding is described in DESIGN.md, not launched.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from chaos_oneapi_amd import lyap as L
+    from chaos_oneapi_amd import workloads as W
+
+    ctx = L.Context(0)
+    if args.workload == "rossler-2048-surrogate":
+        args.steps = 2 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        rec = bench_surrogate(args, ctx, W.rossler_dverk(2048, precision=L.PREC_MIXED),
+                              "rossler-2048x2048-dverk65-adaptive-surrogate")
+    else:
+        args.steps = 3 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        if args.workload == "rossler-512":
+            spec, name = W.rossler_dverk(512, precision=L.PREC_MIXED), "rossler-512x512-dverk65-adaptive"
+        elif args.workload == "lorenz-64":
+            spec, name = W.lorenz_fixed(64, precision=L.PREC_MIXED), "lorenz-64x64-dopri5-fixed"
+        else:
+            raise SystemExit(f"unknown workload {args.workload}")
+        rec = bench_sweep(args, ctx, spec, name)
     print(json.dumps(rec), flush=True)
     ctx.close()
 

@@ -297,19 +297,34 @@ LYAP_D T coef_sum_e(const RowE<TB, T, LD>& row, const T (&k)[LD][N], int i) {
 // models: f(x) and J(x) v.  T is double, or float for the mixed tangent (the
 // caller rounds the state and parameters to float first).
 // ---------------------------------------------------------------------------
-struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27
+// Each model gives f(x), its Jacobian data at a stage state and J v from that
+// data.  The Jacobian data are the entries that depend on the state; entries
+// that are differences of state and parameter are formed in FP64 and rounded
+// once to the tangent type T (float in the mixed form), so they do not lose
+// digits to cancellation in binary32.  With T = double this is exactly the
+// FP64 Jacobian.  jnorm(x) = max row sum of |J(x)| caps adaptive steps at
+// h*jnorm <= 1, which keeps the tangent integration accurate where the base
+// error estimate vanishes (stable fixed points).
+struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27; (alpha, lambda)
     static constexpr int ID = LYAP_MODEL_SHIMIZU, N = 3, P = 2;
     template <class T>
-    LYAP_D static void rhs(const T* x, const T* p, T* o) {
+    struct J {
+        T x, oz;  // x, 1 - z
+    };
+    LYAP_D static void rhs(const double* x, const double* p, double* o) {
         o[0] = x[1];
         o[1] = fma(-p[0], x[1], fma(-x[0], x[2], x[0]));
         o[2] = fma(x[0], x[0], -p[1] * x[2]);
     }
     template <class T>
-    LYAP_D static void jvp(const T* x, const T* p, const T* v, T* o) {
+    LYAP_D static J<T> jac(const double* x, const double* p) {
+        return J<T>{(T)x[0], (T)(1.0 - x[2])};
+    }
+    template <class T>
+    LYAP_D static void jvp(const J<T>& j, const T* p, const T* v, T* o) {
         o[0] = v[1];
-        o[1] = fma(T(1) - x[2], v[0], fma(-p[0], v[1], -x[0] * v[2]));
-        o[2] = fma(T(2) * x[0], v[0], -p[1] * v[2]);
+        o[1] = fma(j.oz, v[0], fma(-p[0], v[1], -j.x * v[2]));
+        o[2] = fma(T(2) * j.x, v[0], -p[1] * v[2]);
     }
     LYAP_D static double jnorm(const double* x, const double* p) {
         const double r0 = 1.0;
@@ -320,22 +335,26 @@ struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27
     }
 };
 
-// Each model also gives jnorm(x) = max row sum of |J(x)|: adaptive steps are
-// capped at h*jnorm <= 1, which keeps the tangent integration accurate where
-// the base error estimate vanishes (stable fixed points).
 struct Lorenz {  // (sigma, rho, beta)
     static constexpr int ID = LYAP_MODEL_LORENZ, N = 3, P = 3;
     template <class T>
-    LYAP_D static void rhs(const T* x, const T* p, T* o) {
+    struct J {
+        T x, y, rz;  // x, y, rho - z
+    };
+    LYAP_D static void rhs(const double* x, const double* p, double* o) {
         o[0] = p[0] * (x[1] - x[0]);
         o[1] = fma(x[0], p[1] - x[2], -x[1]);
         o[2] = fma(x[0], x[1], -p[2] * x[2]);
     }
     template <class T>
-    LYAP_D static void jvp(const T* x, const T* p, const T* v, T* o) {
+    LYAP_D static J<T> jac(const double* x, const double* p) {
+        return J<T>{(T)x[0], (T)x[1], (T)(p[1] - x[2])};
+    }
+    template <class T>
+    LYAP_D static void jvp(const J<T>& j, const T* p, const T* v, T* o) {
         o[0] = p[0] * (v[1] - v[0]);
-        o[1] = fma(p[1] - x[2], v[0], -fma(x[0], v[2], v[1]));
-        o[2] = fma(x[1], v[0], fma(x[0], v[1], -p[2] * v[2]));
+        o[1] = fma(j.rz, v[0], -fma(j.x, v[2], v[1]));
+        o[2] = fma(j.y, v[0], fma(j.x, v[1], -p[2] * v[2]));
     }
     LYAP_D static double jnorm(const double* x, const double* p) {
         const double r0 = 2.0 * fabs(p[0]);
@@ -349,16 +368,23 @@ struct Lorenz {  // (sigma, rho, beta)
 struct Rossler {  // (a, c, b)
     static constexpr int ID = LYAP_MODEL_ROSSLER, N = 3, P = 3;
     template <class T>
-    LYAP_D static void rhs(const T* x, const T* p, T* o) {
+    struct J {
+        T z, xc;  // z, x - c
+    };
+    LYAP_D static void rhs(const double* x, const double* p, double* o) {
         o[0] = -(x[1] + x[2]);
         o[1] = fma(p[0], x[1], x[0]);
         o[2] = fma(x[2], x[0] - p[1], p[2]);
     }
     template <class T>
-    LYAP_D static void jvp(const T* x, const T* p, const T* v, T* o) {
+    LYAP_D static J<T> jac(const double* x, const double* p) {
+        return J<T>{(T)x[2], (T)(x[0] - p[1])};
+    }
+    template <class T>
+    LYAP_D static void jvp(const J<T>& j, const T* p, const T* v, T* o) {
         o[0] = -(v[1] + v[2]);
         o[1] = fma(p[0], v[1], v[0]);
-        o[2] = fma(x[2], v[0], (x[0] - p[1]) * v[2]);
+        o[2] = fma(j.z, v[0], j.xc * v[2]);
     }
     LYAP_D static double jnorm(const double* x, const double* p) {
         const double r0 = 2.0;

@@ -164,7 +164,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                     const double xp1 = __shfl(arg, src_p1), xm1 = __shfl(arg, src_m1),
                                  xm2 = __shfl(arg, src_m2);
                     jm1[s] = (T)xm1;
-                    jd[s] = (T)xp1 - (T)xm2;
+                    jd[s] = (T)(xp1 - xm2);
                     if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
                     else k[s] = fma(xp1 - xm2, xm1, fma(-dd, arg, F));
                 }

@@ -26,7 +26,7 @@ namespace lyap {
 // ---------------------------------------------------------------------------
 template <class M, int INTEG, bool ERR, class TX>
 LYAP_D void base_step(const double* x, const double* p, double h, double* xn, double* ev,
-                      TX (&X)[Tab<INTEG>::S][M::N], double* k1_io) {
+                      typename M::template J<TX> (&X)[Tab<INTEG>::S], double* k1_io) {
     using TB = Tab<INTEG>;
     constexpr int S = TB::S, N = M::N;
     constexpr bool FSAL = (INTEG == LYAP_INT_DOPRI5) && ERR;
@@ -45,13 +45,13 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
             bool any;
             const double acc = coef_sum_a(s, row, k, i, any);
             arg[i] = any ? fma(h, acc, x[i]) : x[i];
-            X[s][i] = (TX)arg[i];
         }
+        if (ERR || TB::for_solution(s)) X[s] = M::template jac<TX>(arg, p);
         if (s == 0 && FSAL) {
 #pragma unroll
             for (int i = 0; i < N; ++i) k[0][i] = k1_io[i];
         } else {
-            M::template rhs<double>(arg, p, k[s]);
+            M::rhs(arg, p, k[s]);
         }
     }
     {
@@ -74,8 +74,8 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
 // FP64 form: stages in double.  Mixed form: stages in binary32, increment
 // h*sum(b_j K_j) formed in binary32 and added to the FP64 vector.
 template <class M, int INTEG, bool MIXED, class TX>
-LYAP_D void tangent_step(const TX (&X)[Tab<INTEG>::S][M::N], const double* p, const float* pf,
-                         double h, double* v) {
+LYAP_D void tangent_step(const typename M::template J<TX> (&X)[Tab<INTEG>::S], const double* p,
+                         const float* pf, double h, double* v) {
     using TB = Tab<INTEG>;
     constexpr int S = TB::S, N = M::N;
     using T = typename std::conditional<MIXED, float, double>::type;
@@ -91,17 +91,16 @@ LYAP_D void tangent_step(const TX (&X)[Tab<INTEG>::S][M::N], const double* p, co
             for (int i = 0; i < N; ++i) kv[s][i] = T(0);
             continue;
         }
-        T arg[N], xs[N];
+        T arg[N];
         const RowA<TB, T, S> row(s);
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             bool any;
             const T acc = coef_sum_a(s, row, kv, i, any);
             arg[i] = any ? fma(hT, acc, vt[i]) : vt[i];
-            xs[i] = (T)X[s][i];
         }
-        if constexpr (MIXED) M::template jvp<float>(xs, pf, arg, kv[s]);
-        else M::template jvp<double>(xs, p, arg, kv[s]);
+        if constexpr (MIXED) M::template jvp<float>(X[s], pf, arg, kv[s]);
+        else M::template jvp<double>(X[s], p, arg, kv[s]);
     }
     const RowB<TB, T, S> rb;
 #pragma unroll
@@ -232,7 +231,7 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
                 ctrl.fac_old = 1e-4;
                 ctrl.log_fac_old = log_floor;
                 ctrl.rejected = false;
-                if (INTEG == LYAP_INT_DOPRI5 && ADAPT) M::template rhs<double>(x, p, k1);
+                if (INTEG == LYAP_INT_DOPRI5 && ADAPT) M::rhs(x, p, k1);
                 active = true;
             } else {
                 exhausted = true;
@@ -244,7 +243,7 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
         uint32_t st = LYAP_PT_OK;
         bool finished = false;
         double xn[N], ln[K];
-        TX X[S][N];
+        typename M::template J<TX> X[S];
         if constexpr (ADAPT) {
             if ((int64_t)nacc + (int64_t)nrej >= max_steps) {
                 st = LYAP_PT_MAXSTEPS;
@@ -361,6 +360,7 @@ template <class M>
 LYAP_D void frozen_tangent_step(const double* xf, const double* p, double h, double* v) {
     using TB = Tab<LYAP_INT_DVERK>;
     constexpr int S = TB::S, N = M::N;
+    const typename M::template J<double> jf = M::template jac<double>(xf, p);
     double kv[S][N];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -377,7 +377,7 @@ LYAP_D void frozen_tangent_step(const double* xf, const double* p, double h, dou
             const double acc = coef_sum_a(s, row, kv, i, any);
             arg[i] = any ? fma(h, acc, v[i]) : v[i];
         }
-        M::template jvp<double>(xf, p, arg, kv[s]);
+        M::template jvp<double>(jf, p, arg, kv[s]);
     }
     const RowB<TB, double, S> rb;
 #pragma unroll
@@ -387,7 +387,7 @@ LYAP_D void frozen_tangent_step(const double* xf, const double* p, double h, dou
 template <class M>
 LYAP_D void plain_step(double* x, const double* p, double h) {
     double xn[M::N];
-    double X[Tab<LYAP_INT_DVERK>::S][M::N];
+    typename M::template J<double> X[Tab<LYAP_INT_DVERK>::S];
     base_step<M, LYAP_INT_DVERK, false, double>(x, p, h, xn, nullptr, X, nullptr);
 #pragma unroll
     for (int i = 0; i < M::N; ++i) x[i] = xn[i];

@@ -124,6 +124,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     rerun_idx = torch.nonzero(rerun_mask).flatten().to(torch.int64).contiguous()
     lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
     t = mark("rerun", t)
+    rerun_kernel_ms = ctx.kernel_ms()
 
     # 5. assemble
     lam = pred[:, :3].to(torch.float64)
@@ -136,7 +137,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     stats = dict(points=n, train_points=n_t, rerun=int(rerun_idx.numel()),
                  flagged=int(flagged.sum().item()), verify=n_ver,
                  rerun_fraction=float(rerun_idx.numel()) / n, losses=losses,
-                 rerun_steps=float(na_r.double().sum().item() + nr_r.double().sum().item()))
+                 rerun_kernel_ms=rerun_kernel_ms,
+                 rerun_acc_steps=float(na_r.double().sum().item()),
+                 rerun_rej_steps=float(nr_r.double().sum().item()))
     if judge:
         t = time.perf_counter()
         pdesc = L.SweepDesc.from_buffer_copy(fd)

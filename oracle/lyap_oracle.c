@@ -229,32 +229,42 @@ double orc_jac_norm(int model, int n, const double* x, const double* p) {
     return m > r2 ? m : r2;
 }
 
-/* the same products evaluated in binary32: state and parameters are rounded
-   to float first, then every operation is a float operation */
+/* the same products with binary32 arithmetic (the mixed form): Jacobian
+   entries that are differences of state and parameter (rho - z, x - c,
+   1 - z, x_{i+1} - x_{i-2}) are formed in FP64 from the FP64 stage state and
+   rounded once; every other operand is rounded to float first and every
+   operation is a float operation */
 void orc_jvp_f32(int model, int n, const double* xd, const double* pd, const float* v, float* out) {
     float x[ORC_MAX_DIM], p[ORC_MAX_PARAMS];
     for (int i = 0; i < n; ++i) x[i] = (float)xd[i];
     for (int i = 0; i < ORC_MAX_PARAMS; ++i) p[i] = (float)pd[i];
     switch (model) {
-    case ORC_MODEL_SHIMIZU:
+    case ORC_MODEL_SHIMIZU: {
+        const float oz = (float)(1.0 - xd[2]);
         out[0] = v[1];
-        out[1] = fmaf(1.0f - x[2], v[0], fmaf(-p[0], v[1], -x[0] * v[2]));
+        out[1] = fmaf(oz, v[0], fmaf(-p[0], v[1], -x[0] * v[2]));
         out[2] = fmaf(2.0f * x[0], v[0], -p[1] * v[2]);
         break;
-    case ORC_MODEL_LORENZ:
+    }
+    case ORC_MODEL_LORENZ: {
+        const float rz = (float)(pd[1] - xd[2]);
         out[0] = p[0] * (v[1] - v[0]);
-        out[1] = fmaf(p[1] - x[2], v[0], -fmaf(x[0], v[2], v[1]));
+        out[1] = fmaf(rz, v[0], -fmaf(x[0], v[2], v[1]));
         out[2] = fmaf(x[1], v[0], fmaf(x[0], v[1], -p[2] * v[2]));
         break;
-    case ORC_MODEL_ROSSLER:
+    }
+    case ORC_MODEL_ROSSLER: {
+        const float xc = (float)(xd[0] - pd[1]);
         out[0] = -(v[1] + v[2]);
         out[1] = fmaf(p[0], v[1], v[0]);
-        out[2] = fmaf(x[2], v[0], (x[0] - p[1]) * v[2]);
+        out[2] = fmaf(x[2], v[0], xc * v[2]);
         break;
+    }
     case ORC_MODEL_L96:
         for (int i = 0; i < n; ++i) {
             int ip1 = (i + 1) % n, im1 = (i + n - 1) % n, im2 = (i + n - 2) % n;
-            out[i] = fmaf(v[ip1] - v[im2], x[im1], fmaf(x[ip1] - x[im2], v[im1], -p[1] * v[i]));
+            const float dx = (float)(xd[ip1] - xd[im2]);
+            out[i] = fmaf(v[ip1] - v[im2], x[im1], fmaf(dx, v[im1], -p[1] * v[i]));
         }
         break;
     default:
