@@ -13,6 +13,11 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
       FP64 judge on a seeded 1 % verification subsample.
   rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
   lorenz-64     Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01; one mixed-form sweep.
+  batch-mix     2^20 points (--batch-points) drawn as 40% Lorenz, 30% Rossler, 20%
+      Lorenz-96 N=8, 8% N=16, 2% N=32, adaptive Dopri5 rtol=1e-9, mixed form; a
+      step is one replay of the captured persistent batched launch, tuned first
+      by the autotuner on a 1/16 sample.  Parameters are uploaded once, before
+      the timed region.
 
 The plain FP64 sweep of the same grid is timed once for the speed-up.  Prints
 one JSON line.
@@ -212,6 +217,83 @@ def bench_surrogate(args, ctx, spec, wl_name):
     return rec
 
 
+def bench_batch(args, ctx, n_points, wl_name):
+    """The batched uneven sweep: one persistent launch over the five classes,
+    replayed from its hipGraph; tuned on a 1/16 sample by the autotuner."""
+    import numpy as np
+    import torch
+
+    from chaos_oneapi_amd import autotune as A
+    from chaos_oneapi_amd import flops as F
+    from chaos_oneapi_amd import lyap as L
+    from chaos_oneapi_amd import workloads as W
+
+    dev = torch.device("cuda", 0)
+    mix = W.batch_mix(n_points)
+
+    def device_classes(precision):
+        out = []
+        for c in mix:
+            spec = dict(c["spec"], precision=precision)
+            out.append(dict(desc=L.finalize(L.make_desc(spec)), count=c["count"],
+                            params=torch.from_numpy(c["params"]).to(dev).contiguous()))
+        return out
+
+    classes = device_classes(L.PREC_MIXED)
+    t0 = time.perf_counter()
+    tune = A.tune_batch(ctx, classes, cache=A.Cache(None))
+    tune_s = time.perf_counter() - t0
+    b = L.Batch(ctx, classes, **A.batch_kwargs(tune.best))
+    for _ in range(args.warmup):
+        b.launch()
+    torch.cuda.synchronize()
+    kern, t0 = [], time.perf_counter()
+    for _ in range(args.steps):
+        b.launch()
+        kern.append(b.last_ms())
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    outs = [tuple(t.cpu().numpy() for t in o) for o in b.outputs]
+    b.close()
+    fl = sum(F.sweep_flops(c["desc"], o[2], o[3]) for c, o in zip(classes, outs))
+    # the same classes as separate single-class launches (default shapes)
+    single_ms = 0.0
+    for c in classes:
+        ctx.run(c["desc"], count=c["count"], params=c["params"])
+        single_ms += ctx.kernel_ms()
+    # plain FP64: the same batch in FP64 (same tuning)
+    c64 = device_classes(L.PREC_FP64)
+    try:
+        b64 = L.Batch(ctx, c64, **A.batch_kwargs(tune.best))
+    except L.LyapError:
+        b64 = L.Batch(ctx, c64)
+    b64.launch()
+    plain_ms = b64.last_ms()
+    outs64 = [tuple(t.cpu().numpy() for t in o) for o in b64.outputs]
+    b64.close()
+    per_class, worst = {}, 0.0
+    for c, o, o64 in zip(mix, outs, outs64):
+        ok = (o[1] == 0) & (o64[1] == 0)
+        d1 = float(np.max(np.abs(o[0][ok, 0] - o64[0][ok, 0]))) if ok.any() else 0.0
+        worst = max(worst, d1)
+        per_class[c["name"]] = {"points": c["count"], "status_ok": int(ok.sum()),
+                                "mean_steps": float(o[2].mean()), "max_steps": int(o[2].max()),
+                                "max_dlambda1_vs_fp64": d1}
+    kern_avg = sum(kern) / len(kern)
+    rec = base_record(args, n_points / (ms_per_step * 1e-3), ms_per_step, classes[0]["desc"])
+    rec["config"] = {
+        "workload": wl_name, "points": n_points, "chosen_form": "mixed, one persistent batched launch",
+        "tuning": A.batch_kwargs(tune.best), "tuning_s": round(tune_s, 2),
+        "tuning_timings_ms_sample": {str(k): round(v, 2) for k, v in tune.timings.items()},
+        "plain_fp64_points_per_s": n_points / (plain_ms * 1e-3),
+        "speedup_vs_plain_fp64": plain_ms / ms_per_step,
+        "single_class_launches_ms": single_ms, "batched_kernel_ms": kern_avg,
+        "max_dlambda1_chosen_vs_plain_fp64": worst, "classes": per_class,
+    }
+    rec["roofline"] = roofline(fl / (kern_avg * 1e-3) / 1e12, "k_batch (mixed) persistent launch")
+    return rec
+
+
 def base_record(args, value, ms_per_step, desc):
     from chaos_oneapi_amd import lyap as L
 
@@ -241,6 +323,7 @@ def main():
     ap.add_argument("--train-n", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--fourier-sigma", type=float, default=6.0)
+    ap.add_argument("--batch-points", type=int, default=1 << 20)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a "
@@ -258,15 +341,23 @@ def main():
         rec = bench_surrogate(args, ctx, W.rossler_dverk(2048, precision=L.PREC_MIXED),
                               "rossler-2048x2048-dverk65-adaptive-surrogate")
     else:
-        args.steps = 3 if args.steps is None else args.steps
-        args.warmup = 1 if args.warmup is None else args.warmup
+        if args.workload != "batch-mix":
+            args.steps = 3 if args.steps is None else args.steps
+            args.warmup = 1 if args.warmup is None else args.warmup
         if args.workload == "rossler-512":
             spec, name = W.rossler_dverk(512, precision=L.PREC_MIXED), "rossler-512x512-dverk65-adaptive"
         elif args.workload == "lorenz-64":
             spec, name = W.lorenz_fixed(64, precision=L.PREC_MIXED), "lorenz-64x64-dopri5-fixed"
+        elif args.workload == "batch-mix":
+            spec, name = None, f"batch-mix-{args.batch_points}-dopri5-adaptive-1e-9"
         else:
             raise SystemExit(f"unknown workload {args.workload}")
-        rec = bench_sweep(args, ctx, spec, name)
+        if spec is None:
+            args.steps = 1 if args.steps is None else args.steps
+            args.warmup = 0 if args.warmup is None else args.warmup
+            rec = bench_batch(args, ctx, args.batch_points, name)
+        else:
+            rec = bench_sweep(args, ctx, spec, name)
     print(json.dumps(rec), flush=True)
     ctx.close()
 

@@ -1,0 +1,250 @@
+"""Launch-shape autotuner for the sweep kernels.
+
+Knobs, all exposed by liblyap.so:
+  - batched launch (lyap_batch_tuning): Lorenz-96 points per wavefront
+    (layout), the register budget / waves per SIMD (occupancy3), the
+    persistent grid size (waves_per_cu) and the queue order of the classes;
+  - single Lorenz-96 sweeps (lyap_ctx_set_option): the LDS tile shape, i.e.
+    one point per block of W waves (l96_waves_N) or P points per wave
+    (l96_ppw_N);
+  - single 3-D sweeps: the persistent grid size (lane_max_blocks).
+
+Each candidate is timed on a sample of the workload (a seeded subset of the
+points with the full integration times, so per-point costs keep their
+distribution), the fastest is kept, and the choice is cached in a JSON file
+keyed by the device name and a signature of the workload.  Every candidate
+computes the same results bit for bit (the layouts change which lanes do the
+work, not the arithmetic), so tuning never changes the output.
+"""
+from __future__ import annotations
+
+import json
+import os
+from dataclasses import dataclass, field
+
+from . import lyap as L
+
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "chaos_oneapi_amd", "autotune.json")
+
+# batched-launch candidates: (layout, occupancy3, waves_per_cu, order)
+BATCH_CANDIDATES = (
+    (0, False, 0, 0), (0, True, 0, 0), (1, False, 0, 0), (1, True, 0, 0),
+    (0, False, 0, 2), (0, True, 0, 2),
+)
+
+# single Lorenz-96 sweep candidates per N: ("waves", W) or ("ppw", P)
+L96_CANDIDATES = {
+    8: (("waves", 1), ("ppw", 8), ("ppw", 4), ("ppw", 2)),
+    16: (("waves", 2), ("waves", 4), ("ppw", 4), ("ppw", 2), ("ppw", 1)),
+    32: (("waves", 4), ("waves", 2), ("ppw", 1), ("ppw", 2)),
+}
+
+
+@dataclass
+class TuneResult:
+    best: tuple
+    timings: dict = field(default_factory=dict)  # candidate -> ms
+    cached: bool = False
+
+
+class Cache:
+    """A small JSON store {key: {"best": [...], "timings": {...}}}."""
+
+    def __init__(self, path: str | None = DEFAULT_CACHE):
+        self.path = path
+        self.data = {}
+        if path and os.path.exists(path):
+            try:
+                with open(path) as f:
+                    self.data = json.load(f)
+            except (OSError, ValueError):
+                self.data = {}
+
+    def get(self, key: str):
+        return self.data.get(key)
+
+    def put(self, key: str, best, timings: dict) -> None:
+        self.data[key] = {"best": list(best), "timings": {str(k): v for k, v in timings.items()}}
+        if not self.path:
+            return
+        os.makedirs(os.path.dirname(self.path), exist_ok=True)
+        tmp = self.path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(self.data, f, indent=1, sort_keys=True)
+        os.replace(tmp, self.path)
+
+
+def device_name(device: int = 0) -> str:
+    import torch
+
+    return torch.cuda.get_device_properties(device).name
+
+
+def _bucket(n: int) -> int:
+    """Counts enter the key rounded to a power of two (tuning transfers)."""
+    return 1 << max(0, int(n).bit_length() - 1)
+
+
+def desc_signature(desc) -> str:
+    return (f"m{desc.model}d{desc.dim}i{desc.integrator}a{int(desc.adaptive)}p{desc.precision}"
+            f"r{desc.rtol:.0e}t{desc.t_transient + desc.t_slave:g}+{desc.t_average:g}/{desc.t_renorm:g}")
+
+
+def batch_key(device: str, classes) -> str:
+    parts = [f"{desc_signature(c['desc'])}x{_bucket(c['count'])}" for c in classes]
+    return "batch|" + device + "|" + ";".join(parts)
+
+
+def pick_best(timings: dict):
+    """Fastest candidate; ties go to the earlier one (the default comes first)."""
+    best, best_ms = None, float("inf")
+    for cand, ms in timings.items():
+        if ms < best_ms:
+            best, best_ms = cand, ms
+    return best
+
+
+def sample_indices(count: int, frac: float, min_points: int, seed: int):
+    import numpy as np
+
+    k = min(count, max(min_points, int(round(count * frac))))
+    rng = np.random.default_rng(seed)
+    return np.sort(rng.choice(count, size=k, replace=False)) if k < count else np.arange(count)
+
+
+def sample_classes(classes, frac: float = 1 / 16, min_points: int = 256, seed: int = 5):
+    """A seeded subset of every class's points (explicit parameter rows)."""
+    import torch
+
+    out = []
+    for j, c in enumerate(classes):
+        if c.get("params") is None:
+            raise ValueError("batch tuning samples classes with explicit parameters")
+        idx = sample_indices(c["count"], frac, min_points, seed + j)
+        t = torch.as_tensor(idx, device=c["params"].device)
+        s = dict(c)
+        s["params"] = c["params"][t].contiguous()
+        s["count"] = len(idx)
+        if c.get("inits") is not None:
+            s["inits"] = c["inits"][t].contiguous()
+        out.append(s)
+    return out
+
+
+def tune_batch(ctx, classes, frac: float = 1 / 16, min_points: int = 256, repeats: int = 1,
+               cache: Cache | None = None, candidates=BATCH_CANDIDATES, force: bool = False,
+               timer=None) -> TuneResult:
+    """Pick the batched-launch tuning for `classes` (the Batch class list).
+
+    timer(cand, sample) -> ms may replace the GPU timing (host-side tests)."""
+    cache = cache if cache is not None else Cache()
+    key = batch_key(device_name(ctx.device) if timer is None else "test", classes)
+    hit = cache.get(key)
+    if hit and not force:
+        return TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
+    sample = sample_classes(classes, frac, min_points)
+    timings = {}
+    for cand in candidates:
+        try:
+            timings[cand] = timer(cand, sample) if timer else _time_batch(ctx, sample, cand, repeats)
+        except L.LyapError:
+            continue  # a variant that is not built for this integrator/precision
+    if not timings:
+        raise RuntimeError("no batch tuning candidate could run")
+    best = pick_best(timings)
+    cache.put(key, best, timings)
+    return TuneResult(best=best, timings=timings)
+
+
+def _time_batch(ctx, sample, cand, repeats: int) -> float:
+    layout, occ3, wpc, order = cand
+    b = L.Batch(ctx, sample, layout=layout, occupancy3=occ3, waves_per_cu=wpc, order=order)
+    try:
+        ms = []
+        for _ in range(repeats):
+            b.launch()
+            ms.append(b.last_ms())
+        return min(ms)
+    finally:
+        b.close()
+
+
+def batch_kwargs(best) -> dict:
+    layout, occ3, wpc, order = best
+    return dict(layout=layout, occupancy3=bool(occ3), waves_per_cu=wpc, order=order)
+
+
+def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 256, cache: Cache | None = None,
+             force: bool = False, apply: bool = True, timer=None) -> TuneResult:
+    """Pick the Lorenz-96 tile shape of a single sweep over `desc`'s grid and
+    (apply=True) set it on ctx."""
+    import torch
+
+    cache = cache if cache is not None else Cache()
+    n_total = desc.n1 * desc.n2
+    key = f"l96|{device_name(ctx.device) if timer is None else 'test'}|{desc_signature(desc)}x{_bucket(n_total)}"
+    hit = cache.get(key)
+    if hit and not force:
+        res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
+    else:
+        idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 3), dtype=torch.int64)
+        timings = {}
+        for cand in L96_CANDIDATES[desc.dim]:
+            if timer:
+                timings[cand] = timer(cand, idx)
+                continue
+            _set_l96(ctx, desc.dim, cand)
+            try:
+                ctx.run(desc, idx_list=idx.to(torch.device("cuda", ctx.device)))
+                timings[cand] = ctx.kernel_ms()
+            finally:
+                _set_l96(ctx, desc.dim, None)
+        res = TuneResult(best=pick_best(timings), timings=timings)
+        cache.put(key, res.best, timings)
+    if apply and timer is None:
+        _set_l96(ctx, desc.dim, res.best)
+    return res
+
+
+def _set_l96(ctx, dim: int, cand) -> None:
+    ctx.set_option(f"l96_ppw_{dim}", 0)
+    ctx.set_option(f"l96_waves_{dim}", 0)
+    if cand is None:
+        return
+    kind, v = cand
+    ctx.set_option(f"l96_{kind}_{dim}", int(v))
+
+
+def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1024, cache: Cache | None = None,
+               force: bool = False, apply: bool = True, timer=None) -> TuneResult:
+    """Pick the persistent grid cap of a single 3-D sweep (lane_max_blocks:
+    0 = every block that fits, else that many 256-lane blocks) and
+    (apply=True) set it on ctx."""
+    import torch
+
+    cache = cache if cache is not None else Cache()
+    n_total = desc.n1 * desc.n2
+    name = device_name(ctx.device) if timer is None else "test"
+    cus = torch.cuda.get_device_properties(ctx.device).multi_processor_count if timer is None else 256
+    key = f"lanes|{name}|{desc_signature(desc)}x{_bucket(n_total)}"
+    hit = cache.get(key)
+    if hit and not force:
+        res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
+    else:
+        idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 4), dtype=torch.int64)
+        timings = {}
+        for cand in ((0,), (2 * cus,), (cus,)):
+            if timer:
+                timings[cand] = timer(cand, idx)
+                continue
+            ctx.set_option("lane_max_blocks", cand[0])
+            try:
+                ctx.run(desc, idx_list=idx.to(torch.device("cuda", ctx.device)))
+                timings[cand] = ctx.kernel_ms()
+            finally:
+                ctx.set_option("lane_max_blocks", 0)
+        res = TuneResult(best=pick_best(timings), timings=timings)
+        cache.put(key, res.best, timings)
+    if apply and timer is None:
+        ctx.set_option("lane_max_blocks", int(res.best[0]))
+    return res

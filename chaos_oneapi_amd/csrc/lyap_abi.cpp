@@ -7,7 +7,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <string>
+#include <vector>
 
 #include "chaos_lyap.h"
 #include "lyap_launch.hpp"
@@ -106,7 +108,9 @@ struct lyap_ctx {
     int device = 0;
     int num_cus = 0;
     int l96_waves[3] = {0, 0, 0};  // waves per point for N = 8, 16, 32 (0: default)
+    int l96_ppw[3] = {0, 0, 0};    // points per wave (wave-level drain) for N = 8, 16, 32; 0: off
     int lane_max_blocks = 0;
+    int l96_max_blocks = 0;
     unsigned long long* counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -280,7 +284,9 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
     } else if (fd.model == LYAP_MODEL_L96) {
         if (fd.method != LYAP_METHOD_VAR)
             return fail(LYAP_ERR_UNSUPPORTED, "Lorenz-96 runs the coupled variational method only");
-        e = lyap::launch_l96_sweep(fd, io, s, c->num_cus, c->l96_waves[fd.dim == 8 ? 0 : (fd.dim == 16 ? 1 : 2)]);
+        const int j = fd.dim == 8 ? 0 : (fd.dim == 16 ? 1 : 2);
+        e = lyap::launch_l96_sweep(fd, io, s, c->num_cus, c->l96_waves[j], c->l96_ppw[j],
+                                   c->l96_max_blocks);
     } else {
         return fail(LYAP_ERR_UNSUPPORTED, "no kernel for model %d (dimension %d)", fd.model, fd.dim);
     }
@@ -323,9 +329,21 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
             c->l96_waves[j] = (int)value;
             return LYAP_SUCCESS;
         }
+    for (int j = 0; j < 3; ++j)
+        if (k == "l96_ppw_" + std::to_string(dims[j])) {
+            if (value != 0 && !lyap::l96_wave_supported(dims[j], (int)value))
+                return fail(LYAP_ERR_INVALID, "%s = %lld is not a built layout", key, (long long)value);
+            c->l96_ppw[j] = (int)value;
+            return LYAP_SUCCESS;
+        }
     if (k == "lane_max_blocks") {
         if (value < 0) return fail(LYAP_ERR_INVALID, "lane_max_blocks must be >= 0");
         c->lane_max_blocks = (int)value;
+        return LYAP_SUCCESS;
+    }
+    if (k == "l96_max_blocks") {
+        if (value < 0) return fail(LYAP_ERR_INVALID, "l96_max_blocks must be >= 0");
+        c->l96_max_blocks = (int)value;
         return LYAP_SUCCESS;
     }
     return fail(LYAP_ERR_INVALID, "unknown option '%s'", key);
@@ -380,6 +398,229 @@ int lyap_cuda_lyapunov(lyap_ctx* c, int block_num, int block_size, size_t task_n
     if (!d_lyap_exps) return fail(LYAP_ERR_INVALID, "lyap_cuda_lyapunov: lyapExps is NULL");
     return run_finalized(c, d, 0, (int64_t)task_num, nullptr, d_params, d_inits, d_lyap_exps,
                          nullptr, nullptr, nullptr, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched mixed-model launch
+// ---------------------------------------------------------------------------
+struct lyap_batch {
+    int device = 0;
+    int n = 0;                              // classes on the device (empty ones dropped)
+    lyap::BatchClass* d_cls = nullptr;      // n records
+    unsigned long long* d_counters = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool timed = false;
+    int64_t waves = 0;
+    lyap_batch_tuning tuning{};
+};
+
+namespace {
+
+int batch_kind(const lyap_sweep_desc& d) {
+    switch (d.model) {
+        case LYAP_MODEL_SHIMIZU: return lyap::BATCH_SHIMIZU;
+        case LYAP_MODEL_LORENZ: return lyap::BATCH_LORENZ;
+        case LYAP_MODEL_ROSSLER: return lyap::BATCH_ROSSLER;
+        case LYAP_MODEL_L96:
+            return d.dim == 8 ? lyap::BATCH_L96_8
+                              : (d.dim == 16 ? lyap::BATCH_L96_16 : (d.dim == 32 ? lyap::BATCH_L96_32 : -1));
+        default: return -1;
+    }
+}
+
+// queue position of a class kind: per-lane 3-D points are the longest tasks
+// (one lane runs a whole point), then Lorenz-96 by N descending
+int longest_first_rank(int kind) {
+    switch (kind) {
+        case lyap::BATCH_LORENZ: return 0;
+        case lyap::BATCH_ROSSLER: return 1;
+        case lyap::BATCH_SHIMIZU: return 2;
+        case lyap::BATCH_L96_32: return 3;
+        case lyap::BATCH_L96_16: return 4;
+        default: return 5;
+    }
+}
+
+void batch_free(lyap_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    if (b->capture) hipStreamDestroy(b->capture);
+    if (b->ev0) hipEventDestroy(b->ev0);
+    if (b->ev1) hipEventDestroy(b->ev1);
+    if (b->d_cls) hipFree(b->d_cls);
+    if (b->d_counters) hipFree(b->d_counters);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lyap_batch_create(lyap_ctx* c, const lyap_batch_class* classes, int n_classes,
+                      const lyap_batch_tuning* tuning, lyap_batch** out) {
+    if (!c || !classes || !out) return fail(LYAP_ERR_INVALID, "lyap_batch_create: NULL argument");
+    *out = nullptr;
+    if (n_classes < 1 || n_classes > lyap::kMaxBatchClasses)
+        return fail(LYAP_ERR_INVALID, "a batch holds 1 to %d classes, got %d", lyap::kMaxBatchClasses,
+                    n_classes);
+    lyap_batch_tuning tu{};
+    if (tuning) tu = *tuning;
+    if (tu.layout < 0 || tu.layout >= lyap::kBatchLayouts)
+        return fail(LYAP_ERR_INVALID, "batch layout %d is not built", tu.layout);
+    if (tu.waves_per_cu < 0 || tu.order < 0 || tu.order > 2)
+        return fail(LYAP_ERR_INVALID, "batch tuning: waves_per_cu >= 0, order in 0..2");
+
+    std::vector<lyap::BatchClass> recs;
+    std::vector<int> rank;
+    int integ = -1, adapt = -1, prec = -1;
+    int64_t wave_tasks = 0;
+    for (int j = 0; j < n_classes; ++j) {
+        const lyap_batch_class& k = classes[j];
+        lyap_sweep_desc fd = k.desc;
+        int rc = lyap_desc_finalize(&fd);
+        if (rc != LYAP_SUCCESS) return fail(rc, "batch class %d: %s", j, g_last_error.c_str());
+        if (fd.method != LYAP_METHOD_VAR)
+            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: a batch runs the variational method only", j);
+        const int kind = batch_kind(fd);
+        if (kind < 0)
+            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: no batch kernel for model %d dimension %d",
+                        j, fd.model, fd.dim);
+        if (j == 0) {
+            integ = fd.integrator;
+            adapt = fd.adaptive ? 1 : 0;
+            prec = fd.precision;
+        } else if (fd.integrator != integ || (fd.adaptive ? 1 : 0) != adapt || fd.precision != prec) {
+            return fail(LYAP_ERR_INVALID,
+                        "batch class %d: integrator, adaptive flag and precision must match class 0", j);
+        }
+        if (k.count < 0) return fail(LYAP_ERR_INVALID, "batch class %d: negative count", j);
+        if (k.count == 0) continue;
+        if (!k.d_lambda) return fail(LYAP_ERR_INVALID, "batch class %d: d_lambda is NULL", j);
+        if (!k.d_params && k.count > (int64_t)fd.n1 * (int64_t)fd.n2)
+            return fail(LYAP_ERR_INVALID, "batch class %d: %lld points beyond the %d x %d grid", j,
+                        (long long)k.count, fd.n1, fd.n2);
+        lyap::BatchClass r;
+        std::memset(&r, 0, sizeof(r));
+        r.d = fd;
+        r.io = lyap::SweepIO{0,         k.count,   nullptr,     k.d_params, k.d_inits, k.d_lambda,
+                             k.d_status, k.d_n_acc, k.d_n_rej, nullptr,    spread_stride(k.count)};
+        r.kind = kind;
+        recs.push_back(r);
+        rank.push_back(tu.order == 0 ? longest_first_rank(kind)
+                                     : (tu.order == 2 ? (kind >= lyap::BATCH_L96_8 ? 0 : 1) : 0));
+        const int ppw = lyap::batch_points_per_wave(tu.layout, fd.dim);
+        wave_tasks += (k.count + ppw - 1) / ppw;
+    }
+    if (!lyap::batch_variant_built(integ, adapt != 0, prec == LYAP_PREC_MIXED, tu.layout,
+                                   tu.occupancy3 != 0))
+        return fail(LYAP_ERR_UNSUPPORTED,
+                    "batch kernel layout %d%s is built for adaptive Dopri5 only", tu.layout,
+                    tu.occupancy3 ? " (3 waves per SIMD)" : "");
+    // stable order by rank
+    std::vector<int> ord(recs.size());
+    for (size_t j = 0; j < ord.size(); ++j) ord[j] = (int)j;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rank[a] < rank[b]; });
+
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    lyap_batch* b = new lyap_batch();
+    b->device = c->device;
+    b->tuning = tu;
+    b->n = (int)recs.size();
+    if (b->n == 0) {
+        *out = b;
+        return LYAP_SUCCESS;
+    }
+    int per_cu = 0;
+    if ((e = lyap::batch_max_waves_per_cu(integ, adapt != 0, prec == LYAP_PREC_MIXED, tu.layout,
+                                          tu.occupancy3 != 0, &per_cu)) != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "batch occupancy");
+    }
+    if (per_cu < 1) per_cu = 1;
+    if (tu.waves_per_cu > 0 && tu.waves_per_cu < per_cu) per_cu = tu.waves_per_cu;
+    b->waves = (int64_t)per_cu * c->num_cus;
+    if (b->waves > wave_tasks) b->waves = wave_tasks;
+    if (b->waves < 1) b->waves = 1;
+
+    std::vector<lyap::BatchClass> sorted;
+    for (int j : ord) sorted.push_back(recs[j]);
+    if ((e = hipMalloc(&b->d_counters, 64 * sizeof(unsigned long long))) != hipSuccess ||
+        (e = hipMalloc(&b->d_cls, sorted.size() * sizeof(lyap::BatchClass))) != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "hipMalloc(batch)");
+    }
+    for (size_t j = 0; j < sorted.size(); ++j) sorted[j].io.counter = b->d_counters + 8 * j;
+    if ((e = hipMemcpy(b->d_cls, sorted.data(), sorted.size() * sizeof(lyap::BatchClass),
+                       hipMemcpyHostToDevice)) != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "hipMemcpy(batch classes)");
+    }
+    // capture: counter reset (a kernel, not a memset node) + the persistent kernel
+    if ((e = hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreate(&b->ev0)) != hipSuccess || (e = hipEventCreate(&b->ev1)) != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "batch stream/events");
+    }
+    if ((e = hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal)) != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "hipStreamBeginCapture");
+    }
+    hipError_t el = lyap::launch_batch(b->d_cls, b->n, integ, adapt != 0, prec == LYAP_PREC_MIXED,
+                                       tu.layout, tu.occupancy3 != 0, b->waves, b->capture);
+    e = hipStreamEndCapture(b->capture, &b->graph);
+    if (el != hipSuccess || e != hipSuccess) {
+        batch_free(b);
+        return hip_fail(el != hipSuccess ? el : e, "batch graph capture");
+    }
+    if ((e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0)) != hipSuccess) {
+        batch_free(b);
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    *out = b;
+    return LYAP_SUCCESS;
+}
+
+int lyap_batch_launch(lyap_batch* b, void* stream) {
+    if (!b) return fail(LYAP_ERR_INVALID, "lyap_batch_launch: NULL batch");
+    if (b->n == 0) return LYAP_SUCCESS;
+    hipError_t e = hipSetDevice(b->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipStream_t s = (hipStream_t)stream;
+    if ((e = hipEventRecord(b->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    if ((e = hipGraphLaunch(b->exec, s)) != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    if ((e = hipEventRecord(b->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    b->timed = true;
+    return LYAP_SUCCESS;
+}
+
+int lyap_batch_last_ms(lyap_batch* b, float* ms) {
+    if (!b || !ms) return fail(LYAP_ERR_INVALID, "lyap_batch_last_ms: NULL argument");
+    if (!b->timed) return fail(LYAP_ERR_INVALID, "the batch has not been launched");
+    hipError_t e = hipEventSynchronize(b->ev1);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    if ((e = hipEventElapsedTime(ms, b->ev0, b->ev1)) != hipSuccess)
+        return hip_fail(e, "hipEventElapsedTime");
+    return LYAP_SUCCESS;
+}
+
+int lyap_batch_info(lyap_batch* b, int64_t* waves, int* n_classes) {
+    if (!b) return fail(LYAP_ERR_INVALID, "lyap_batch_info: NULL batch");
+    if (waves) *waves = b->waves;
+    if (n_classes) *n_classes = b->n;
+    return LYAP_SUCCESS;
+}
+
+int lyap_batch_destroy(lyap_batch* b) {
+    batch_free(b);
+    return LYAP_SUCCESS;
 }
 
 }  // extern "C"

@@ -16,6 +16,7 @@
 // all branches are uniform; they are also made wave-uniform explicitly.
 // The launch is persistent: a block takes its next point from the counter.
 #include "lyap_device.hpp"
+#include "lyap_l96_wave.hpp"
 #include "lyap_launch.hpp"
 
 namespace lyap {
@@ -23,29 +24,6 @@ namespace lyap {
 __global__ void k_reset_counter_l96(unsigned long long* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
 }
-
-namespace {
-
-template <int N>
-LYAP_D double group_sum(double v) {
-#pragma unroll
-    for (int o = N / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
-    return v;
-}
-
-template <int N>
-LYAP_D double group_max(double v) {
-#pragma unroll
-    for (int o = N / 2; o >= 1; o >>= 1) {
-        const double w = __shfl_xor(v, o);
-        v = v > w ? v : w;
-    }
-    return v;
-}
-
-LYAP_D bool wave_uniform(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
-
-}  // namespace
 
 template <int N, int W, int INTEG, bool ADAPT, bool MIXED>
 __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const SweepIO io) {
@@ -327,42 +305,72 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
     }
 }
 
+// wave-level drain: P points per wave64, one wave per workgroup
+template <int N, int P, int INTEG, bool ADAPT, bool MIXED>
+__global__ __launch_bounds__(64) void k_l96_wave(const lyap_sweep_desc d, const SweepIO io) {
+    __shared__ double lds[l96_wave_lds_doubles<N, P>()];
+    l96_wave_drain<N, P, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, lds);
+}
+
 namespace {
 
-template <int N, int W, int INTEG, bool ADAPT, bool MIXED>
-hipError_t launch_one(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus) {
-    auto kern = k_l96<N, W, INTEG, ADAPT, MIXED>;
+template <class Kern>
+hipError_t launch_persistent_l96(Kern kern, int threads, const lyap_sweep_desc& d, const SweepIO& io,
+                                 int64_t points_per_block, hipStream_t s, int num_cus, int max_blocks) {
     int per_cu = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 64 * W, 0);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     int64_t blocks = (int64_t)per_cu * num_cus;
-    if (blocks > io.count) blocks = io.count;
+    const int64_t need = (io.count + points_per_block - 1) / points_per_block;
+    if (blocks > need) blocks = need;
+    if (max_blocks > 0 && blocks > max_blocks) blocks = max_blocks;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_reset_counter_l96, dim3(1), dim3(64), 0, s, io.counter);
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(64 * W), 0, s, d, io);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(threads), 0, s, d, io);
     return hipGetLastError();
 }
 
-template <int N, int W>
-hipError_t launch_nw(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus) {
-    const int key = (d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0);
-    switch (key) {
-#define LYAP_L96_CASE(I, A, X) \
-    case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0): return launch_one<N, W, I, A, X>(d, io, s, num_cus);
-        LYAP_L96_CASE(LYAP_INT_DVERK, false, false)
-        LYAP_L96_CASE(LYAP_INT_DVERK, false, true)
-        LYAP_L96_CASE(LYAP_INT_DVERK, true, false)
-        LYAP_L96_CASE(LYAP_INT_DVERK, true, true)
-        LYAP_L96_CASE(LYAP_INT_DOPRI5, false, false)
-        LYAP_L96_CASE(LYAP_INT_DOPRI5, false, true)
-        LYAP_L96_CASE(LYAP_INT_DOPRI5, true, false)
-        LYAP_L96_CASE(LYAP_INT_DOPRI5, true, true)
-#undef LYAP_L96_CASE
-        default:
-            return hipErrorInvalidValue;
-    }
+template <int N, int W, int INTEG, bool ADAPT, bool MIXED>
+hipError_t launch_one(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
+                      int max_blocks) {
+    return launch_persistent_l96(k_l96<N, W, INTEG, ADAPT, MIXED>, 64 * W, d, io, 1, s, num_cus,
+                                 max_blocks);
 }
+
+template <int N, int P, int INTEG, bool ADAPT, bool MIXED>
+hipError_t launch_wave_one(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
+                           int max_blocks) {
+    return launch_persistent_l96(k_l96_wave<N, P, INTEG, ADAPT, MIXED>, 64, d, io, P, s, num_cus,
+                                 max_blocks);
+}
+
+#define LYAP_L96_VARIANTS(LAUNCH, ...)                                                    \
+    switch ((d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0)) {           \
+        case (LYAP_INT_DVERK << 2) | 0: return LAUNCH<__VA_ARGS__, LYAP_INT_DVERK, false, false>(d, io, s, num_cus, max_blocks);   \
+        case (LYAP_INT_DVERK << 2) | 1: return LAUNCH<__VA_ARGS__, LYAP_INT_DVERK, false, true>(d, io, s, num_cus, max_blocks);    \
+        case (LYAP_INT_DVERK << 2) | 2: return LAUNCH<__VA_ARGS__, LYAP_INT_DVERK, true, false>(d, io, s, num_cus, max_blocks);    \
+        case (LYAP_INT_DVERK << 2) | 3: return LAUNCH<__VA_ARGS__, LYAP_INT_DVERK, true, true>(d, io, s, num_cus, max_blocks);     \
+        case (LYAP_INT_DOPRI5 << 2) | 0: return LAUNCH<__VA_ARGS__, LYAP_INT_DOPRI5, false, false>(d, io, s, num_cus, max_blocks); \
+        case (LYAP_INT_DOPRI5 << 2) | 1: return LAUNCH<__VA_ARGS__, LYAP_INT_DOPRI5, false, true>(d, io, s, num_cus, max_blocks);  \
+        case (LYAP_INT_DOPRI5 << 2) | 2: return LAUNCH<__VA_ARGS__, LYAP_INT_DOPRI5, true, false>(d, io, s, num_cus, max_blocks);  \
+        case (LYAP_INT_DOPRI5 << 2) | 3: return LAUNCH<__VA_ARGS__, LYAP_INT_DOPRI5, true, true>(d, io, s, num_cus, max_blocks);   \
+        default: return hipErrorInvalidValue;                                                  \
+    }
+
+template <int N, int W>
+hipError_t launch_nw(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
+                     int max_blocks) {
+    LYAP_L96_VARIANTS(launch_one, N, W)
+}
+
+template <int N, int P>
+hipError_t launch_np(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
+                     int max_blocks) {
+    LYAP_L96_VARIANTS(launch_wave_one, N, P)
+}
+
+#undef LYAP_L96_VARIANTS
 
 }  // namespace
 
@@ -373,17 +381,38 @@ bool l96_supported(int n, int waves) {
            (n == 32 && (waves == 2 || waves == 4));
 }
 
+bool l96_wave_supported(int n, int ppw) {
+    return (n == 8 && (ppw == 1 || ppw == 2 || ppw == 4 || ppw == 8)) ||
+           (n == 16 && (ppw == 1 || ppw == 2 || ppw == 4)) || (n == 32 && (ppw == 1 || ppw == 2));
+}
+
 hipError_t launch_l96_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
-                            int waves) {
+                            int waves, int ppw, int max_blocks) {
+    if (d.method != LYAP_METHOD_VAR) return hipErrorInvalidValue;
+    if (ppw > 0) {
+        if (!l96_wave_supported(d.dim, ppw)) return hipErrorInvalidValue;
+        switch (d.dim * 16 + ppw) {
+            case 8 * 16 + 1: return launch_np<8, 1>(d, io, s, num_cus, max_blocks);
+            case 8 * 16 + 2: return launch_np<8, 2>(d, io, s, num_cus, max_blocks);
+            case 8 * 16 + 4: return launch_np<8, 4>(d, io, s, num_cus, max_blocks);
+            case 8 * 16 + 8: return launch_np<8, 8>(d, io, s, num_cus, max_blocks);
+            case 16 * 16 + 1: return launch_np<16, 1>(d, io, s, num_cus, max_blocks);
+            case 16 * 16 + 2: return launch_np<16, 2>(d, io, s, num_cus, max_blocks);
+            case 16 * 16 + 4: return launch_np<16, 4>(d, io, s, num_cus, max_blocks);
+            case 32 * 16 + 1: return launch_np<32, 1>(d, io, s, num_cus, max_blocks);
+            case 32 * 16 + 2: return launch_np<32, 2>(d, io, s, num_cus, max_blocks);
+            default: return hipErrorInvalidValue;
+        }
+    }
     if (waves <= 0) waves = l96_default_waves(d.dim);
-    if (d.method != LYAP_METHOD_VAR || !l96_supported(d.dim, waves)) return hipErrorInvalidValue;
+    if (!l96_supported(d.dim, waves)) return hipErrorInvalidValue;
     switch (d.dim * 16 + waves) {
-        case 8 * 16 + 1: return launch_nw<8, 1>(d, io, s, num_cus);
-        case 16 * 16 + 1: return launch_nw<16, 1>(d, io, s, num_cus);
-        case 16 * 16 + 2: return launch_nw<16, 2>(d, io, s, num_cus);
-        case 16 * 16 + 4: return launch_nw<16, 4>(d, io, s, num_cus);
-        case 32 * 16 + 2: return launch_nw<32, 2>(d, io, s, num_cus);
-        case 32 * 16 + 4: return launch_nw<32, 4>(d, io, s, num_cus);
+        case 8 * 16 + 1: return launch_nw<8, 1>(d, io, s, num_cus, max_blocks);
+        case 16 * 16 + 1: return launch_nw<16, 1>(d, io, s, num_cus, max_blocks);
+        case 16 * 16 + 2: return launch_nw<16, 2>(d, io, s, num_cus, max_blocks);
+        case 16 * 16 + 4: return launch_nw<16, 4>(d, io, s, num_cus, max_blocks);
+        case 32 * 16 + 2: return launch_nw<32, 2>(d, io, s, num_cus, max_blocks);
+        case 32 * 16 + 4: return launch_nw<32, 4>(d, io, s, num_cus, max_blocks);
         default: return hipErrorInvalidValue;
     }
 }

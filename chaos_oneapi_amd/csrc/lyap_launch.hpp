@@ -17,11 +17,43 @@ int set_error(int code, const char* fmt, ...);
 hipError_t launch_lane_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
                              int num_cus, int max_blocks);
 
-// Lorenz-96, one point per block of `waves` wave64s (lyap_l96.hip);
-// waves <= 0 picks the default for the dimension
+// Lorenz-96 (lyap_l96.hip): ppw > 0 runs the wave-level drain with ppw
+// points per wave64; otherwise one point per block of `waves` wave64s
+// (waves <= 0 picks the default for the dimension); max_blocks > 0 caps the
+// persistent grid
 hipError_t launch_l96_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
-                            int waves);
+                            int waves, int ppw, int max_blocks);
 bool l96_supported(int n, int waves);
+bool l96_wave_supported(int n, int ppw);
 int l96_default_waves(int n);
+
+// ---------------------------------------------------------------------------
+// batched mixed-model launch (lyap_batch.hip)
+// ---------------------------------------------------------------------------
+enum BatchKind : int32_t {
+    BATCH_SHIMIZU = 0,
+    BATCH_LORENZ = 1,
+    BATCH_ROSSLER = 2,
+    BATCH_L96_8 = 3,
+    BATCH_L96_16 = 4,
+    BATCH_L96_32 = 5,
+};
+constexpr int kMaxBatchClasses = 16;
+constexpr int kBatchLayouts = 2;
+
+// one class of a batch as the kernel reads it (device memory)
+struct BatchClass {
+    lyap_sweep_desc d;  // finalised
+    SweepIO io;         // io.counter: this class's own work counter
+    int32_t kind;       // BatchKind
+    int32_t pad;
+};
+
+bool batch_variant_built(int integ, bool adapt, bool mixed, int layout, bool w3);
+int batch_points_per_wave(int layout, int dim);
+hipError_t batch_max_waves_per_cu(int integ, bool adapt, bool mixed, int layout, bool w3, int* out);
+// reset of the class counters + the persistent kernel on `waves` one-wave workgroups
+hipError_t launch_batch(const BatchClass* d_cls, int n_cls, int integ, bool adapt, bool mixed,
+                        int layout, bool w3, int64_t waves, hipStream_t s);
 
 }  // namespace lyap

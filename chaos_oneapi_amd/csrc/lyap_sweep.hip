@@ -15,341 +15,17 @@
 //   k_qrref_lane   reference QR mode      cuda/lyapunov.dp.cpp:229-259
 //   k_fdref_lane   reference FD mode      cuda/lyapunov.dp.cpp:5-129
 #include "lyap_device.hpp"
+#include "lyap_lane.hpp"
 #include "lyap_launch.hpp"
 
 namespace lyap {
-
-// ---------------------------------------------------------------------------
-// base RK step: x -> xn; stage states X[s] (for the coupled tangent); error
-// vector when ERR.  With FSAL (Dopri5 + ERR) k1 comes in from the previous
-// accepted step and the new k1 (= f(xn)) goes out.
-// ---------------------------------------------------------------------------
-template <class M, int INTEG, bool ERR, class TX>
-LYAP_D void base_step(const double* x, const double* p, double h, double* xn, double* ev,
-                      typename M::template J<TX> (&X)[Tab<INTEG>::S], double* k1_io) {
-    using TB = Tab<INTEG>;
-    constexpr int S = TB::S, N = M::N;
-    constexpr bool FSAL = (INTEG == LYAP_INT_DOPRI5) && ERR;
-    double k[S][N];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        if (!ERR && !TB::for_solution(s)) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) k[s][i] = 0.0;
-            continue;
-        }
-        double arg[N];
-        const RowA<TB, double, S> row(s);
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            bool any;
-            const double acc = coef_sum_a(s, row, k, i, any);
-            arg[i] = any ? fma(h, acc, x[i]) : x[i];
-        }
-        if (ERR || TB::for_solution(s)) X[s] = M::template jac<TX>(arg, p);
-        if (s == 0 && FSAL) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) k[0][i] = k1_io[i];
-        } else {
-            M::rhs(arg, p, k[s]);
-        }
-    }
-    {
-        const RowB<TB, double, S> rb;
-#pragma unroll
-        for (int i = 0; i < N; ++i) xn[i] = fma(h, coef_sum_b(rb, k, i), x[i]);
-    }
-    if (ERR) {
-        const RowE<TB, double, S> re;
-#pragma unroll
-        for (int i = 0; i < N; ++i) ev[i] = h * coef_sum_e(re, k, i);
-    }
-    if (FSAL) {
-#pragma unroll
-        for (int i = 0; i < N; ++i) k1_io[i] = k[S - 1][i];
-    }
-}
-
-// coupled tangent step for one deviation vector v (FP64 storage).
-// FP64 form: stages in double.  Mixed form: stages in binary32, increment
-// h*sum(b_j K_j) formed in binary32 and added to the FP64 vector.
-template <class M, int INTEG, bool MIXED, class TX>
-LYAP_D void tangent_step(const typename M::template J<TX> (&X)[Tab<INTEG>::S], const double* p,
-                         const float* pf, double h, double* v) {
-    using TB = Tab<INTEG>;
-    constexpr int S = TB::S, N = M::N;
-    using T = typename std::conditional<MIXED, float, double>::type;
-    const T hT = (T)h;
-    T vt[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) vt[i] = (T)v[i];
-    T kv[S][N];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-        if (!TB::for_solution(s)) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) kv[s][i] = T(0);
-            continue;
-        }
-        T arg[N];
-        const RowA<TB, T, S> row(s);
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            bool any;
-            const T acc = coef_sum_a(s, row, kv, i, any);
-            arg[i] = any ? fma(hT, acc, vt[i]) : vt[i];
-        }
-        if constexpr (MIXED) M::template jvp<float>(X[s], pf, arg, kv[s]);
-        else M::template jvp<double>(X[s], p, arg, kv[s]);
-    }
-    const RowB<TB, T, S> rb;
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        if constexpr (MIXED) {
-            const float inc = hT * coef_sum_b(rb, kv, i);
-            v[i] = v[i] + (double)inc;
-        } else {
-            v[i] = fma(h, coef_sum_b(rb, kv, i), v[i]);
-        }
-    }
-}
-
-// modified Gram-Schmidt on K rows of length N (sequential fma dot products)
-template <int K, int N>
-LYAP_D void mgs_rows(double (&V)[K][N], double* ln) {
-#pragma unroll
-    for (int j = 0; j < K; ++j) {
-#pragma unroll
-        for (int r = 0; r < j; ++r) {
-            double c = V[r][0] * V[j][0];
-#pragma unroll
-            for (int i = 1; i < N; ++i) c = fma(V[r][i], V[j][i], c);
-#pragma unroll
-            for (int i = 0; i < N; ++i) V[j][i] = fma(-c, V[r][i], V[j][i]);
-        }
-        double s = V[j][0] * V[j][0];
-#pragma unroll
-        for (int i = 1; i < N; ++i) s = fma(V[j][i], V[j][i], s);
-        const double nrm = sqrt(s);
-        ln[j] = det_log(nrm);
-        const double inv = 1.0 / nrm;
-#pragma unroll
-        for (int i = 0; i < N; ++i) V[j][i] = V[j][i] * inv;
-    }
-}
-
-template <int N>
-LYAP_D double err_norm_seq(const double* ev, const double* x, const double* xn, double rtol,
-                           double atol) {
-    double q[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        const double ax = fabs(x[i]), an = fabs(xn[i]);
-        const double sk = fma(rtol, ax > an ? ax : an, atol);
-        q[i] = ev[i] / sk;
-    }
-    double s = q[0] * q[0];
-#pragma unroll
-    for (int i = 1; i < N; ++i) s = fma(q[i], q[i], s);
-    return sqrt(s / (double)N);
-}
-
-template <int N>
-LYAP_D bool escaped(const double* x, double radius) {
-    bool e = false;
-#pragma unroll
-    for (int i = 0; i < N; ++i) e = e || !(fabs(x[i]) <= radius);
-    return e;
-}
-
-// wave-aggregated refill: lanes with want=true get consecutive work items
-LYAP_D int64_t grab_items(bool want, unsigned long long* counter) {
-    const uint64_t mask = __ballot(want);
-    if (mask == 0) return -1;
-    const int lane = (int)__lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    unsigned long long base = 0;
-    if (lane == leader) base = atomicAdd(counter, (unsigned long long)__popcll(mask));
-    base = __shfl(base, leader);
-    const uint64_t below = mask & ((1ULL << lane) - 1ULL);
-    return want ? (int64_t)(base + (unsigned long long)__popcll(below)) : -1;
-}
 
 // ---------------------------------------------------------------------------
 // the sweep's method: one lane per point, persistent
 // ---------------------------------------------------------------------------
 template <class M, int INTEG, bool ADAPT, bool MIXED>
 __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const SweepIO io) {
-    constexpr int N = M::N, P = M::P, K = M::N, S = Tab<INTEG>::S;
-    using TX = typename std::conditional<MIXED, float, double>::type;
-    // scalars the step loop reads; everything else of the descriptor is
-    // re-read through an opaque pointer when a lane takes a new point, so the
-    // compiler does not keep (and spill) the whole descriptor in SGPRs
-    const int64_t n_tr = d.n_renorm_transient;
-    const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
-    const int64_t max_steps = d.max_steps, spr = d.steps_per_renorm;
-    const double rtol = d.rtol, atol = d.atol, h_min = d.h_min, h_max = d.h_max;
-    const double t_renorm = d.t_renorm, esc = d.escape_radius, h_fixed = d.h;
-    const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
-                               : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
-    const int n_exps = d.n_exps;
-    const double log_floor = det_log(1e-4);
-
-    double x[N], V[K][N], sum[K], comp[K], p[P], k1[N];
-    float pf[P];
-    double t = 0.0, h = 0.0;
-    int64_t m = 0, step_in = 0, item = -1;
-    uint32_t nacc = 0, nrej = 0;
-    Ctrl ctrl{1e-4, log_floor, false};
-    bool active = false, exhausted = false;
-
-    for (;;) {
-        const int64_t got = grab_items(!active && !exhausted, io.counter);
-        if (!active && !exhausted) {
-            if (got < io.count) {
-                KernargDesc* dp = kernarg_desc();
-                item = item_position(io, got);
-                const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
-                point_params<P>(*dp, idx, io.params, p);
-#pragma unroll
-                for (int j = 0; j < P; ++j) pf[j] = (float)p[j];
-#pragma unroll
-                for (int i = 0; i < N; ++i)
-                    x[i] = io.inits ? io.inits[idx * N + i] : init_component(*dp, idx, i, p);
-#pragma unroll
-                for (int r = 0; r < K; ++r) {
-                    sum[r] = 0.0;
-                    comp[r] = 0.0;
-#pragma unroll
-                    for (int i = 0; i < N; ++i) V[r][i] = (r == i) ? 1.0 : 0.0;
-                }
-                t = 0.0;
-                h = dp->h;
-                m = 0;
-                step_in = 0;
-                nacc = nrej = 0;
-                ctrl.fac_old = 1e-4;
-                ctrl.log_fac_old = log_floor;
-                ctrl.rejected = false;
-                if (INTEG == LYAP_INT_DOPRI5 && ADAPT) M::rhs(x, p, k1);
-                active = true;
-            } else {
-                exhausted = true;
-            }
-        }
-        if (!__any(active)) break;
-        if (!active) continue;
-
-        uint32_t st = LYAP_PT_OK;
-        bool finished = false;
-        double xn[N], ln[K];
-        typename M::template J<TX> X[S];
-        if constexpr (ADAPT) {
-            if ((int64_t)nacc + (int64_t)nrej >= max_steps) {
-                st = LYAP_PT_MAXSTEPS;
-                finished = true;
-            } else {
-                const double t_next = (double)(m + 1) * t_renorm;
-                double h_try = h;
-                bool clipped = false;
-                const double jn = M::jnorm(x, p);
-                if (h_try * jn > 1.0) h_try = 1.0 / jn;
-                // land on the boundary when it is within 1/16 of the step, so no
-                // round-off sliver is left before it
-                if (t_next - t <= 1.0625 * h_try) {
-                    h_try = t_next - t;
-                    clipped = true;
-                }
-                if (h_try < h_min) {
-                    st = LYAP_PT_STEP_UNDERFLOW;
-                    finished = true;
-                } else {
-                    double ev[N], k1_next[N];
-#pragma unroll
-                    for (int i = 0; i < N; ++i) k1_next[i] = k1[i];
-                    base_step<M, INTEG, true, TX>(x, p, h_try, xn, ev, X, k1_next);
-                    const double err = err_norm_seq<N>(ev, x, xn, rtol, atol);
-                    if (!(err < kInf)) {
-                        st = LYAP_PT_NONFINITE;
-                        finished = true;
-                    } else {
-                        double h_next;
-                        const double lerr = det_log(err);
-                        const bool acc =
-                            ctrl_update<INTEG>(ctrl, err, lerr, log_floor, h_try, h_max, h_next);
-                        h = h_next;
-                        if (!acc) {
-                            ++nrej;
-                        } else {
-#pragma unroll
-                            for (int r = 0; r < K; ++r)
-                                tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_try, V[r]);
-#pragma unroll
-                            for (int i = 0; i < N; ++i) {
-                                x[i] = xn[i];
-                                k1[i] = k1_next[i];
-                            }
-                            t = clipped ? t_next : t + h_try;
-                            ++nacc;
-                            if (escaped<N>(x, esc)) {
-                                st = LYAP_PT_ESCAPED;
-                                finished = true;
-                            } else if (clipped) {
-                                mgs_rows<K, N>(V, ln);
-                                ++m;
-                                if (m > n_tr) {
-#pragma unroll
-                                    for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
-                                }
-                                if (m == m_total) finished = true;
-                            }
-                        }
-                    }
-                }
-            }
-        } else {
-            base_step<M, INTEG, false, TX>(x, p, h_fixed, xn, nullptr, X, nullptr);
-#pragma unroll
-            for (int r = 0; r < K; ++r) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_fixed, V[r]);
-#pragma unroll
-            for (int i = 0; i < N; ++i) x[i] = xn[i];
-            ++nacc;
-            if (++step_in == spr) {
-                step_in = 0;
-                bool bad = false;
-#pragma unroll
-                for (int i = 0; i < N; ++i) bad = bad || !isfinite(x[i]);
-                if (bad) {
-                    st = LYAP_PT_NONFINITE;
-                    finished = true;
-                } else if (escaped<N>(x, esc)) {
-                    st = LYAP_PT_ESCAPED;
-                    finished = true;
-                } else {
-                    mgs_rows<K, N>(V, ln);
-                    ++m;
-                    if (m > n_tr) {
-#pragma unroll
-                        for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
-                    }
-                    if (m == m_total) finished = true;
-                }
-            }
-        }
-        if (finished) {
-            for (int r = 0; r < n_exps; ++r) {
-                double s = 0.0;
-#pragma unroll
-                for (int q = 0; q < K; ++q)
-                    if (q == r) s = sum[q];
-                io.lambda[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
-            }
-            if (io.status) io.status[item] = st;
-            if (io.n_acc) io.n_acc[item] = nacc;
-            if (io.n_rej) io.n_rej[item] = nrej;
-            active = false;
-        }
-    }
+    lane_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io);
 }
 
 // ---------------------------------------------------------------------------

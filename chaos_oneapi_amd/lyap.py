@@ -32,6 +32,8 @@ EXPORTS = (
     "lyap_sweep_run", "lyap_last_kernel_ms", "lyap_cuda_lyapunov", "lyap_ctx_set_option",
     "lyap_mlp_create", "lyap_mlp_destroy", "lyap_mlp_backward", "lyap_mlp_train_step",
     "lyap_mlp_forward", "lyap_mlp_param_count", "lyap_mlp_get", "lyap_mlp_set_params",
+    "lyap_batch_create", "lyap_batch_launch", "lyap_batch_last_ms", "lyap_batch_info",
+    "lyap_batch_destroy",
 )
 
 
@@ -67,6 +69,20 @@ class SweepDesc(ctypes.Structure):
             v = getattr(self, name)
             out[name] = list(v) if name in ("base_params", "init_state") else v
         return out
+
+
+class BatchClassRec(ctypes.Structure):
+    """lyap_batch_class"""
+    _fields_ = [("desc", SweepDesc), ("count", ctypes.c_int64), ("d_params", ctypes.c_void_p),
+                ("d_inits", ctypes.c_void_p), ("d_lambda", ctypes.c_void_p),
+                ("d_status", ctypes.c_void_p), ("d_n_acc", ctypes.c_void_p),
+                ("d_n_rej", ctypes.c_void_p)]
+
+
+class BatchTuning(ctypes.Structure):
+    """lyap_batch_tuning"""
+    _fields_ = [("layout", ctypes.c_int32), ("occupancy3", ctypes.c_int32),
+                ("waves_per_cu", ctypes.c_int32), ("order", ctypes.c_int32)]
 
 
 class ModelInfo(ctypes.Structure):
@@ -105,6 +121,11 @@ def load(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise ImportError(f"{path} is missing: run chaos_oneapi_amd.build.build() first "
                           "(the sweep has no CPU fallback)")
+    # torch ships its own libamdhip64.so.7; importing it first makes liblyap's
+    # NEEDED entry bind to that already-loaded runtime, so both share one HIP
+    # runtime (device buffers and streams pass between them).  Loading liblyap
+    # first would bring in a second runtime, after which torch sees no GPU.
+    import torch  # noqa: F401
     L = ctypes.CDLL(path)
     P, i32, i64, d, vp = ctypes.POINTER, ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
     L.lyap_last_error.restype = ctypes.c_char_p
@@ -127,6 +148,11 @@ def load(path: str = LIB_PATH):
     L.lyap_mlp_param_count.argtypes = [vp, P(i64)]
     L.lyap_mlp_get.argtypes = [vp, ctypes.c_int, vp, i64]
     L.lyap_mlp_set_params.argtypes = [vp, vp, i64]
+    L.lyap_batch_create.argtypes = [vp, P(BatchClassRec), ctypes.c_int, P(BatchTuning), P(vp)]
+    L.lyap_batch_launch.argtypes = [vp, vp]
+    L.lyap_batch_last_ms.argtypes = [vp, P(ctypes.c_float)]
+    L.lyap_batch_info.argtypes = [vp, P(i64), P(ctypes.c_int)]
+    L.lyap_batch_destroy.argtypes = [vp]
     L.lyap_cuda_lyapunov.argtypes = [
         vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         vp, i32, i32, vp, i32, d, d, vp, i32, d, d, d, d, vp]
@@ -248,6 +274,74 @@ class Context:
             int(dimension), int(model), _ptr(params), int(params_dim), float(eps), float(h),
             _ptr(lyap_exps), int(exps_num), float(time_skip), float(time_skip_clp),
             float(calc_time), float(add_skip_steps), ctypes.c_void_p(s.cuda_stream)))
+
+
+class Batch:
+    """A captured batched launch (lyap_batch_*) over classes of points.
+
+    classes: list of dicts with keys desc (SweepDesc), count, and optional
+    params / inits (device tensors, AoS).  Output tensors are allocated here
+    and stay owned by this object (the captured graph holds their addresses).
+    """
+
+    def __init__(self, ctx: Context, classes, layout: int = 0, occupancy3: bool = False,
+                 waves_per_cu: int = 0, order: int = 0):
+        import torch
+
+        dev = torch.device("cuda", ctx.device)
+        self.ctx = ctx
+        self.outputs = []
+        self._keep = []
+        recs = (BatchClassRec * len(classes))()
+        for j, c in enumerate(classes):
+            desc, n = c["desc"], int(c["count"])
+            lam = torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev)
+            st = torch.empty(n, dtype=torch.int32, device=dev)
+            na = torch.empty(n, dtype=torch.int32, device=dev)
+            nr = torch.empty(n, dtype=torch.int32, device=dev)
+            self.outputs.append((lam, st, na, nr))
+            for key in ("params", "inits"):
+                t = c.get(key)
+                if t is not None:
+                    assert t.is_cuda and t.is_contiguous() and t.dtype == torch.float64
+                    self._keep.append(t)
+            r = recs[j]
+            r.desc = desc
+            r.count = n
+            r.d_params = c["params"].data_ptr() if c.get("params") is not None else None
+            r.d_inits = c["inits"].data_ptr() if c.get("inits") is not None else None
+            r.d_lambda, r.d_status = lam.data_ptr(), st.data_ptr()
+            r.d_n_acc, r.d_n_rej = na.data_ptr(), nr.data_ptr()
+        tu = BatchTuning(int(layout), int(bool(occupancy3)), int(waves_per_cu), int(order))
+        self._h = ctypes.c_void_p()
+        check(load().lyap_batch_create(ctx._h, recs, len(classes), ctypes.byref(tu),
+                                       ctypes.byref(self._h)))
+        waves, ncls = ctypes.c_int64(), ctypes.c_int()
+        check(load().lyap_batch_info(self._h, ctypes.byref(waves), ctypes.byref(ncls)))
+        self.waves, self.n_classes = int(waves.value), int(ncls.value)
+
+    def launch(self, stream=None):
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", self.ctx.device))
+        check(load().lyap_batch_launch(self._h, ctypes.c_void_p(s.cuda_stream)))
+        return self.outputs
+
+    def last_ms(self) -> float:
+        ms = ctypes.c_float()
+        check(load().lyap_batch_last_ms(self._h, ctypes.byref(ms)))
+        return float(ms.value)
+
+    def close(self):
+        if self._h:
+            load().lyap_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Mlp:

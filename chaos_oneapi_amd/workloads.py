@@ -56,3 +56,47 @@ def lorenz96_dopri(n: int = 256, dim: int = 16, precision: int = L.PREC_FP64,
              init_param_offset=0, init_scale=0.1, seed=20240603)
     d.update(grid_fields(0, 2.0, 16.0, n, 1, 0.5, 1.5, n))
     return d
+
+
+# class mix of the batched uneven sweep: (name, share)
+BATCH_MIX = (("lorenz", 0.40), ("rossler", 0.30), ("lorenz96_8", 0.20), ("lorenz96_16", 0.08),
+             ("lorenz96_32", 0.02))
+
+
+def batch_mix(n_points: int = 1 << 20, seed: int = 20240604, precision: int = L.PREC_MIXED,
+              time_scale: float = 1.0, tol: float = 1e-9) -> list:
+    """Batched uneven sweep: n_points drawn with the seed as 40% Lorenz, 30%
+    Rossler, 20% Lorenz-96 N=8, 8% N=16 and 2% N=32, each with parameters
+    uniform over its sweep ranges (Lorenz rho in [0,200], beta in [0.5,4];
+    Rossler a in [0.05,0.4], c in [2,20]; Lorenz-96 F in [2,16], d in
+    [0.5,1.5]), all adaptive Dopri5 at rtol=atol=tol.  Each class keeps the
+    times, renormalisation interval and initial-state rule of its single-model
+    workload; time_scale shrinks the transient and averaging times (tests).
+
+    Returns a list of dicts {name, spec, count, params} with params an
+    [count, n_params] float64 numpy array, in the mix order."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    share = np.array([s for _, s in BATCH_MIX])
+    cls = rng.choice(len(BATCH_MIX), size=n_points, p=share / share.sum())
+    counts = np.bincount(cls, minlength=len(BATCH_MIX))
+    out = []
+    for j, (name, _) in enumerate(BATCH_MIX):
+        n = int(counts[j])
+        if name == "lorenz":
+            spec = lorenz_fixed(2, precision=precision, t_transient=100.0 * time_scale,
+                                t_average=1000.0 * time_scale)
+            p = np.stack([np.full(n, 10.0), rng.uniform(0.0, 200.0, n), rng.uniform(0.5, 4.0, n)], 1)
+        elif name == "rossler":
+            spec = rossler_dverk(2, precision=precision, t_transient=200.0 * time_scale,
+                                 t_average=2000.0 * time_scale)
+            p = np.stack([rng.uniform(0.05, 0.4, n), rng.uniform(2.0, 20.0, n), np.full(n, 0.2)], 1)
+        else:
+            dim = int(name.split("_")[1])
+            spec = lorenz96_dopri(2, dim=dim, precision=precision, t_transient=100.0 * time_scale,
+                                  t_average=500.0 * time_scale)
+            p = np.stack([rng.uniform(2.0, 16.0, n), rng.uniform(0.5, 1.5, n)], 1)
+        spec.update(integrator=L.INT_DOPRI5, adaptive=1, rtol=tol, atol=tol)
+        out.append(dict(name=name, spec=spec, count=n, params=np.ascontiguousarray(p)))
+    return out

@@ -153,7 +153,11 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  * by the autotuner.  Keys:
  *   "l96_waves_8", "l96_waves_16", "l96_waves_32": wave64s per point of the
  *       Lorenz-96 kernel (N=8: 1; N=16: 1, 2, 4; N=32: 2, 4; 0 = default)
+ *   "l96_ppw_8", "l96_ppw_16", "l96_ppw_32": points per wave64 of the
+ *       wave-level Lorenz-96 drain (N=8: 1, 2, 4, 8; N=16: 1, 2, 4; N=32: 1, 2);
+ *       0 (default) = the block-per-point kernel above
  *   "lane_max_blocks": cap on the persistent grid of the 3-D kernels (0 = none)
+ *   "l96_max_blocks": cap on the persistent grid of the Lorenz-96 kernels (0 = none)
  */
 int lyap_ctx_set_option(lyap_ctx* ctx, const char* key, int64_t value);
 
@@ -177,6 +181,52 @@ int lyap_cuda_lyapunov(lyap_ctx* ctx, int block_num, int block_size, size_t task
                        double integration_step, double* d_lyap_exps, int32_t exps_num,
                        double time_skip, double time_skip_clp, double calc_time,
                        double add_skip_steps, void* stream);
+
+/*
+ * Batched mixed-model sweep (no counterpart in the reference, which launches
+ * one model per cudaLyapunov call, cuda/lyapunov.dp.cpp:321-367).  Up to 16
+ * classes, each one model and dimension with its own descriptor, points and
+ * outputs, run by ONE persistent launch: one-wave workgroups walk the classes
+ * in queue order and drain each through its own work counter (3-D flows one
+ * point per lane, Lorenz-96 N = 8/16/32 several points per wave).  The counter
+ * reset (a kernel) and the launch are captured once into a hipGraph at create
+ * time; lyap_batch_launch replays it on the caller's stream.  All classes share
+ * the integrator, the adaptive flag and the precision; the method is VAR.
+ * Buffers are device memory owned by the caller and must stay valid for the
+ * life of the batch (the graph holds their addresses).
+ */
+typedef struct lyap_batch_class {
+    lyap_sweep_desc desc;    /* finalised by lyap_batch_create */
+    int64_t count;           /* points of the class (0: class skipped) */
+    const double* d_params;  /* optional [count * n_params] AoS, else grid points 0..count */
+    const double* d_inits;   /* optional [count * dim] AoS, else the seeded rule */
+    double* d_lambda;        /* [count * n_exps] */
+    uint32_t* d_status;      /* optional [count] */
+    uint32_t* d_n_acc;       /* optional [count] */
+    uint32_t* d_n_rej;       /* optional [count] */
+} lyap_batch_class;
+
+typedef struct lyap_batch_tuning {
+    int32_t layout;        /* Lorenz-96 points per wave for N = 8/16/32: 0 -> 8/4/1, 1 -> 4/2/1 */
+    int32_t occupancy3;    /* 1: kernel built for 3 waves per SIMD (168 VGPRs, may spill) */
+    int32_t waves_per_cu;  /* persistent grid = min(waves_per_cu, fit) * CUs; 0 = as many as fit */
+    int32_t order;         /* queue order: 0 longest task first (3-D lanes, then L96 32/16/8),
+                              1 as given, 2 Lorenz-96 classes first */
+} lyap_batch_tuning;
+
+typedef struct lyap_batch lyap_batch;
+
+/* tuning NULL = all zero; layouts other than 0 and occupancy3 are built for
+   adaptive Dopri5 only */
+int lyap_batch_create(lyap_ctx* ctx, const lyap_batch_class* classes, int n_classes,
+                      const lyap_batch_tuning* tuning, lyap_batch** out);
+/* replays the captured graph on `stream` (stream-ordered, returns at once) */
+int lyap_batch_launch(lyap_batch* b, void* stream);
+/* duration of the last replay (events on the launch stream; waits for it) */
+int lyap_batch_last_ms(lyap_batch* b, float* ms);
+/* persistent grid size and the number of non-empty classes */
+int lyap_batch_info(lyap_batch* b, int64_t* waves, int* n_classes);
+int lyap_batch_destroy(lyap_batch* b);
 
 /*
  * Surrogate network (no counterpart in the reference): (p1, p2) -> 2*n_fourier

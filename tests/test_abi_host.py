@@ -124,3 +124,21 @@ def test_null_arguments_return_codes():
                               None) == L.ERR_INVALID
     assert lib.lyap_desc_finalize(None) == L.ERR_INVALID
     assert lib.lyap_model_lookup(None, None) == L.ERR_INVALID
+
+
+def test_batch_entry_points_reject_null():
+    lib = L.load()
+    out = ctypes.c_void_p()
+    assert lib.lyap_batch_create(None, None, 1, None, ctypes.byref(out)) == L.ERR_INVALID
+    assert lib.lyap_batch_launch(None, None) == L.ERR_INVALID
+    assert lib.lyap_batch_last_ms(None, None) == L.ERR_INVALID
+    assert lib.lyap_batch_info(None, None, None) == L.ERR_INVALID
+    assert lib.lyap_batch_destroy(None) == L.SUCCESS
+    assert b"NULL" in lib.lyap_last_error()
+
+
+def test_batch_record_layout():
+    # lyap_batch_class: the descriptor, then count and six pointers
+    assert L.BatchClassRec.count.offset == ctypes.sizeof(L.SweepDesc)
+    assert ctypes.sizeof(L.BatchClassRec) == ctypes.sizeof(L.SweepDesc) + 7 * 8
+    assert ctypes.sizeof(L.BatchTuning) == 16

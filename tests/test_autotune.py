@@ -1,0 +1,103 @@
+"""Autotuner host logic (candidate selection, cache, sampling) with a stand-in
+timer; the GPU timing path runs in tests/test_gpu_batch.py."""
+import numpy as np
+import torch
+
+from chaos_oneapi_amd import autotune as A
+from chaos_oneapi_amd import lyap as L
+from chaos_oneapi_amd import workloads as W
+
+
+def cpu_classes(n=5000, seed=3):
+    out = []
+    for c in W.batch_mix(n, seed=seed, time_scale=0.01):
+        out.append(dict(desc=L.finalize(L.make_desc(c["spec"])), count=c["count"],
+                        params=torch.from_numpy(c["params"]), name=c["name"]))
+    return out
+
+
+def test_pick_best_prefers_fastest_then_first():
+    assert A.pick_best({"a": 3.0, "b": 1.0, "c": 2.0}) == "b"
+    assert A.pick_best({"a": 1.0, "b": 1.0}) == "a"
+
+
+def test_sample_indices_seeded_sorted_unique():
+    a = A.sample_indices(10000, 1 / 16, 256, 9)
+    b = A.sample_indices(10000, 1 / 16, 256, 9)
+    assert len(a) == 625 and np.array_equal(a, b)
+    assert np.all(np.diff(a) > 0)
+    assert len(A.sample_indices(100, 1 / 16, 256, 9)) == 100  # small classes run whole
+
+
+def test_sample_classes_keeps_rows():
+    cls = cpu_classes()
+    s = A.sample_classes(cls, frac=0.1, min_points=10)
+    for c, sc in zip(cls, s):
+        idx = A.sample_indices(c["count"], 0.1, 10, 5 + cls.index(c))
+        assert sc["count"] == len(idx)
+        assert torch.equal(sc["params"], c["params"][torch.as_tensor(idx)])
+
+
+def test_tune_batch_picks_and_caches(tmp_path):
+    cls = cpu_classes()
+    cache = A.Cache(str(tmp_path / "tune.json"))
+    cost = {(0, False, 0, 0): 10.0, (0, True, 0, 0): 7.0, (1, False, 0, 0): 9.0,
+            (1, True, 0, 0): 8.0, (0, False, 0, 2): 12.0, (0, True, 0, 2): 7.5}
+    calls = []
+
+    def timer(cand, sample):
+        calls.append(cand)
+        assert sum(c["count"] for c in sample) < sum(c["count"] for c in cls)
+        return cost[cand]
+
+    r = A.tune_batch(None, cls, cache=cache, timer=timer, frac=0.1, min_points=10)
+    assert r.best == (0, True, 0, 0) and not r.cached
+    assert len(calls) == len(A.BATCH_CANDIDATES)
+    r2 = A.tune_batch(None, cls, cache=A.Cache(str(tmp_path / "tune.json")), timer=timer)
+    assert r2.cached and tuple(r2.best) == (0, True, 0, 0)
+    assert len(calls) == len(A.BATCH_CANDIDATES)  # served from the file
+    assert A.batch_kwargs(r2.best) == dict(layout=0, occupancy3=True, waves_per_cu=0, order=0)
+
+
+def test_tune_batch_skips_unbuilt_variants(tmp_path):
+    cls = cpu_classes()
+
+    def timer(cand, sample):
+        if cand[0] == 1:
+            raise L.LyapError(L.ERR_UNSUPPORTED, "not built")
+        return 5.0 - cand[3]
+
+    r = A.tune_batch(None, cls, cache=A.Cache(None), timer=timer, frac=0.1, min_points=10)
+    assert r.best == (0, False, 0, 2)
+    assert all(c[0] == 0 for c in r.timings)
+
+
+def test_batch_key_buckets_counts():
+    cls = cpu_classes()
+    k1 = A.batch_key("dev", cls)
+    more = [dict(c, count=c["count"] + 1) for c in cls]
+    assert A.batch_key("dev", more) == k1 or any(
+        A._bucket(c["count"]) != A._bucket(c["count"] + 1) for c in cls)
+    assert A.batch_key("other", cls) != k1
+
+
+def test_tune_l96_candidates_cover_built_layouts(tmp_path):
+    spec = W.lorenz96_dopri(n=32, dim=16)
+    desc = L.finalize(L.make_desc(spec))
+    seen = []
+
+    def timer(cand, idx):
+        seen.append(cand)
+        assert len(idx) == 256
+        return {"waves": 3.0, "ppw": 2.0}[cand[0]] + 0.1 * cand[1]
+
+    r = A.tune_l96(None, desc, cache=A.Cache(None), timer=timer)
+    assert r.best == ("ppw", 1)
+    assert seen == list(A.L96_CANDIDATES[16])
+
+
+def test_tune_lanes_grid_caps():
+    desc = L.finalize(L.make_desc(W.rossler_dverk(n=64)))
+    r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=lambda c, idx: 1.0 + (c[0] == 0))
+    assert r.best == (512,)
+    assert set(r.timings) == {(0,), (512,), (256,)}

@@ -99,3 +99,26 @@ def test_l96_matches_oracle(gpu_ctx, n, waves, integ, adaptive, prec):
     finally:
         gpu_ctx.set_option(f"l96_waves_{n}", 0)
     assert_match(g, o, TOL_FP64, exact=True)
+
+
+L96_WAVE_LAYOUTS = [(8, 1), (8, 2), (8, 4), (8, 8), (16, 1), (16, 2), (16, 4), (32, 1), (32, 2)]
+
+
+@pytest.mark.parametrize("n,ppw", L96_WAVE_LAYOUTS)
+@pytest.mark.parametrize("integ,adaptive,prec", [(L.INT_DOPRI5, 1, L.PREC_FP64),
+                                                 (L.INT_DOPRI5, 1, L.PREC_MIXED),
+                                                 (L.INT_DVERK, 1, L.PREC_MIXED),
+                                                 (L.INT_DOPRI5, 0, L.PREC_FP64)])
+def test_l96_wave_drain_matches_oracle(gpu_ctx, n, ppw, integ, adaptive, prec):
+    """Wave-level drain (ppw points per wave64, slots refilling on their own):
+    a grid of 2 workgroups over 25 points makes every slot take several points."""
+    spec = W.lorenz96_dopri(n=5, dim=n, precision=prec, t_transient=2.0, t_average=6.0)
+    spec.update(integrator=integ, adaptive=adaptive)
+    gpu_ctx.set_option(f"l96_ppw_{n}", ppw)
+    gpu_ctx.set_option("l96_max_blocks", 2)
+    try:
+        g, o = run_both(gpu_ctx, spec)
+    finally:
+        gpu_ctx.set_option(f"l96_ppw_{n}", 0)
+        gpu_ctx.set_option("l96_max_blocks", 0)
+    assert_match(g, o, TOL_FP64, exact=True)
