@@ -199,7 +199,8 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
         "rerun_fraction": st["rerun_fraction"], "train_points": st["train_points"],
-        "verify_points": st["verify"],
+        "verify_points": st["verify"], "calibration_points": st["calib"],
+        "std_multiplier_kappa": st["kappa"], "calibration_agreement": st["calib_agreement"],
         "max_dlambda1_chosen_vs_fp64_verify": st.get("max_dlambda1_chosen_vs_fp64"),
         "class_agreement_verify": st.get("class_agreement"),
         "surrogate_answered_verify": st.get("surrogate_answered_verify"),
@@ -239,10 +240,15 @@ def bench_batch(args, ctx, n_points, wl_name):
                             params=torch.from_numpy(c["params"]).to(dev).contiguous()))
         return out
 
+    def progress(msg):
+        print(f"[batch-mix] {msg}", file=sys.stderr, flush=True)
+
     classes = device_classes(L.PREC_MIXED)
+    progress("classes " + ", ".join(f"{c['name']} {c['count']}" for c in mix))
     t0 = time.perf_counter()
     tune = A.tune_batch(ctx, classes, cache=A.Cache(None))
     tune_s = time.perf_counter() - t0
+    progress(f"tuned in {tune_s:.1f} s: {tune.best} {tune.timings}")
     b = L.Batch(ctx, classes, **A.batch_kwargs(tune.best))
     for _ in range(args.warmup):
         b.launch()
@@ -251,6 +257,7 @@ def bench_batch(args, ctx, n_points, wl_name):
     for _ in range(args.steps):
         b.launch()
         kern.append(b.last_ms())
+        progress(f"batched launch {kern[-1]:.0f} ms")
     torch.cuda.synchronize()
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     outs = [tuple(t.cpu().numpy() for t in o) for o in b.outputs]
@@ -258,9 +265,10 @@ def bench_batch(args, ctx, n_points, wl_name):
     fl = sum(F.sweep_flops(c["desc"], o[2], o[3]) for c, o in zip(classes, outs))
     # the same classes as separate single-class launches (default shapes)
     single_ms = 0.0
-    for c in classes:
+    for c, m in zip(classes, mix):
         ctx.run(c["desc"], count=c["count"], params=c["params"])
         single_ms += ctx.kernel_ms()
+        progress(f"single-class launch {m['name']}: {ctx.kernel_ms():.0f} ms")
     # plain FP64: the same batch in FP64 (same tuning)
     c64 = device_classes(L.PREC_FP64)
     try:
@@ -269,6 +277,7 @@ def bench_batch(args, ctx, n_points, wl_name):
         b64 = L.Batch(ctx, c64)
     b64.launch()
     plain_ms = b64.last_ms()
+    progress(f"FP64 batched launch {plain_ms:.0f} ms")
     outs64 = [tuple(t.cpu().numpy() for t in o) for o in b64.outputs]
     b64.close()
     per_class, worst = {}, 0.0

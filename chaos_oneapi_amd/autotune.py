@@ -3,11 +3,13 @@
 Knobs, all exposed by liblyap.so:
   - batched launch (lyap_batch_tuning): Lorenz-96 points per wavefront
     (layout), the register budget / waves per SIMD (occupancy3), the
-    persistent grid size (waves_per_cu) and the queue order of the classes;
+    persistent grid size (waves_per_cu), the queue order of the classes and
+    how many 3-D lanes gather before a renormalisation (gather);
   - single Lorenz-96 sweeps (lyap_ctx_set_option): the LDS tile shape, i.e.
     one point per block of W waves (l96_waves_N) or P points per wave
     (l96_ppw_N);
-  - single 3-D sweeps: the persistent grid size (lane_max_blocks).
+  - single 3-D sweeps: the persistent grid size (lane_max_blocks) and the
+    renormalisation gather (lane_gather).
 
 Each candidate is timed on a sample of the workload (a seeded subset of the
 points with the full integration times, so per-point costs keep their
@@ -26,10 +28,10 @@ from . import lyap as L
 
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "chaos_oneapi_amd", "autotune.json")
 
-# batched-launch candidates: (layout, occupancy3, waves_per_cu, order)
+# batched-launch candidates: (layout, occupancy3, waves_per_cu, order, gather)
 BATCH_CANDIDATES = (
-    (0, False, 0, 0), (0, True, 0, 0), (1, False, 0, 0), (1, True, 0, 0),
-    (0, False, 0, 2), (0, True, 0, 2),
+    (0, False, 0, 0, 0), (0, True, 0, 0, 0), (1, False, 0, 0, 0), (1, True, 0, 0, 0),
+    (1, True, 0, 0, 8), (1, True, 0, 0, 32), (1, True, 0, 2, 0),
 )
 
 # single Lorenz-96 sweep candidates per N: ("waves", W) or ("ppw", P)
@@ -112,10 +114,21 @@ def sample_indices(count: int, frac: float, min_points: int, seed: int):
     return np.sort(rng.choice(count, size=k, replace=False)) if k < count else np.arange(count)
 
 
-def sample_classes(classes, frac: float = 1 / 16, min_points: int = 256, seed: int = 5):
-    """A seeded subset of every class's points (explicit parameter rows)."""
+MIN_SAMPLE_TOTAL = 1 << 16
+
+
+def sample_classes(classes, frac: float = 1 / 16, min_points: int = 256, seed: int = 5,
+                   min_total: int = MIN_SAMPLE_TOTAL):
+    """A seeded subset of every class's points (explicit parameter rows).
+
+    The sample keeps at least min_total points overall (the whole batch when it
+    is smaller): a sample that does not fill the GPU several times over is
+    timed by its few slowest points, not by the launch shape."""
     import torch
 
+    total = sum(int(c["count"]) for c in classes)
+    if total and frac * total < min_total:
+        frac = min(1.0, min_total / total)
     out = []
     for j, c in enumerate(classes):
         if c.get("params") is None:
@@ -157,8 +170,7 @@ def tune_batch(ctx, classes, frac: float = 1 / 16, min_points: int = 256, repeat
 
 
 def _time_batch(ctx, sample, cand, repeats: int) -> float:
-    layout, occ3, wpc, order = cand
-    b = L.Batch(ctx, sample, layout=layout, occupancy3=occ3, waves_per_cu=wpc, order=order)
+    b = L.Batch(ctx, sample, **batch_kwargs(cand))
     try:
         ms = []
         for _ in range(repeats):
@@ -170,8 +182,9 @@ def _time_batch(ctx, sample, cand, repeats: int) -> float:
 
 
 def batch_kwargs(best) -> dict:
-    layout, occ3, wpc, order = best
-    return dict(layout=layout, occupancy3=bool(occ3), waves_per_cu=wpc, order=order)
+    layout, occ3, wpc, order = best[:4]
+    gather = best[4] if len(best) > 4 else 0
+    return dict(layout=layout, occupancy3=bool(occ3), waves_per_cu=wpc, order=order, gather=gather)
 
 
 def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 256, cache: Cache | None = None,
@@ -215,11 +228,13 @@ def _set_l96(ctx, dim: int, cand) -> None:
     ctx.set_option(f"l96_{kind}_{dim}", int(v))
 
 
-def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1024, cache: Cache | None = None,
-               force: bool = False, apply: bool = True, timer=None) -> TuneResult:
-    """Pick the persistent grid cap of a single 3-D sweep (lane_max_blocks:
-    0 = every block that fits, else that many 256-lane blocks) and
-    (apply=True) set it on ctx."""
+def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
+               cache: Cache | None = None, force: bool = False, apply: bool = True,
+               timer=None) -> TuneResult:
+    """Pick the launch shape of a single 3-D sweep and (apply=True) set it on
+    ctx: (lane_max_blocks, lane_gather) — the persistent grid cap (0 = every
+    block that fits, else that many 256-lane blocks) and how many lanes of a
+    wave gather before renormalising together."""
     import torch
 
     cache = cache if cache is not None else Cache()
@@ -233,18 +248,21 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1024, cache: C
     else:
         idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 4), dtype=torch.int64)
         timings = {}
-        for cand in ((0,), (2 * cus,), (cus,)):
+        for cand in ((0, 0), (0, 8), (0, 16), (2 * cus, 0)):
             if timer:
                 timings[cand] = timer(cand, idx)
                 continue
             ctx.set_option("lane_max_blocks", cand[0])
+            ctx.set_option("lane_gather", cand[1])
             try:
                 ctx.run(desc, idx_list=idx.to(torch.device("cuda", ctx.device)))
                 timings[cand] = ctx.kernel_ms()
             finally:
                 ctx.set_option("lane_max_blocks", 0)
+                ctx.set_option("lane_gather", 0)
         res = TuneResult(best=pick_best(timings), timings=timings)
         cache.put(key, res.best, timings)
     if apply and timer is None:
         ctx.set_option("lane_max_blocks", int(res.best[0]))
+        ctx.set_option("lane_gather", int(res.best[1]))
     return res

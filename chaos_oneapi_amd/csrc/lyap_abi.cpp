@@ -111,6 +111,7 @@ struct lyap_ctx {
     int l96_ppw[3] = {0, 0, 0};    // points per wave (wave-level drain) for N = 8, 16, 32; 0: off
     int lane_max_blocks = 0;
     int l96_max_blocks = 0;
+    int lane_gather = 0;
     unsigned long long* counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -276,8 +277,9 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipStream_t s = (hipStream_t)stream;
-    lyap::SweepIO io{first,    count,    d_idx_list, d_params, d_inits,  d_lambda,
-                     d_status, d_n_acc,  d_n_rej,    c->counter, spread_stride(count)};
+    lyap::SweepIO io{first,    count,    d_idx_list, d_params,   d_inits,
+                     d_lambda, d_status, d_n_acc,    d_n_rej,    c->counter,
+                     spread_stride(count), c->lane_gather, 0};
     if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     if (lane_model(fd.model)) {
         e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks);
@@ -339,6 +341,11 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
     if (k == "lane_max_blocks") {
         if (value < 0) return fail(LYAP_ERR_INVALID, "lane_max_blocks must be >= 0");
         c->lane_max_blocks = (int)value;
+        return LYAP_SUCCESS;
+    }
+    if (k == "lane_gather") {
+        if (value < 0 || value > 64) return fail(LYAP_ERR_INVALID, "lane_gather must be in [0, 64]");
+        c->lane_gather = (int)value;
         return LYAP_SUCCESS;
     }
     if (k == "l96_max_blocks") {
@@ -474,8 +481,8 @@ int lyap_batch_create(lyap_ctx* c, const lyap_batch_class* classes, int n_classe
     if (tuning) tu = *tuning;
     if (tu.layout < 0 || tu.layout >= lyap::kBatchLayouts)
         return fail(LYAP_ERR_INVALID, "batch layout %d is not built", tu.layout);
-    if (tu.waves_per_cu < 0 || tu.order < 0 || tu.order > 2)
-        return fail(LYAP_ERR_INVALID, "batch tuning: waves_per_cu >= 0, order in 0..2");
+    if (tu.waves_per_cu < 0 || tu.order < 0 || tu.order > 2 || tu.gather < 0 || tu.gather > 64)
+        return fail(LYAP_ERR_INVALID, "batch tuning: waves_per_cu >= 0, order in 0..2, gather in 0..64");
 
     std::vector<lyap::BatchClass> recs;
     std::vector<int> rank;
@@ -509,8 +516,9 @@ int lyap_batch_create(lyap_ctx* c, const lyap_batch_class* classes, int n_classe
         lyap::BatchClass r;
         std::memset(&r, 0, sizeof(r));
         r.d = fd;
-        r.io = lyap::SweepIO{0,         k.count,   nullptr,     k.d_params, k.d_inits, k.d_lambda,
-                             k.d_status, k.d_n_acc, k.d_n_rej, nullptr,    spread_stride(k.count)};
+        r.io = lyap::SweepIO{0,         k.count,   nullptr,    k.d_params, k.d_inits,
+                             k.d_lambda, k.d_status, k.d_n_acc, k.d_n_rej, nullptr,
+                             spread_stride(k.count), tu.gather, 0};
         r.kind = kind;
         recs.push_back(r);
         rank.push_back(tu.order == 0 ? longest_first_rank(kind)

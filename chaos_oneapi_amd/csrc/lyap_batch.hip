@@ -78,6 +78,8 @@ LYAP_D SweepIO uniform_io(const SweepIO& g) {
     io.n_rej = uni(g.n_rej);
     io.counter = uni(g.counter);
     io.stride = uni(g.stride);
+    io.gather = (int32_t)uni((uint32_t)g.gather);
+    io.pad = 0;
     return io;
 }
 

@@ -505,6 +505,9 @@ struct SweepIO {
     uint32_t* n_rej;
     unsigned long long* counter;  // work counter (persistent kernels)
     int64_t stride;               // work item i takes position (i * stride) mod count
+    int32_t gather;               // per-lane kernels: lanes at a renormalisation wait until
+                                  // this many lanes of the wave wait (<= 1: renormalise at once)
+    int32_t pad;
 };
 
 // position of work item `item`: a multiplicative permutation of [0, count)

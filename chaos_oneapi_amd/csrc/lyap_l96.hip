@@ -6,7 +6,8 @@
 // replicated in every N-lane group (the replicas cost no extra instructions),
 // so the Jacobian at each stage state is available to every vector's lanes.
 //   - neighbour couplings x_{i+1}, x_{i-1}, x_{i-2} (and of each vector) are
-//     lane shuffles inside the N-lane group;
+//     moves inside the N-lane group (DPP row rotations for N = 16, lane
+//     shuffles otherwise);
 //   - the error norm and the Jacobian-norm step cap are xor-butterfly
 //     reductions over the group (offsets N/2 .. 1), identical in every lane;
 //   - renormalisation stages the N x N deviation matrix in LDS, transposed so
@@ -44,10 +45,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
     const int lane = tid & 63;
     const int i = lane % N;
     const int gid = (tid >> 6) * G + lane / N;
-    const int base_lane = lane - i;
-    const int src_p1 = base_lane + (i + 1) % N;
-    const int src_m1 = base_lane + (i + N - 1) % N;
-    const int src_m2 = base_lane + (i + N - 2) % N;
+    const Ring<N> ring(lane);
 
     const int64_t n_tr = d.n_renorm_transient;
     const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
@@ -84,7 +82,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
         uint32_t st = LYAP_PT_OK;
         double k1 = 0.0;
         if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
-            const double xp1 = __shfl(x, src_p1), xm1 = __shfl(x, src_m1), xm2 = __shfl(x, src_m2);
+            const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
             k1 = fma(xp1 - xm2, xm1, fma(-dd, x, F));
         }
 
@@ -99,8 +97,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                 }
                 t_next = (double)(mr + 1) * t_renorm;
                 {
-                    const double xp1 = __shfl(x, src_p1), xm1 = __shfl(x, src_m1),
-                                 xm2 = __shfl(x, src_m2);
+                    const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
                     const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
                     const double jn = group_max<N>(row);
                     if (h_try * jn > 1.0) h_try = 1.0 / jn;
@@ -139,8 +136,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                         any = true;
                     }
                     const double arg = any ? fma(h_try, acc, x) : x;
-                    const double xp1 = __shfl(arg, src_p1), xm1 = __shfl(arg, src_m1),
-                                 xm2 = __shfl(arg, src_m2);
+                    const double xp1 = ring.p1(arg), xm1 = ring.m1(arg), xm2 = ring.m2(arg);
                     jm1[s] = (T)xm1;
                     jd[s] = (T)(xp1 - xm2);
                     if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
@@ -208,8 +204,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                             any = true;
                         }
                         const T varg = any ? fma(hT, acc, v0) : v0;
-                        const T vp1 = __shfl(varg, src_p1), vm1 = __shfl(varg, src_m1),
-                                vm2 = __shfl(varg, src_m2);
+                        const T vp1 = ring.p1(varg), vm1 = ring.m1(varg), vm2 = ring.m2(varg);
                         kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
                     }
                     T accv = T(0);

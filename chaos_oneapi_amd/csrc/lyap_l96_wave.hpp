@@ -1,12 +1,15 @@
 // Lorenz-96 device code shared by the block-per-point sweep kernel
 // (lyap_l96.hip) and the wave-level drain used by the batched launch.
 //
-// Wave-level drain layout: a wave holds P points ("slots"), LP = 64/P lanes
-// each.  Inside a slot, lane u owns component i = u % N of the base state and
-// of E = N/G deviation vectors, G = LP/N being the vector groups per point:
-// lane u holds the vectors r = u/N + G*m, m < E.  The slot's base state is
-// replicated in each of its G groups (G = 1 at P = 64/N: no replication).
-//   - neighbour couplings are shuffles inside the N-lane group;
+// Wave-level drain layout: a wave holds P points ("slots") on LP = 64/P lanes
+// each, as G = LP/N groups of N lanes (WaveMap: consecutive lanes for N >= 16,
+// two groups interleaved per 16-lane row for N = 8).  Lane (group vg of the
+// slot, component i) owns component i of the base state and of E = N/G
+// deviation vectors, r = vg + G*m, m < E; u = vg*N + i is its position in the
+// slot.  The slot's base state is replicated in each of its G groups (G = 1 at
+// P = 64/N: no replication).
+//   - neighbour couplings are moves inside the N-lane group (Ring: DPP row
+//     rotations for N = 8 and 16, lane shuffles for N = 32);
 //   - the error norm and the Jacobian-norm cap are xor butterflies over the
 //     group, bit-identical in every lane of the group;
 //   - renormalisation stages the slot's N x N deviation matrix in the wave's
@@ -43,6 +46,93 @@ LYAP_D double group_max(double v) {
 
 LYAP_D bool wave_uniform(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
 
+// row rotation by DPP: lane i of each 16-lane row receives lane (i - R) mod 16
+template <int R>
+LYAP_D float dpp_ror(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x120 + R,
+                                                                 0xF, 0xF, false));
+}
+template <int R>
+LYAP_D double dpp_ror(double v) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x120 + R, 0xF, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x120 + R, 0xF, 0xF,
+                                                              false);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
+// Couplings x_{i+1}, x_{i-1}, x_{i-2} of component i inside its N-lane group
+// (periodic).  CS is the lane stride between consecutive components: 1 for
+// groups of consecutive lanes, 2 for two N = 8 groups interleaved in one
+// 16-lane row.  Groups that are whole DPP rows (N = 16, CS = 1; N = 8, CS = 2)
+// move by row rotations on the VALU; others by lane shuffles (ds_bpermute).
+// Every lane of a group is active whenever one is (the callers' control is
+// group-uniform).
+template <int N, int CS = 1>
+struct Ring {
+    static constexpr bool kDpp = N * CS == 16;
+    int src_p1, src_m1, src_m2;
+    LYAP_D explicit Ring(int lane) {
+        static_assert(CS == 1 || (CS == 2 && N == 8), "component stride");
+        const int i = CS == 1 ? lane % N : (lane & 15) / CS;
+        const int base = lane - i * CS;
+        src_p1 = base + ((i + 1) % N) * CS;
+        src_m1 = base + ((i + N - 1) % N) * CS;
+        src_m2 = base + ((i + N - 2) % N) * CS;
+    }
+    template <class T>
+    LYAP_D T p1(T v) const {
+        if constexpr (kDpp) return dpp_ror<16 - CS>(v);
+        else return __shfl(v, src_p1);
+    }
+    template <class T>
+    LYAP_D T m1(T v) const {
+        if constexpr (kDpp) return dpp_ror<CS>(v);
+        else return __shfl(v, src_m1);
+    }
+    template <class T>
+    LYAP_D T m2(T v) const {
+        if constexpr (kDpp) return dpp_ror<2 * CS>(v);
+        else return __shfl(v, src_m2);
+    }
+};
+
+// Lanes of the wave-level drain: lane -> (group q, component c).  N >= 16:
+// group q holds lanes q*N .. q*N+N-1.  N = 8: the two groups of each 16-lane
+// row are interleaved (lane = 16*row + 2*c + h, q = 2*row + h), so that the
+// component rotations above are row rotations.
+template <int N>
+struct WaveMap {
+    static constexpr int CS = N >= 16 ? 1 : 16 / N;
+    LYAP_D static int comp(int lane) { return CS == 1 ? lane % N : (lane & 15) / CS; }
+    LYAP_D static int group(int lane) { return CS == 1 ? lane / N : (lane >> 4) * CS + (lane & 15) % CS; }
+    LYAP_D static int lane_of(int q, int c) { return CS == 1 ? q * N + c : 16 * (q / CS) + c * CS + q % CS; }
+    // ballot bits of the lanes of group q
+    LYAP_D static uint64_t group_bits(int q) {
+        if constexpr (CS == 1) return (N == 64 ? ~0ULL : ((1ULL << N) - 1ULL)) << (q * N);
+        else return 0x5555ULL << (16 * (q / CS) + q % CS);
+    }
+};
+
+// butterfly reductions over a group whose components sit CS lanes apart (the
+// pairings, hence the order of the additions, are those of group_sum<N>)
+template <int N, int CS>
+LYAP_D double group_sum_s(double v) {
+#pragma unroll
+    for (int o = N / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o * CS);
+    return v;
+}
+
+template <int N, int CS>
+LYAP_D double group_max_s(double v) {
+#pragma unroll
+    for (int o = N / 2; o >= 1; o >>= 1) {
+        const double w = __shfl_xor(v, o * CS);
+        v = v > w ? v : w;
+    }
+    return v;
+}
+
 // orders this wave's LDS writes before its later LDS reads (cross-lane)
 LYAP_D void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -66,23 +156,26 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
     constexpr int E = K / G;    // vectors per lane
     constexpr int S = TB::S;
     static_assert(64 % P == 0 && LP % N == 0 && K % G == 0 && E >= 1, "layout");
-    constexpr uint64_t SLOT_MASK = LP == 64 ? ~0ULL : ((1ULL << LP) - 1ULL);
 
+    using Map = WaveMap<N>;
+    constexpr int CS = Map::CS;
     const int lane = (int)__lane_id();
-    const int lead = lane - lane % LP;  // the slot's first lane
-    const int u = lane % LP;
-    const int i = u % N;
-    const int vg = u / N;
-    const int base_lane = lane - i;
-    const int src_p1 = base_lane + (i + 1) % N;
-    const int src_m1 = base_lane + (i + N - 1) % N;
-    const int src_m2 = base_lane + (i + N - 2) % N;
+    const int q = Map::group(lane);   // N-lane group of the wave
+    const int i = Map::comp(lane);    // component
+    const int slot = q / G;           // point slot
+    const int vg = q % G;             // vector group within the slot
+    const int u = vg * N + i;         // position within the slot
+    const int lead = Map::lane_of(slot * G, 0);  // the slot's first lane (u = 0)
+    uint64_t slot_bits = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) slot_bits |= Map::group_bits(slot * G + g);
+    const Ring<N, CS> ring(lane);
     constexpr int RS = N + 1;  // LDS row stride (doubles)
-    double* sV = lds + (lane / LP) * (N * RS);
+    double* sV = lds + slot * (N * RS);
 
     // a decision of the slot: the value its first lane computed
     auto slot_uniform = [lead](bool b) { return __shfl((int)b, lead) != 0; };
-    auto slot_any = [lead](bool b) { return ((__ballot(b) >> lead) & SLOT_MASK) != 0ULL; };
+    auto slot_any = [slot_bits](bool b) { return (__ballot(b) & slot_bits) != 0ULL; };
 
     const int64_t n_tr = d.n_renorm_transient;
     const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
@@ -129,8 +222,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 ctrl.log_fac_old = log_floor;
                 ctrl.rejected = false;
                 if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
-                    const double xp1 = __shfl(x, src_p1), xm1 = __shfl(x, src_m1),
-                                 xm2 = __shfl(x, src_m2);
+                    const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
                     k1 = fma(xp1 - xm2, xm1, fma(-dd, x, F));
                 }
                 active = true;
@@ -152,9 +244,9 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 go = false;
             } else {
                 t_next = (double)(mr + 1) * t_renorm;
-                const double xp1 = __shfl(x, src_p1), xm1 = __shfl(x, src_m1), xm2 = __shfl(x, src_m2);
+                const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
                 const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
-                const double jn = group_max<N>(row);
+                const double jn = group_max_s<N, CS>(row);
                 if (h_try * jn > 1.0) h_try = 1.0 / jn;
                 if (slot_uniform(t_next - t <= 1.0625 * h_try)) {
                     h_try = t_next - t;
@@ -192,8 +284,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     any = true;
                 }
                 const double arg = any ? fma(h_try, acc, x) : x;
-                const double xp1 = __shfl(arg, src_p1), xm1 = __shfl(arg, src_m1),
-                             xm2 = __shfl(arg, src_m2);
+                const double xp1 = ring.p1(arg), xm1 = ring.m1(arg), xm2 = ring.m2(arg);
                 jm1[s] = (T)xm1;
                 jd[s] = (T)(xp1 - xm2);
                 if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
@@ -222,8 +313,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 const double ev = h_try * acce;
                 const double ax = fabs(x), an = fabs(xn);
                 const double sk = fma(rtol, ax > an ? ax : an, atol);
-                const double q = ev / sk;
-                const double err = sqrt(group_sum<N>(q * q) / (double)N);
+                const double qe = ev / sk;
+                const double err = sqrt(group_sum_s<N, CS>(qe * qe) / (double)N);
                 if (slot_uniform(!(err < kInf))) {
                     st = LYAP_PT_NONFINITE;
                     finished = true;
@@ -261,8 +352,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                             any = true;
                         }
                         const T varg = any ? fma(hT, acc, v0) : v0;
-                        const T vp1 = __shfl(varg, src_p1), vm1 = __shfl(varg, src_m1),
-                                vm2 = __shfl(varg, src_m2);
+                        const T vp1 = ring.p1(varg), vm1 = ring.m1(varg), vm2 = ring.m2(varg);
                         kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
                     }
                     T accv = T(0);
@@ -316,12 +406,12 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                         }
                         wave_lds_sync();
                         if (u > j && u < K) {
-                            const double* q = sV + j * RS;
-                            double cdot = q[0] * row[0];
+                            const double* qj = sV + j * RS;
+                            double cdot = qj[0] * row[0];
 #pragma unroll 8
-                            for (int c = 1; c < N; ++c) cdot = fma(q[c], row[c], cdot);
+                            for (int c = 1; c < N; ++c) cdot = fma(qj[c], row[c], cdot);
 #pragma unroll 8
-                            for (int c = 0; c < N; ++c) row[c] = fma(-cdot, q[c], row[c]);
+                            for (int c = 0; c < N; ++c) row[c] = fma(-cdot, qj[c], row[c]);
                         }
                     }
                     wave_lds_sync();

@@ -192,6 +192,25 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     uint32_t nacc = 0, nrej = 0;
     Ctrl ctrl{1e-4, log_floor, false};
     bool active = false, exhausted = false;
+    // a lane that reached a renormalisation boundary waits (pending) until
+    // io.gather lanes of the wave wait, or every active lane does; then they
+    // renormalise together.  Each lane's arithmetic is unchanged, only the
+    // wave executes the MGS block less often with most lanes masked off.
+    bool pending = false;
+    const int gather = io.gather;
+
+    auto write_out = [&](uint32_t st) {
+        for (int r = 0; r < n_exps; ++r) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (q == r) s = sum[q];
+            io.lambda[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
+        }
+        if (io.status) io.status[item] = st;
+        if (io.n_acc) io.n_acc[item] = nacc;
+        if (io.n_rej) io.n_rej[item] = nrej;
+    };
 
     for (;;) {
         const int64_t got = grab_items(!active && !exhausted, io.counter);
@@ -228,11 +247,33 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
             }
         }
         if (!__any(active)) break;
-        if (!active) continue;
+        {
+            const uint64_t pm = __ballot(active && pending);
+            if (pm) {
+                const int np = __popcll(pm);
+                if (np >= gather || np == __popcll(__ballot(active))) {
+                    if (pending) {
+                        double ln[K];
+                        mgs_rows<K, N>(V, ln);
+                        ++m;
+                        if (m > n_tr) {
+#pragma unroll
+                            for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
+                        }
+                        pending = false;
+                        if (m == m_total) {
+                            write_out(LYAP_PT_OK);
+                            active = false;
+                        }
+                    }
+                }
+            }
+        }
+        if (!active || pending) continue;
 
         uint32_t st = LYAP_PT_OK;
         bool finished = false;
-        double xn[N], ln[K];
+        double xn[N];
         typename M::template J<TX> X[S];
         if constexpr (ADAPT) {
             if ((int64_t)nacc + (int64_t)nrej >= max_steps) {
@@ -285,13 +326,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                                 st = LYAP_PT_ESCAPED;
                                 finished = true;
                             } else if (clipped) {
-                                mgs_rows<K, N>(V, ln);
-                                ++m;
-                                if (m > n_tr) {
-#pragma unroll
-                                    for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
-                                }
-                                if (m == m_total) finished = true;
+                                pending = true;
                             }
                         }
                     }
@@ -316,27 +351,12 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                     st = LYAP_PT_ESCAPED;
                     finished = true;
                 } else {
-                    mgs_rows<K, N>(V, ln);
-                    ++m;
-                    if (m > n_tr) {
-#pragma unroll
-                        for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
-                    }
-                    if (m == m_total) finished = true;
+                    pending = true;
                 }
             }
         }
         if (finished) {
-            for (int r = 0; r < n_exps; ++r) {
-                double s = 0.0;
-#pragma unroll
-                for (int q = 0; q < K; ++q)
-                    if (q == r) s = sum[q];
-                io.lambda[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
-            }
-            if (io.status) io.status[item] = st;
-            if (io.n_acc) io.n_acc[item] = nacc;
-            if (io.n_rej) io.n_rej[item] = nrej;
+            write_out(st);
             active = false;
         }
     }

@@ -10,6 +10,7 @@
 // Adam runs on the FP32 master weights and re-issues the BF16 copies.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -30,6 +31,7 @@ namespace mlp {
 constexpr int kOut = 5;       // mu1, mu2, mu3, log-variance, escape logit
 constexpr int kOutPad = 16;   // head rows padded to one MFMA tile
 constexpr int kLayers = 4;
+constexpr int kMaxSplits = 16;  // split-K of the weight-gradient products
 
 LYAP_HD uint16_t f2bf(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -79,6 +81,53 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
             *c = (beta != 0.f ? beta * *c : 0.f) + acc[j][i] + bv;
         }
     }
+}
+
+// Split-K form for the weight gradients (M, N <= 256, K = the batch): split
+// s of the waves computes C_s over k in [s*kc, min(K, (s+1)*kc)) into
+// P + s*M*N; k_sum_splits adds the splits in order (deterministic).
+template <int NB>
+__global__ __launch_bounds__(256) void k_gemm_nt_splitk(const uint16_t* __restrict__ A,
+                                                        const uint16_t* __restrict__ B,
+                                                        float* __restrict__ P, int M, int N, int K,
+                                                        int kc, int splits) {
+    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63);
+    const int tiles_n = N / (16 * NB);
+    const int tiles = (M / 16) * tiles_n;
+    const int split = wave / tiles, t = wave % tiles;
+    if (split >= splits) return;
+    const int m0 = (t / tiles_n) * 16;
+    const int n0 = (t % tiles_n) * (16 * NB);
+    const int k_lo = split * kc, k_hi = min(K, k_lo + kc);
+    const int r = lane & 15, q = lane >> 4;
+    v4f acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+    const uint16_t* a_row = A + (size_t)(m0 + r) * K + 4 * q;
+    for (int k0 = k_lo; k0 < k_hi; k0 += 16) {
+        const v4s a = *reinterpret_cast<const v4s*>(a_row + k0);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const v4s b = *reinterpret_cast<const v4s*>(B + (size_t)(n0 + 16 * j + r) * K + 4 * q + k0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+    float* Ps = P + (size_t)split * M * N;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int col = n0 + 16 * j + r;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) Ps[(size_t)(m0 + 4 * q + i) * N + col] = acc[j][i];
+    }
+}
+
+__global__ void k_sum_splits(const float* __restrict__ P, int splits, int64_t mn, float* __restrict__ C) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mn) return;
+    float s = P[i];
+    for (int k = 1; k < splits; ++k) s += P[(size_t)k * mn + i];
+    C[i] = s;
 }
 
 // ---------------------------------------------------------------------------
@@ -180,13 +229,24 @@ __global__ void k_head_loss(const float* __restrict__ Z, const double* __restric
     if (loss && b < n_valid) atomicAdd(loss, l * inv_n);
 }
 
-// db[j] = sum_b dZ[b][j]
-__global__ void k_colsum(const float* __restrict__ dZ, int n, int H, float* __restrict__ db) {
-    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (j >= H) return;
+// db[j] = sum_b dZ[b][j]: one block of 64 columns x 16 row groups; thread
+// (c, g) sums rows g, g+16, ... in order (coalesced across c), then the 16
+// partial sums are added in a fixed tree, so the result is deterministic
+__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ dZ, int n, int H,
+                                                 float* __restrict__ db) {
+    __shared__ float part[16][64];
+    const int c = (int)(threadIdx.x & 63), g = (int)(threadIdx.x >> 6);
+    const int j = (int)blockIdx.x * 64 + c;
     float s = 0.f;
-    for (int b = 0; b < n; ++b) s += dZ[(size_t)b * H + j];
-    db[j] = s;
+    if (j < H)
+        for (int b = g; b < n; b += 16) s += dZ[(size_t)b * H + j];
+    part[g][c] = s;
+    __syncthreads();
+    for (int w = 8; w >= 1; w >>= 1) {
+        if (g < w) part[g][c] += part[g + w][c];
+        __syncthreads();
+    }
+    if (g == 0 && j < H) db[j] = part[0][c];
 }
 
 // dZ (bf16, row-major) -> fp32 for the bias reduction of hidden layers
@@ -277,6 +337,7 @@ struct lyap_mlp {
     float* dh = nullptr;
     float* dzf = nullptr;
     float* loss = nullptr;
+    float* split_ws = nullptr;  // kMaxSplits x (largest weight matrix) partial products
     int64_t step = 0;
     std::vector<void*> allocs;
 };
@@ -344,6 +405,9 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->loss, 4);
+    size_t max_w = 0;
+    for (int l = 0; l < kLayers; ++l) max_w = std::max(max_w, (size_t)dims[l + 1] * dims[l]);
+    rc |= mlp_alloc(m, (void**)&m->split_ws, (size_t)kMaxSplits * max_w * 4);
     if (rc != LYAP_SUCCESS) {
         lyap_mlp_destroy(m);
         return LYAP_ERR_HIP;
@@ -392,7 +456,8 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
         hipLaunchKernelGGL(k_issue_bf16, dim3((n + 255) / 256), dim3(256), 0, 0, m->master + m->off_w[l],
                            dims[l + 1], dims[l], m->wt[l], m->wn[l]);
     }
-    e = hipDeviceSynchronize();
+    // the null stream only: work of the caller's other streams is not waited for
+    e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) {
         lyap_mlp_destroy(m);
         return mfail(LYAP_ERR_HIP, "mlp init: %s", hipGetErrorString(e));
@@ -408,6 +473,29 @@ static void gemm(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, 
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     if (nb == 4) hipLaunchKernelGGL(k_gemm_nt<4>, grid, block, 0, s, A, B, C, bias, M, N, K, beta);
     else hipLaunchKernelGGL(k_gemm_nt<1>, grid, block, 0, s, A, B, C, bias, M, N, K, beta);
+}
+
+// weight-gradient product (small M x N, long K): split-K when K is long
+static void gemm_wgrad(lyap_mlp* m, hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, int M,
+                       int N, int K) {
+    int splits = K / 256;
+    if (splits > kMaxSplits) splits = kMaxSplits;
+    if (splits < 2) {
+        gemm(s, A, B, C, nullptr, M, N, K, 0.f);
+        return;
+    }
+    const int kc = ((K + splits - 1) / splits + 15) / 16 * 16;
+    splits = (K + kc - 1) / kc;
+    const int nb = (N % 64 == 0) ? 4 : 1;
+    const int waves = splits * (M / 16) * (N / (16 * nb));
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    if (nb == 4)
+        hipLaunchKernelGGL(k_gemm_nt_splitk<4>, grid, block, 0, s, A, B, m->split_ws, M, N, K, kc, splits);
+    else
+        hipLaunchKernelGGL(k_gemm_nt_splitk<1>, grid, block, 0, s, A, B, m->split_ws, M, N, K, kc, splits);
+    const int64_t mn = (int64_t)M * N;
+    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, m->split_ws, splits,
+                       mn, C);
 }
 
 static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s) {
@@ -437,8 +525,8 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
                        m->dzf, m->loss);
     for (int l = kLayers - 1; l >= 0; --l) {
         // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T
-        gemm(s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], nullptr, D[l + 1], D[l], n_pad, 0.f);
-        hipLaunchKernelGGL(k_colsum, dim3((D[l + 1] + 63) / 64), dim3(64), 0, s, m->dzf, n_pad, D[l + 1],
+        gemm_wgrad(m, s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad);
+        hipLaunchKernelGGL(k_colsum, dim3((D[l + 1] + 63) / 64), dim3(1024), 0, s, m->dzf, n_pad, D[l + 1],
                            m->grad + m->off_b[l]);
         if (l == 0) break;
         // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU
@@ -535,7 +623,7 @@ int lyap_mlp_set_params(lyap_mlp* m, const float* host, int64_t n) {
         hipLaunchKernelGGL(k_issue_bf16, dim3((nw + 255) / 256), dim3(256), 0, 0, m->master + m->off_w[l],
                            m->dims[l + 1], m->dims[l], m->wt[l], m->wn[l]);
     }
-    MLP_HIP(hipDeviceSynchronize());
+    MLP_HIP(hipStreamSynchronize(nullptr));
     return LYAP_SUCCESS;
 }
 

@@ -82,7 +82,8 @@ class BatchClassRec(ctypes.Structure):
 class BatchTuning(ctypes.Structure):
     """lyap_batch_tuning"""
     _fields_ = [("layout", ctypes.c_int32), ("occupancy3", ctypes.c_int32),
-                ("waves_per_cu", ctypes.c_int32), ("order", ctypes.c_int32)]
+                ("waves_per_cu", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("gather", ctypes.c_int32)]
 
 
 class ModelInfo(ctypes.Structure):
@@ -285,7 +286,7 @@ class Batch:
     """
 
     def __init__(self, ctx: Context, classes, layout: int = 0, occupancy3: bool = False,
-                 waves_per_cu: int = 0, order: int = 0):
+                 waves_per_cu: int = 0, order: int = 0, gather: int = 0):
         import torch
 
         dev = torch.device("cuda", ctx.device)
@@ -312,7 +313,7 @@ class Batch:
             r.d_inits = c["inits"].data_ptr() if c.get("inits") is not None else None
             r.d_lambda, r.d_status = lam.data_ptr(), st.data_ptr()
             r.d_n_acc, r.d_n_rej = na.data_ptr(), nr.data_ptr()
-        tu = BatchTuning(int(layout), int(bool(occupancy3)), int(waves_per_cu), int(order))
+        tu = BatchTuning(int(layout), int(bool(occupancy3)), int(waves_per_cu), int(order), int(gather))
         self._h = ctypes.c_void_p()
         check(load().lyap_batch_create(ctx._h, recs, len(classes), ctypes.byref(tu),
                                        ctypes.byref(self._h)))

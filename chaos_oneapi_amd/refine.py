@@ -5,11 +5,17 @@ Pipeline, all on the device through liblyap.so:
      computed by the method in the chosen precision;
   2. the surrogate network (lyap_mlp_*) trained on it with Adam;
   3. inference over the fine grid;
-  4. the method is rerun where the prediction is uncertain (std of lambda1
-     above std_thr), near the chaos boundary (|mu1| < lam_thr), or where the
-     escape probability is not decided, plus on a seeded random verification
-     subsample (verify_frac of the grid);
+  4. the method is rerun where the prediction is uncertain (kappa * std of
+     lambda1 above std_thr), near the chaos boundary (|mu1| < lam_thr), or
+     where the escape probability is not decided, plus on seeded random
+     calibration and verification subsamples (calib_frac, verify_frac of the
+     grid, disjoint); kappa >= 1 is the smallest multiplier at which the
+     surrogate's answers on the calibration subsample agree with the method's
+     class on calib_target of them (the predicted std alone is not calibrated);
   5. the plain FP64 method judges the verification subsample.
+The calibration and verification runs (chosen form, and the judge) do not
+depend on the surrogate, so they run on a side stream, with a context of their
+own, concurrently with steps 1-3; the rerun of step 4 skips those points.
 The returned map takes the method's values where it ran and the surrogate's
 elsewhere.  Gates reported on the verification subsample: max |dlambda1|
 between the chosen form and FP64 on computed points, and agreement of the
@@ -53,18 +59,51 @@ class RefineResult:
     stats: dict = field(default_factory=dict)
 
 
+KAPPAS = (1.0, 1.5, 2.0, 3.0, 5.0, 8.0, 16.0)
+
+
+def answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi):
+    """Points the surrogate answers at std multiplier kappa: confidently
+    escaping, or decided (not escaping), away from lambda1 = 0 and with
+    kappa * std below the threshold.  Works on numpy arrays and torch tensors."""
+    escaping = pbad >= bad_hi
+    settled = (kappa * std1 <= std_thr) & (abs(mu1) >= lam_thr) & (pbad <= bad_lo)
+    return escaping | settled
+
+
+def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target):
+    """Smallest kappa in KAPPAS whose answerable calibration points agree with
+    their computed class on at least `target` of them (inf: answer nothing).
+    pred [n,5] numpy, lam/status the method's values at the same points."""
+    import numpy as np
+
+    computed = classify(lam[:, 0], status)
+    sur_status = np.where(pred[:, 4] >= 0.5, 2, 0)
+    sur = classify(pred[:, 0], sur_status)
+    finite = np.isfinite(pred[:, 0])
+    for kappa in KAPPAS:
+        a = answerable(pred[:, 0], pred[:, 3], pred[:, 4], kappa, std_thr, lam_thr, bad_lo, bad_hi) & finite
+        if not a.any():
+            return kappa, 1.0, 0
+        agree = float(np.mean(sur[a] == computed[a]))
+        if agree >= target:
+            return kappa, agree, int(a.sum())
+    return float("inf"), 1.0, 0
+
+
 def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4096,
            lr: float = 1e-3, seed: int = 7, verify_frac: float = 0.01, std_thr: float = 0.01,
            lam_thr: float = 0.02, bad_lo: float = 0.02, bad_hi: float = 0.98,
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
-           fourier_sigma: float = 6.0) -> RefineResult:
+           fourier_sigma: float = 6.0, calib_frac: float = 0.01,
+           calib_target: float = 0.9995) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
     tm = {}
 
-    def mark(name, t0):
-        torch.cuda.synchronize()
+    def mark_stream(name, t0, stream):
+        stream.synchronize()
         tm[name] = time.perf_counter() - t0
         return time.perf_counter()
 
@@ -74,6 +113,33 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     end2 = fd.start2 + fd.delta2 * (fd.n2 - 1)
     t_all = t = time.perf_counter()
 
+    # the verification subsample does not depend on the surrogate: its chosen-form
+    # values and the FP64 judge run on a side stream (own context, so its own
+    # work counter) while the main stream computes the training map, trains and
+    # infers
+    gv = torch.Generator(device="cpu").manual_seed(seed + 1)
+    n_ver = max(1, int(round(verify_frac * n)))
+    n_cal = max(1, int(round(calib_frac * n)))
+    perm = torch.randperm(n, generator=gv)
+    verify = perm[:n_ver].sort().values.to(dev)
+    calib = perm[n_ver:n_ver + n_cal].sort().values.to(dev)
+    main = torch.cuda.current_stream(dev)
+    side, side_j = torch.cuda.Stream(dev), torch.cuda.Stream(dev)
+    side.wait_stream(main)
+    side_j.wait_stream(main)
+    side_ctx, judge_ctx = L.Context(ctx.device), L.Context(ctx.device)
+    both = torch.cat([calib, verify]).contiguous()
+    with torch.cuda.stream(side):
+        # one launch for both subsamples: one tail instead of two
+        lam_cv, st_cv, _, _ = side_ctx.run(fd, idx_list=both, stream=side)
+        lam_c, st_c = lam_cv[:n_cal], st_cv[:n_cal]
+        lam_v, st_v = lam_cv[n_cal:], st_cv[n_cal:]
+    if judge:
+        with torch.cuda.stream(side_j):
+            pdesc = L.SweepDesc.from_buffer_copy(fd)
+            pdesc.precision = L.PREC_FP64
+            lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
+
     # 1. training map
     tspec = dict(fine)
     tspec.update(W.grid_fields(fd.sweep_idx1, fd.start1, end1, train_n, fd.sweep_idx2, fd.start2,
@@ -81,9 +147,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     td = L.finalize(L.make_desc(tspec))
     lam_t, st_t, _, _ = ctx.run(td)
     p_t = grid_params(td, torch.arange(train_n * train_n, device=dev))
-    t = mark("train_map", t)
+    t = mark_stream("train_map", t, main)
 
-    # 2. training
+    # 2. training (one shuffle per epoch; batches are contiguous slices)
     ok = st_t == 0
     lam_ok = lam_t[ok]
     ym = lam_ok.mean(dim=0).tolist() if lam_ok.shape[0] else [0.0, 0.0, 0.0]
@@ -98,33 +164,47 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     losses = []
     for ep in range(epochs):
         perm = torch.randperm(n_t, generator=gen).to(dev)
+        pp, lp, sp = p_t[perm], lam_t[perm], st_t[perm]
         for b0 in range(0, n_t - batch + 1, batch):
-            sel = perm[b0:b0 + batch]
-            net.train_step(p_t[sel].contiguous(), lam_t[sel].contiguous(), st_t[sel].contiguous(), loss)
-        losses.append(loss.item())
-    t = mark("train", t)
+            net.train_step(pp[b0:b0 + batch], lp[b0:b0 + batch], sp[b0:b0 + batch], loss)
+        losses.append(loss)
+        loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    t = mark_stream("train", t, main)
 
     # 3. inference
     pred = torch.empty((n, 5), dtype=torch.float32, device=dev)
     for c0 in range(0, n, infer_batch):
         c1 = min(n, c0 + infer_batch)
         net.forward(grid_params(fd, torch.arange(c0, c1, device=dev)), pred[c0:c1])
-    t = mark("infer", t)
+    t = mark_stream("infer", t, main)
 
-    # 4. rerun set and verification subsample
+    # 4. calibration: the std multiplier at which the surrogate's answers on the
+    #    calibration subsample agree with the method (the verification
+    #    subsample stays out of it), then the rerun set: every point the
+    #    surrogate does not answer; calibration and verification points are
+    #    computed on the side stream already
+    side.synchronize()
+    kappa, cal_agree, cal_answered = calibrate_kappa(
+        pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
+        bad_lo, bad_hi, calib_target)
+    tm["calibrate"] = time.perf_counter() - t
     mu1, std1, pbad = pred[:, 0], pred[:, 3], pred[:, 4]
-    undecided = (pbad > bad_lo) & (pbad < bad_hi)
-    flagged = (std1 > std_thr) | (mu1.abs() < lam_thr) | undecided | ~torch.isfinite(mu1)
-    flagged &= ~(pbad >= bad_hi)  # confidently escaping: answered by the surrogate
-    gv = torch.Generator(device="cpu").manual_seed(seed + 1)
-    n_ver = max(1, int(round(verify_frac * n)))
-    verify = torch.randperm(n, generator=gv)[:n_ver].sort().values.to(dev)
-    rerun_mask = flagged.clone()
-    rerun_mask[verify] = True
-    rerun_idx = torch.nonzero(rerun_mask).flatten().to(torch.int64).contiguous()
+    if kappa == float("inf"):
+        flagged = torch.ones(n, dtype=torch.bool, device=dev)
+    else:
+        flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi)
+        flagged |= ~torch.isfinite(mu1)
+    vmask = torch.zeros(n, dtype=torch.bool, device=dev)
+    vmask[verify] = True
+    cmask = torch.zeros(n, dtype=torch.bool, device=dev)
+    cmask[calib] = True
+    rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask).flatten().to(torch.int64).contiguous()
     lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
-    t = mark("rerun", t)
+    t = mark_stream("rerun", t, main)
     rerun_kernel_ms = ctx.kernel_ms()
+    torch.cuda.synchronize(dev)
+    side_ctx.close()
+    judge_ctx.close()
 
     # 5. assemble
     lam = pred[:, :3].to(torch.float64)
@@ -132,27 +212,25 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     lam[pbad >= 0.5] = float("nan")
     lam[rerun_idx] = lam_r
     status[rerun_idx] = st_r
+    lam[verify] = lam_v
+    status[verify] = st_v
+    lam[calib] = lam_c
+    status[calib] = st_c
+    rerun_mask = flagged | vmask | cmask
     tm["total_chosen"] = time.perf_counter() - t_all
 
-    stats = dict(points=n, train_points=n_t, rerun=int(rerun_idx.numel()),
-                 flagged=int(flagged.sum().item()), verify=n_ver,
-                 rerun_fraction=float(rerun_idx.numel()) / n, losses=losses,
-                 rerun_kernel_ms=rerun_kernel_ms,
+    n_rerun = int(rerun_idx.numel()) + n_ver + n_cal
+    stats = dict(points=n, train_points=n_t, rerun=n_rerun,
+                 flagged=int(flagged.sum().item()), verify=n_ver, calib=n_cal,
+                 kappa=kappa, calib_agreement=cal_agree, calib_answered=cal_answered,
+                 rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
+                 rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),
                  rerun_acc_steps=float(na_r.double().sum().item()),
                  rerun_rej_steps=float(nr_r.double().sum().item()))
     if judge:
-        t = time.perf_counter()
-        pdesc = L.SweepDesc.from_buffer_copy(fd)
-        pdesc.precision = L.PREC_FP64
-        lam_j, st_j, _, _ = ctx.run(pdesc, idx_list=verify.contiguous())
-        t = mark("judge", t)
         v = verify.cpu().numpy()
         lj, sj = lam_j.cpu().numpy(), st_j.cpu().numpy()
-        # chosen-form computed values at the verification points
-        pos = {int(k): i for i, k in enumerate(rerun_idx.cpu().numpy())}
-        lr_np, sr_np = lam_r.cpu().numpy(), st_r.cpu().numpy()
-        lc = np.array([lr_np[pos[k]] for k in v])
-        sc = np.array([sr_np[pos[k]] for k in v])
+        lc, sc = lam_v.cpu().numpy(), st_v.cpu().numpy()
         both = (sc == 0) & (sj == 0)
         stats["max_dlambda1_chosen_vs_fp64"] = float(np.max(np.abs(lc[both, 0] - lj[both, 0]))) if both.any() else 0.0
         stats["max_dlambda_chosen_vs_fp64"] = float(np.max(np.abs(lc[both] - lj[both]))) if both.any() else 0.0

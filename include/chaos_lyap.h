@@ -158,6 +158,9 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *       0 (default) = the block-per-point kernel above
  *   "lane_max_blocks": cap on the persistent grid of the 3-D kernels (0 = none)
  *   "l96_max_blocks": cap on the persistent grid of the Lorenz-96 kernels (0 = none)
+ *   "lane_gather": 3-D kernels: lanes at a renormalisation wait until this many
+ *       lanes of the wave wait, then renormalise together (0 or 1 = at once;
+ *       results are the same bits either way)
  */
 int lyap_ctx_set_option(lyap_ctx* ctx, const char* key, int64_t value);
 
@@ -212,6 +215,8 @@ typedef struct lyap_batch_tuning {
     int32_t waves_per_cu;  /* persistent grid = min(waves_per_cu, fit) * CUs; 0 = as many as fit */
     int32_t order;         /* queue order: 0 longest task first (3-D lanes, then L96 32/16/8),
                               1 as given, 2 Lorenz-96 classes first */
+    int32_t gather;        /* 3-D lanes at a renormalisation wait until this many lanes of the
+                              wave wait, then renormalise together (0, 1: at once) */
 } lyap_batch_tuning;
 
 typedef struct lyap_batch lyap_batch;

@@ -141,4 +141,4 @@ def test_batch_record_layout():
     # lyap_batch_class: the descriptor, then count and six pointers
     assert L.BatchClassRec.count.offset == ctypes.sizeof(L.SweepDesc)
     assert ctypes.sizeof(L.BatchClassRec) == ctypes.sizeof(L.SweepDesc) + 7 * 8
-    assert ctypes.sizeof(L.BatchTuning) == 16
+    assert ctypes.sizeof(L.BatchTuning) == 20

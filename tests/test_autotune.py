@@ -31,7 +31,7 @@ def test_sample_indices_seeded_sorted_unique():
 
 def test_sample_classes_keeps_rows():
     cls = cpu_classes()
-    s = A.sample_classes(cls, frac=0.1, min_points=10)
+    s = A.sample_classes(cls, frac=0.1, min_points=10, min_total=100)
     for c, sc in zip(cls, s):
         idx = A.sample_indices(c["count"], 0.1, 10, 5 + cls.index(c))
         assert sc["count"] == len(idx)
@@ -41,22 +41,25 @@ def test_sample_classes_keeps_rows():
 def test_tune_batch_picks_and_caches(tmp_path):
     cls = cpu_classes()
     cache = A.Cache(str(tmp_path / "tune.json"))
-    cost = {(0, False, 0, 0): 10.0, (0, True, 0, 0): 7.0, (1, False, 0, 0): 9.0,
-            (1, True, 0, 0): 8.0, (0, False, 0, 2): 12.0, (0, True, 0, 2): 7.5}
+    cost = {(0, False, 0, 0, 0): 10.0, (0, True, 0, 0, 0): 7.0, (1, False, 0, 0, 0): 9.0,
+            (1, True, 0, 0, 0): 8.0, (1, True, 0, 0, 8): 7.5, (1, True, 0, 0, 32): 7.2,
+            (1, True, 0, 2, 0): 12.0}
     calls = []
 
     def timer(cand, sample):
         calls.append(cand)
-        assert sum(c["count"] for c in sample) < sum(c["count"] for c in cls)
+        # 5000 points < the minimum sample: the whole batch is timed
+        assert sum(c["count"] for c in sample) == sum(c["count"] for c in cls)
         return cost[cand]
 
     r = A.tune_batch(None, cls, cache=cache, timer=timer, frac=0.1, min_points=10)
-    assert r.best == (0, True, 0, 0) and not r.cached
+    assert r.best == (0, True, 0, 0, 0) and not r.cached
     assert len(calls) == len(A.BATCH_CANDIDATES)
     r2 = A.tune_batch(None, cls, cache=A.Cache(str(tmp_path / "tune.json")), timer=timer)
-    assert r2.cached and tuple(r2.best) == (0, True, 0, 0)
+    assert r2.cached and tuple(r2.best) == (0, True, 0, 0, 0)
     assert len(calls) == len(A.BATCH_CANDIDATES)  # served from the file
-    assert A.batch_kwargs(r2.best) == dict(layout=0, occupancy3=True, waves_per_cu=0, order=0)
+    assert A.batch_kwargs(r2.best) == dict(layout=0, occupancy3=True, waves_per_cu=0, order=0,
+                                           gather=0)
 
 
 def test_tune_batch_skips_unbuilt_variants(tmp_path):
@@ -65,10 +68,10 @@ def test_tune_batch_skips_unbuilt_variants(tmp_path):
     def timer(cand, sample):
         if cand[0] == 1:
             raise L.LyapError(L.ERR_UNSUPPORTED, "not built")
-        return 5.0 - cand[3]
+        return 5.0 - cand[1]
 
     r = A.tune_batch(None, cls, cache=A.Cache(None), timer=timer, frac=0.1, min_points=10)
-    assert r.best == (0, False, 0, 2)
+    assert r.best == (0, True, 0, 0, 0)
     assert all(c[0] == 0 for c in r.timings)
 
 
@@ -98,6 +101,13 @@ def test_tune_l96_candidates_cover_built_layouts(tmp_path):
 
 def test_tune_lanes_grid_caps():
     desc = L.finalize(L.make_desc(W.rossler_dverk(n=64)))
-    r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=lambda c, idx: 1.0 + (c[0] == 0))
-    assert r.best == (512,)
-    assert set(r.timings) == {(0,), (512,), (256,)}
+    r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=lambda c, idx: 2.0 - 0.01 * c[1] + (c[0] > 0))
+    assert r.best == (0, 16)
+    assert set(r.timings) == {(0, 0), (0, 8), (0, 16), (512, 0)}
+
+
+def test_sample_keeps_a_minimum_total():
+    cls = cpu_classes(n=5000)
+    s = A.sample_classes(cls, frac=1 / 16, min_points=1, min_total=2500)
+    tot = sum(c["count"] for c in s)
+    assert 2400 <= tot <= 2600

@@ -61,7 +61,8 @@ def test_batch_matches_oracle(gpu_ctx, prec):
 
 TUNINGS = [dict(layout=0, occupancy3=False, order=0), dict(layout=0, occupancy3=True, order=0),
            dict(layout=1, occupancy3=False, order=2), dict(layout=1, occupancy3=True, order=1),
-           dict(layout=0, occupancy3=False, order=0, waves_per_cu=1)]
+           dict(layout=0, occupancy3=False, order=0, waves_per_cu=1),
+           dict(layout=1, occupancy3=True, order=0, gather=24)]
 
 
 def test_batch_matches_single_class_kernels(gpu_ctx):

@@ -68,6 +68,25 @@ def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
     assert_match(g, o, TOL_FP64, exact=True)
 
 
+@pytest.mark.parametrize("gather", [2, 16, 64])
+@pytest.mark.parametrize("integ,adaptive,prec", [(L.INT_DVERK, 1, L.PREC_MIXED),
+                                                 (L.INT_DOPRI5, 1, L.PREC_FP64),
+                                                 (L.INT_DOPRI5, 0, L.PREC_MIXED)])
+def test_var_lane_gathered_renorm_matches_oracle(gpu_ctx, gather, integ, adaptive, prec):
+    """Lanes at a renormalisation wait for `gather` lanes of the wave, on a grid
+    of 2 blocks so that lanes also refill: every point's bits are unchanged."""
+    spec = W.rossler_dverk(n=24, precision=prec, t_transient=5.0, t_average=20.0)
+    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9)
+    gpu_ctx.set_option("lane_gather", gather)
+    gpu_ctx.set_option("lane_max_blocks", 2)
+    try:
+        g, o = run_both(gpu_ctx, spec)
+    finally:
+        gpu_ctx.set_option("lane_gather", 0)
+        gpu_ctx.set_option("lane_max_blocks", 0)
+    assert_match(g, o, TOL_FP64, exact=True)
+
+
 @pytest.mark.parametrize("method", [L.METHOD_VAR_REF, L.METHOD_QR_REF, L.METHOD_FD_REF])
 @pytest.mark.parametrize("model", ["lorenz", "shimizu"])
 def test_reference_modes_match_oracle(gpu_ctx, model, method):
