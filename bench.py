@@ -18,6 +18,7 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
       step is one replay of the captured persistent batched launch, tuned first
       by the autotuner on a 1/16 sample.  Parameters are uploaded once, before
       the timed region.
+  hbm-copy      a plain 1 GiB device copy: the reference for the HBM byte counters.
 
 The plain FP64 sweep of the same grid is timed once for the speed-up.  Prints
 one JSON line.
@@ -303,6 +304,26 @@ def bench_batch(args, ctx, n_points, wl_name):
     return rec
 
 
+def bench_hbm_copy(args):
+    """A plain device copy of known size: the reference reading for the HBM
+    byte counters (FETCH_SIZE / WRITE_SIZE) and the copy bandwidth."""
+    import torch
+
+    n = 1 << 27  # 128 Mi doubles = 1 GiB
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "device copy bandwidth (read + write)", "value": 2 * n * 8 / dt / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "bytes_read_per_copy": n * 8, "bytes_written_per_copy": n * 8}
+
+
 def base_record(args, value, ms_per_step, desc):
     from chaos_oneapi_amd import lyap as L
 
@@ -339,6 +360,11 @@ def main():
               "single MI355X; multi-GPU sharding is described in DESIGN.md, not launched.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "hbm-copy":
+        args.steps = 5 if args.steps is None else args.steps
+        print(json.dumps(bench_hbm_copy(args)), flush=True)
+        return
 
     from chaos_oneapi_amd import lyap as L
     from chaos_oneapi_amd import workloads as W

@@ -65,6 +65,8 @@ const ModelRow kModels[] = {
      "x_i'=(x_{i+1}-x_{i-2})*x_{i-1}-d*x_i+F (periodic)"},
     {LYAP_MODEL_L96, 32, 2, LYAP_TYPE_FLOW, "lorenz96_32", "F d", "x0..x31",
      "x_i'=(x_{i+1}-x_{i-2})*x_{i-1}-d*x_i+F (periodic)"},
+    {LYAP_MODEL_HENON, 2, 2, LYAP_TYPE_MAP, "henon", "a b", "x y", "x_{n+1}=1-a*x_n^2+y_n; y_{n+1}=b*x_n"},
+    {LYAP_MODEL_LOGISTIC, 1, 1, LYAP_TYPE_MAP, "logistic", "r", "x", "x_{n+1}=r*x_n*(1-x_n)"},
 };
 constexpr int kNumModels = sizeof(kModels) / sizeof(kModels[0]);
 
@@ -97,9 +99,12 @@ int64_t spread_stride(int64_t count) {
     return s;
 }
 
+bool map_model(int32_t model) { return model == LYAP_MODEL_HENON || model == LYAP_MODEL_LOGISTIC; }
+
+// models whose kernels run one point per lane
 bool lane_model(int32_t model) {
     return model == LYAP_MODEL_SHIMIZU || model == LYAP_MODEL_LORENZ ||
-           model == LYAP_MODEL_ROSSLER;
+           model == LYAP_MODEL_ROSSLER || map_model(model);
 }
 
 }  // namespace
@@ -198,6 +203,9 @@ int lyap_desc_finalize(lyap_sweep_desc* d) {
                             kModels[i].name, kModels[i].n_params, d->n_params);
         }
     if (!known) return fail(LYAP_ERR_INVALID, "unknown model %d with dimension %d", d->model, d->dim);
+    // a map advances one iteration per unit of time (the reference's map loops
+    // count t += 1, cuda/lyapunov.dp.cpp:70-122,182-222)
+    if (map_model(d->model)) d->h = 1.0;
     if (d->n_exps < 1 || d->n_exps > d->dim)
         return fail(LYAP_ERR_INVALID, "lyapunov_exponent_num %d outside [1, %d]", d->n_exps, d->dim);
     if (!(d->h > 0.0)) return fail(LYAP_ERR_INVALID, "integration_step must be > 0");
@@ -213,6 +221,11 @@ int lyap_desc_finalize(lyap_sweep_desc* d) {
                     d->init_param_offset);
     if (d->max_steps <= 0) d->max_steps = 20000000LL;
     if (!(d->escape_radius > 0.0)) d->escape_radius = 1e6;
+    if (map_model(d->model)) {
+        if (d->adaptive) return fail(LYAP_ERR_INVALID, "a map has no adaptive step");
+        if (d->method == LYAP_METHOD_QR_REF)
+            return fail(LYAP_ERR_UNSUPPORTED, "the reference QR mode has no map branch");
+    }
     switch (d->method) {
         case LYAP_METHOD_VAR: {
             if (!(d->t_renorm > 0.0))
@@ -243,7 +256,8 @@ int lyap_desc_finalize(lyap_sweep_desc* d) {
         case LYAP_METHOD_FD_REF: {
             if (!lane_model(d->model))
                 return fail(LYAP_ERR_UNSUPPORTED,
-                            "the reference modes run the 3-D models only (model %d)", d->model);
+                            "the reference modes run the 3-D flows and the maps only (model %d)",
+                            d->model);
             if (d->integrator != LYAP_INT_DVERK || d->adaptive || d->precision != LYAP_PREC_FP64)
                 return fail(LYAP_ERR_INVALID,
                             "the reference modes are fixed-step FP64 DVERK (cuda/tools/dverk.dp.cpp)");

@@ -307,6 +307,7 @@ LYAP_D T coef_sum_e(const RowE<TB, T, LD>& row, const T (&k)[LD][N], int i) {
 // error estimate vanishes (stable fixed points).
 struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27; (alpha, lambda)
     static constexpr int ID = LYAP_MODEL_SHIMIZU, N = 3, P = 2;
+    static constexpr bool kMap = false;
     template <class T>
     struct J {
         T x, oz;  // x, 1 - z
@@ -337,6 +338,7 @@ struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27; (alpha, l
 
 struct Lorenz {  // (sigma, rho, beta)
     static constexpr int ID = LYAP_MODEL_LORENZ, N = 3, P = 3;
+    static constexpr bool kMap = false;
     template <class T>
     struct J {
         T x, y, rz;  // x, y, rho - z
@@ -367,6 +369,7 @@ struct Lorenz {  // (sigma, rho, beta)
 
 struct Rossler {  // (a, c, b)
     static constexpr int ID = LYAP_MODEL_ROSSLER, N = 3, P = 3;
+    static constexpr bool kMap = false;
     template <class T>
     struct J {
         T z, xc;  // z, x - c
@@ -392,6 +395,50 @@ struct Rossler {  // (a, c, b)
         const double r2 = fabs(x[2]) + fabs(x[0] - p[1]);
         const double m = r0 > r1 ? r0 : r1;
         return m > r2 ? m : r2;
+    }
+};
+
+// Map-type models (ModelType::map, model_factory.h:22-26): rhs() is the next
+// state f(x), jvp() the product J(x) v (StepMAP / StepMAPVAR,
+// cuda/tools/mapstep.dp.cpp:5-18).  Jacobian entries that combine state and
+// parameters are formed in FP64 and rounded once for the mixed form.
+struct Henon {  // (a, b): x' = 1 - a x^2 + y, y' = b x
+    static constexpr int ID = LYAP_MODEL_HENON, N = 2, P = 2;
+    static constexpr bool kMap = true;
+    template <class T>
+    struct J {
+        T j00;  // -2 a x
+    };
+    LYAP_D static void rhs(const double* x, const double* p, double* o) {
+        o[0] = fma(-p[0], x[0] * x[0], 1.0 + x[1]);
+        o[1] = p[1] * x[0];
+    }
+    template <class T>
+    LYAP_D static J<T> jac(const double* x, const double* p) {
+        return J<T>{(T)((-2.0 * p[0]) * x[0])};
+    }
+    template <class T>
+    LYAP_D static void jvp(const J<T>& j, const T* p, const T* v, T* o) {
+        o[0] = fma(j.j00, v[0], v[1]);
+        o[1] = p[1] * v[0];
+    }
+};
+
+struct Logistic {  // (r): x' = r x (1 - x)
+    static constexpr int ID = LYAP_MODEL_LOGISTIC, N = 1, P = 1;
+    static constexpr bool kMap = true;
+    template <class T>
+    struct J {
+        T j;  // r (1 - 2x)
+    };
+    LYAP_D static void rhs(const double* x, const double* p, double* o) { o[0] = (p[0] * x[0]) * (1.0 - x[0]); }
+    template <class T>
+    LYAP_D static J<T> jac(const double* x, const double* p) {
+        return J<T>{(T)(p[0] * fma(-2.0, x[0], 1.0))};
+    }
+    template <class T>
+    LYAP_D static void jvp(const J<T>& j, const T*, const T* v, T* o) {
+        o[0] = j.j * v[0];
     }
 };
 

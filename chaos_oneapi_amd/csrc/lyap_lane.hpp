@@ -362,4 +362,104 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     }
 }
 
+// Persistent per-lane loop of the sweep's method for maps: per iteration
+// V <- J(x) V (binary32 products in the mixed form), x <- f(x); MGS and the
+// Kahan-summed logs every steps_per_renorm iterations; the escape test after
+// every iteration (a diverging map overflows within a few iterations).
+template <class M, bool MIXED, class GetDesc>
+LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io) {
+    constexpr int N = M::N, P = M::P, K = M::N;
+    using TX = typename std::conditional<MIXED, float, double>::type;
+    const int64_t n_tr = d.n_renorm_transient;
+    const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
+    const int64_t spr = d.steps_per_renorm;
+    const double esc = d.escape_radius;
+    const double t_avg = (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
+    const int n_exps = d.n_exps;
+
+    double x[N], V[K][N], sum[K], comp[K], p[P];
+    float pf[P];
+    int64_t m = 0, step_in = 0, item = -1;
+    uint32_t nacc = 0;
+    bool active = false, exhausted = false;
+
+    for (;;) {
+        const int64_t got = grab_items(!active && !exhausted, io.counter);
+        if (!active && !exhausted) {
+            if (got < io.count) {
+                const auto dp = getd();
+                item = item_position(io, got);
+                const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
+                point_params<P>(*dp, idx, io.params, p);
+#pragma unroll
+                for (int j = 0; j < P; ++j) pf[j] = (float)p[j];
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                    x[i] = io.inits ? io.inits[idx * N + i] : init_component(*dp, idx, i, p);
+#pragma unroll
+                for (int r = 0; r < K; ++r) {
+                    sum[r] = comp[r] = 0.0;
+#pragma unroll
+                    for (int i = 0; i < N; ++i) V[r][i] = (r == i) ? 1.0 : 0.0;
+                }
+                m = step_in = 0;
+                nacc = 0;
+                active = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (!__any(active)) break;
+        if (!active) continue;
+
+        uint32_t st = LYAP_PT_OK;
+        bool finished = false;
+        {
+            const typename M::template J<TX> jx = M::template jac<TX>(x, p);
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                TX v[N], o[N];
+#pragma unroll
+                for (int i = 0; i < N; ++i) v[i] = (TX)V[r][i];
+                if constexpr (MIXED) M::template jvp<float>(jx, pf, v, o);
+                else M::template jvp<double>(jx, p, v, o);
+#pragma unroll
+                for (int i = 0; i < N; ++i) V[r][i] = (double)o[i];
+            }
+            double xn[N];
+            M::rhs(x, p, xn);
+#pragma unroll
+            for (int i = 0; i < N; ++i) x[i] = xn[i];
+            ++nacc;
+        }
+        if (escaped<N>(x, esc)) {
+            st = LYAP_PT_ESCAPED;
+            finished = true;
+        } else if (++step_in == spr) {
+            step_in = 0;
+            double ln[K];
+            mgs_rows<K, N>(V, ln);
+            ++m;
+            if (m > n_tr) {
+#pragma unroll
+                for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
+            }
+            if (m == m_total) finished = true;
+        }
+        if (finished) {
+            for (int r = 0; r < n_exps; ++r) {
+                double s = 0.0;
+#pragma unroll
+                for (int q = 0; q < K; ++q)
+                    if (q == r) s = sum[q];
+                io.lambda[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
+            }
+            if (io.status) io.status[item] = st;
+            if (io.n_acc) io.n_acc[item] = nacc;
+            if (io.n_rej) io.n_rej[item] = 0;
+            active = false;
+        }
+    }
+}
+
 }  // namespace lyap

@@ -340,6 +340,159 @@ __global__ __launch_bounds__(256) void k_fdref_lane(const lyap_sweep_desc d, con
     }
 }
 
+// ---------------------------------------------------------------------------
+// maps
+// ---------------------------------------------------------------------------
+template <class M, bool MIXED>
+__global__ __launch_bounds__(256) void k_map_lane(const lyap_sweep_desc d, const SweepIO io) {
+    map_drain<M, MIXED>(d, [] { return kernarg_desc(); }, io);
+}
+
+template <class M>
+LYAP_D void map_step(double* x, const double* p) {
+    double xn[M::N];
+    M::rhs(x, p, xn);
+#pragma unroll
+    for (int i = 0; i < M::N; ++i) x[i] = xn[i];
+}
+
+// StepMAPVAR for the k row vectors, J at the current base point
+template <class M, int K>
+LYAP_D void map_tangent(const double* x, const double* p, double (&V)[K][M::N]) {
+    const typename M::template J<double> jx = M::template jac<double>(x, p);
+#pragma unroll
+    for (int r = 0; r < K; ++r) {
+        double o[M::N];
+        M::template jvp<double>(jx, p, V[r], o);
+#pragma unroll
+        for (int i = 0; i < M::N; ++i) V[r][i] = o[i];
+    }
+}
+
+// reference VAR mode, map branch: cuda/lyapunov.dp.cpp:182-222,224-225
+template <class M>
+__global__ __launch_bounds__(256) void k_mapvarref_lane(const lyap_sweep_desc d, const SweepIO io) {
+    constexpr int N = M::N, P = M::P, K = M::N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < io.count;
+         item += stride) {
+        const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
+        double p[P], x[N], V[K][N], L[K];
+        point_params<P>(d, idx, io.params, p);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            x[i] = io.inits ? io.inits[idx * N + i] : init_component(d, idx, i, p);
+        for (int64_t s = 0; s < d.ref_steps_transient; ++s) map_step<M>(x, p);
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            L[r] = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) V[r][i] = 0.0;
+            V[r][r] = V[r][r] + d.eps;
+        }
+        for (int64_t s = 0; s < d.ref_steps_slave; ++s) {
+            map_tangent<M, K>(x, p, V);
+            map_step<M>(x, p);
+            cgs_ref<K, N>(V);
+            normalize_ref<K, N>(V, d.eps);
+        }
+        for (int64_t s = 0; s < d.ref_steps_average; ++s) {
+            map_tangent<M, K>(x, p, V);
+            map_step<M>(x, p);
+            cgs_ref<K, N>(V);
+#pragma unroll
+            for (int r = 0; r < K; ++r) L[r] = L[r] + det_log(sqrt(dot_ref<N>(V[r], V[r])) / d.eps);
+            normalize_ref<K, N>(V, d.eps);
+        }
+        bool ok = true;
+        for (int r = 0; r < d.n_exps; ++r) {
+            double lr = 0.0;
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (q == r) lr = L[q];
+            const double lam = lr / d.t_average;
+            ok = ok && isfinite(lam);
+            io.lambda[item * d.n_exps + r] = lam;
+        }
+        if (io.status) io.status[item] = ok ? LYAP_PT_OK : LYAP_PT_NONFINITE;
+        if (io.n_acc)
+            io.n_acc[item] =
+                (uint32_t)(d.ref_steps_transient + d.ref_steps_slave + d.ref_steps_average);
+        if (io.n_rej) io.n_rej[item] = 0;
+    }
+}
+
+// reference FD (two-trajectory) mode, map branch: cuda/lyapunov.dp.cpp:70-122,127-128
+template <class M>
+__global__ __launch_bounds__(256) void k_mapfdref_lane(const lyap_sweep_desc d, const SweepIO io) {
+    constexpr int N = M::N, P = M::P, K = M::N;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t item = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; item < io.count;
+         item += stride) {
+        const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
+        double p[P], x[N], S[K][N], L[K];
+        point_params<P>(d, idx, io.params, p);
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+            x[i] = io.inits ? io.inits[idx * N + i] : init_component(d, idx, i, p);
+        for (int64_t s = 0; s < d.ref_steps_transient; ++s) map_step<M>(x, p);
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            L[r] = 0.0;
+#pragma unroll
+            for (int i = 0; i < N; ++i) S[r][i] = x[i];
+            S[r][r] = S[r][r] + d.eps;
+        }
+        for (int64_t s = 0; s < d.ref_steps_slave; ++s) {
+            map_step<M>(x, p);
+#pragma unroll
+            for (int r = 0; r < K; ++r) {
+                map_step<M>(S[r], p);
+#pragma unroll
+                for (int i = 0; i < N; ++i) S[r][i] = S[r][i] - x[i];
+            }
+            cgs_ref<K, N>(S);
+            normalize_ref<K, N>(S, d.eps);
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+#pragma unroll
+                for (int i = 0; i < N; ++i) S[r][i] = S[r][i] + x[i];
+        }
+        for (int64_t s = 0; s < d.ref_steps_average; ++s) {
+            map_step<M>(x, p);
+#pragma unroll
+            for (int r = 0; r < K; ++r) map_step<M>(S[r], p);
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+#pragma unroll
+                for (int i = 0; i < N; ++i) S[r][i] = S[r][i] - x[i];
+            cgs_ref<K, N>(S);
+#pragma unroll
+            for (int r = 0; r < K; ++r) L[r] = L[r] + det_log(sqrt(dot_ref<N>(S[r], S[r])) / d.eps);
+            normalize_ref<K, N>(S, d.eps);
+#pragma unroll
+            for (int r = 0; r < K; ++r)
+#pragma unroll
+                for (int i = 0; i < N; ++i) S[r][i] = S[r][i] + x[i];
+        }
+        bool ok = true;
+        for (int r = 0; r < d.n_exps; ++r) {
+            double lr = 0.0;
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (q == r) lr = L[q];
+            const double lam = lr / d.t_average;
+            ok = ok && isfinite(lam);
+            io.lambda[item * d.n_exps + r] = lam;
+        }
+        if (io.status) io.status[item] = ok ? LYAP_PT_OK : LYAP_PT_NONFINITE;
+        if (io.n_acc)
+            io.n_acc[item] =
+                (uint32_t)(d.ref_steps_transient + d.ref_steps_slave + d.ref_steps_average);
+        if (io.n_rej) io.n_rej[item] = 0;
+    }
+}
+
 __global__ void k_reset_counter(unsigned long long* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
 }
@@ -404,9 +557,37 @@ static hipError_t dispatch_model(const lyap_sweep_desc& d, const SweepIO& io, hi
     return hipGetLastError();
 }
 
+template <class M>
+static hipError_t dispatch_map(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
+                               int max_blocks) {
+    if (d.method == LYAP_METHOD_VAR) {
+        if (d.adaptive) return hipErrorInvalidValue;
+        if (d.precision == LYAP_PREC_MIXED)
+            return launch_persistent(k_map_lane<M, true>, d, io, s, num_cus, max_blocks);
+        return launch_persistent(k_map_lane<M, false>, d, io, s, num_cus, max_blocks);
+    }
+    const int64_t grid = (io.count + 255) / 256 > 4096 ? 4096 : (io.count + 255) / 256;
+    const dim3 g((unsigned)(grid < 1 ? 1 : grid)), b(256);
+    switch (d.method) {
+        case LYAP_METHOD_VAR_REF:
+            hipLaunchKernelGGL(k_mapvarref_lane<M>, g, b, 0, s, d, io);
+            break;
+        case LYAP_METHOD_FD_REF:
+            hipLaunchKernelGGL(k_mapfdref_lane<M>, g, b, 0, s, d, io);
+            break;
+        default:
+            return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_lane_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
                              int num_cus, int max_blocks) {
     switch (d.model) {
+        case LYAP_MODEL_HENON:
+            return dispatch_map<Henon>(d, io, s, num_cus, max_blocks);
+        case LYAP_MODEL_LOGISTIC:
+            return dispatch_map<Logistic>(d, io, s, num_cus, max_blocks);
         case LYAP_MODEL_SHIMIZU:
             return dispatch_model<Shimizu>(d, io, s, num_cus, max_blocks);
         case LYAP_MODEL_LORENZ:

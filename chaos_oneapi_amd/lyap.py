@@ -18,6 +18,7 @@ MAX_PARAMS = 8
 
 SUCCESS, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOT_FOUND = 0, -1, -2, -3, -4
 MODEL_SHIMIZU, MODEL_LORENZ, MODEL_ROSSLER, MODEL_L96 = 0, 1, 2, 3
+MODEL_HENON, MODEL_LOGISTIC = 4, 5
 TYPE_FLOW, TYPE_MAP = 0, 1
 METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF = 0, 1, 2, 3
 INT_DVERK, INT_DOPRI5 = 0, 1

@@ -1,8 +1,9 @@
 """Surrogate-guided refinement of a sweep (the learned part of the sweep).
 
 Pipeline, all on the device through liblyap.so:
-  1. a coarse training map (train_n x train_n over the same parameter ranges)
-     computed by the method in the chosen precision;
+  1. a coarse training map (train_n x train_n over the same parameter ranges,
+     the stride-s lattice of the fine grid when it has one) computed by the
+     method in the chosen precision;
   2. the surrogate network (lyap_mlp_*) trained on it with Adam;
   3. inference over the fine grid;
   4. the method is rerun where the prediction is uncertain (kappa * std of
@@ -13,9 +14,10 @@ Pipeline, all on the device through liblyap.so:
      surrogate's answers on the calibration subsample agree with the method's
      class on calib_target of them (the predicted std alone is not calibrated);
   5. the plain FP64 method judges the verification subsample.
-The calibration and verification runs (chosen form, and the judge) do not
-depend on the surrogate, so they run on a side stream, with a context of their
-own, concurrently with steps 1-3; the rerun of step 4 skips those points.
+The training map and the calibration and verification runs do not depend on
+the surrogate: they are one launch, and the FP64 judge runs on a side stream
+(with a context of its own) concurrently with steps 1-3; the rerun of step 4
+skips every point already computed.
 The returned map takes the method's values where it ran and the surrogate's
 elsewhere.  Gates reported on the verification subsample: max |dlambda1|
 between the chosen form and FP64 on computed points, and agreement of the
@@ -113,10 +115,13 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     end2 = fd.start2 + fd.delta2 * (fd.n2 - 1)
     t_all = t = time.perf_counter()
 
-    # the verification subsample does not depend on the surrogate: its chosen-form
-    # values and the FP64 judge run on a side stream (own context, so its own
-    # work counter) while the main stream computes the training map, trains and
-    # infers
+    # 1. the training map, and the calibration and verification subsamples: none
+    #    depends on the surrogate.  When the fine grid holds a train_n x train_n
+    #    lattice (stride s = (n-1) // (train_n-1) >= 1 in both directions), the
+    #    training map is that lattice, so the three sets are one chosen-form
+    #    launch over the fine grid (one tail) and their values enter the final
+    #    map; the FP64 judge runs concurrently on a side stream with a context
+    #    of its own (its own work counter)
     gv = torch.Generator(device="cpu").manual_seed(seed + 1)
     n_ver = max(1, int(round(verify_frac * n)))
     n_cal = max(1, int(round(calib_frac * n)))
@@ -124,29 +129,38 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     verify = perm[:n_ver].sort().values.to(dev)
     calib = perm[n_ver:n_ver + n_cal].sort().values.to(dev)
     main = torch.cuda.current_stream(dev)
-    side, side_j = torch.cuda.Stream(dev), torch.cuda.Stream(dev)
-    side.wait_stream(main)
+    side_j = torch.cuda.Stream(dev)
     side_j.wait_stream(main)
-    side_ctx, judge_ctx = L.Context(ctx.device), L.Context(ctx.device)
-    both = torch.cat([calib, verify]).contiguous()
-    with torch.cuda.stream(side):
-        # one launch for both subsamples: one tail instead of two
-        lam_cv, st_cv, _, _ = side_ctx.run(fd, idx_list=both, stream=side)
-        lam_c, st_c = lam_cv[:n_cal], st_cv[:n_cal]
-        lam_v, st_v = lam_cv[n_cal:], st_cv[n_cal:]
+    judge_ctx = L.Context(ctx.device)
     if judge:
         with torch.cuda.stream(side_j):
             pdesc = L.SweepDesc.from_buffer_copy(fd)
             pdesc.precision = L.PREC_FP64
             lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
-
-    # 1. training map
-    tspec = dict(fine)
-    tspec.update(W.grid_fields(fd.sweep_idx1, fd.start1, end1, train_n, fd.sweep_idx2, fd.start2,
-                               end2, train_n))
-    td = L.finalize(L.make_desc(tspec))
-    lam_t, st_t, _, _ = ctx.run(td)
-    p_t = grid_params(td, torch.arange(train_n * train_n, device=dev))
+    s1 = (fd.n1 - 1) // (train_n - 1) if train_n > 1 else 0
+    s2 = (fd.n2 - 1) // (train_n - 1) if train_n > 1 else 0
+    n_t = train_n * train_n
+    lattice = s1 >= 1 and s2 >= 1
+    if lattice:
+        ar = torch.arange(train_n, device=dev, dtype=torch.int64)
+        train_idx = ((ar * s1)[:, None] * fd.n2 + (ar * s2)[None, :]).reshape(-1)
+        first = torch.cat([train_idx, calib, verify]).contiguous()
+        lam_f, st_f, _, _ = ctx.run(fd, idx_list=first)
+        lam_t, st_t = lam_f[:n_t], st_f[:n_t]
+        p_t = grid_params(fd, train_idx)
+    else:
+        tspec = dict(fine)
+        tspec.update(W.grid_fields(fd.sweep_idx1, fd.start1, end1, train_n, fd.sweep_idx2, fd.start2,
+                                   end2, train_n))
+        td = L.finalize(L.make_desc(tspec))
+        lam_t, st_t, _, _ = ctx.run(td)
+        p_t = grid_params(td, torch.arange(n_t, device=dev))
+        first = torch.cat([calib, verify]).contiguous()
+        lam_f, st_f, _, _ = ctx.run(fd, idx_list=first)
+        lam_f = torch.cat([lam_t, lam_f])
+        st_f = torch.cat([st_t, st_f])
+    lam_c, st_c = lam_f[n_t:n_t + n_cal], st_f[n_t:n_t + n_cal]
+    lam_v, st_v = lam_f[n_t + n_cal:], st_f[n_t + n_cal:]
     t = mark_stream("train_map", t, main)
 
     # 2. training (one shuffle per epoch; batches are contiguous slices)
@@ -159,7 +173,6 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
                 ranges=(min(fd.start1, end1), max(fd.start1, end1), min(fd.start2, end2),
                         max(fd.start2, end2)), y_mean=ym, y_std=ys)
     gen = torch.Generator(device="cpu").manual_seed(seed)
-    n_t = train_n * train_n
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
     losses = []
     for ep in range(epochs):
@@ -181,9 +194,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     # 4. calibration: the std multiplier at which the surrogate's answers on the
     #    calibration subsample agree with the method (the verification
     #    subsample stays out of it), then the rerun set: every point the
-    #    surrogate does not answer; calibration and verification points are
-    #    computed on the side stream already
-    side.synchronize()
+    #    surrogate does not answer and that step 1 did not compute
     kappa, cal_agree, cal_answered = calibrate_kappa(
         pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
         bad_lo, bad_hi, calib_target)
@@ -198,12 +209,14 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     vmask[verify] = True
     cmask = torch.zeros(n, dtype=torch.bool, device=dev)
     cmask[calib] = True
-    rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask).flatten().to(torch.int64).contiguous()
+    tmask = torch.zeros(n, dtype=torch.bool, device=dev)
+    if lattice:
+        tmask[train_idx] = True
+    rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask & ~tmask).flatten().to(torch.int64).contiguous()
     lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
     t = mark_stream("rerun", t, main)
     rerun_kernel_ms = ctx.kernel_ms()
     torch.cuda.synchronize(dev)
-    side_ctx.close()
     judge_ctx.close()
 
     # 5. assemble
@@ -216,13 +229,17 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     status[verify] = st_v
     lam[calib] = lam_c
     status[calib] = st_c
-    rerun_mask = flagged | vmask | cmask
+    if lattice:
+        lam[train_idx] = lam_t
+        status[train_idx] = st_t
+    rerun_mask = flagged | vmask | cmask | tmask
     tm["total_chosen"] = time.perf_counter() - t_all
 
-    n_rerun = int(rerun_idx.numel()) + n_ver + n_cal
+    n_rerun = int(rerun_idx.numel()) + n_ver + n_cal + (n_t if lattice else 0)
     stats = dict(points=n, train_points=n_t, rerun=n_rerun,
                  flagged=int(flagged.sum().item()), verify=n_ver, calib=n_cal,
                  kappa=kappa, calib_agreement=cal_agree, calib_answered=cal_answered,
+                 train_on_lattice=bool(lattice),
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),
                  rerun_acc_steps=float(na_r.double().sum().item()),

@@ -100,3 +100,31 @@ def batch_mix(n_points: int = 1 << 20, seed: int = 20240604, precision: int = L.
         spec.update(integrator=L.INT_DOPRI5, adaptive=1, rtol=tol, atol=tol)
         out.append(dict(name=name, spec=spec, count=n, params=np.ascontiguousarray(p)))
     return out
+
+
+def henon(n: int = 256, precision: int = L.PREC_FP64, t_transient: float = 1000.0,
+          t_average: float = 100000.0, t_renorm: float = 10.0) -> dict:
+    """Henon map x' = 1 - a x^2 + y, y' = b x over a in [1.0, 1.42] x b in
+    [0.05, 0.35]; times count iterations (h = 1), renormalise every t_renorm
+    iterations; initial state (0.1, 0.1) + U(+-0.05).  Orbits leaving
+    |x_i| < 1e3 are marked escaped."""
+    d = dict(model=L.MODEL_HENON, dim=2, n_exps=2, n_params=2, method=L.METHOD_VAR,
+             integrator=L.INT_DVERK, adaptive=0, precision=precision, h=1.0,
+             t_transient=t_transient, t_slave=0.0, t_average=t_average, t_renorm=t_renorm,
+             escape_radius=1e3, base_params=[0.0, 0.0], init_state=[0.1, 0.1],
+             init_kind=L.INIT_UNIFORM, init_param_offset=-1, init_scale=0.05, seed=20240605)
+    d.update(grid_fields(0, 1.0, 1.42, n, 1, 0.05, 0.35, n))
+    return d
+
+
+def logistic(n: int = 4096, precision: int = L.PREC_FP64, t_transient: float = 1000.0,
+             t_average: float = 100000.0, t_renorm: float = 10.0) -> dict:
+    """Logistic map x' = r x (1 - x) over r in [2.8, 4.0] (n points, one
+    parameter); times count iterations; initial state 0.3 + U(+-0.1)."""
+    d = dict(model=L.MODEL_LOGISTIC, dim=1, n_exps=1, n_params=1, method=L.METHOD_VAR,
+             integrator=L.INT_DVERK, adaptive=0, precision=precision, h=1.0,
+             t_transient=t_transient, t_slave=0.0, t_average=t_average, t_renorm=t_renorm,
+             escape_radius=1e3, base_params=[0.0], init_state=[0.3], init_kind=L.INIT_UNIFORM,
+             init_param_offset=-1, init_scale=0.1, seed=20240606)
+    d.update(grid_fields(0, 2.8, 4.0, n, 0, 2.8, 2.8, 1))
+    return d

@@ -47,7 +47,9 @@ enum {
     LYAP_MODEL_SHIMIZU = 0, /* reference cuda/models/flows/shimizu.dp.cpp; params (alpha, lambda) */
     LYAP_MODEL_LORENZ = 1,  /* params (sigma, rho, beta) */
     LYAP_MODEL_ROSSLER = 2, /* params (a, c, b) */
-    LYAP_MODEL_L96 = 3      /* Lorenz-96, N in {8,16,32}; params (F, d) */
+    LYAP_MODEL_L96 = 3,     /* Lorenz-96, N in {8,16,32}; params (F, d) */
+    LYAP_MODEL_HENON = 4,   /* Henon map, N = 2; params (a, b) */
+    LYAP_MODEL_LOGISTIC = 5 /* logistic map, N = 1; params (r) */
 };
 /* ModelType of model_factory.h:22-26 */
 enum { LYAP_TYPE_FLOW = 0, LYAP_TYPE_MAP = 1 };

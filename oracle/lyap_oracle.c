@@ -160,10 +160,19 @@ void orc_rhs(int model, int n, const double* x, const double* p, double* out) {
             out[i] = fma(x[ip1] - x[im2], x[im1], fma(-p[1], x[i], p[0]));
         }
         break;
+    case ORC_MODEL_HENON: /* (a, b): x' = 1 - a x^2 + y, y' = b x */
+        out[0] = fma(-p[0], x[0] * x[0], 1.0 + x[1]);
+        out[1] = p[1] * x[0];
+        break;
+    case ORC_MODEL_LOGISTIC: /* (r): x' = r x (1 - x) */
+        out[0] = (p[0] * x[0]) * (1.0 - x[0]);
+        break;
     default:
         for (int i = 0; i < n; ++i) out[i] = NAN;
     }
 }
+
+int orc_is_map(int model) { return model == ORC_MODEL_HENON || model == ORC_MODEL_LOGISTIC; }
 
 void orc_jvp(int model, int n, const double* x, const double* p, const double* v, double* out) {
     switch (model) {
@@ -187,6 +196,13 @@ void orc_jvp(int model, int n, const double* x, const double* p, const double* v
             int ip1 = (i + 1) % n, im1 = (i + n - 1) % n, im2 = (i + n - 2) % n;
             out[i] = fma(v[ip1] - v[im2], x[im1], fma(x[ip1] - x[im2], v[im1], -p[1] * v[i]));
         }
+        break;
+    case ORC_MODEL_HENON: /* J = [[-2 a x, 1], [b, 0]] */
+        out[0] = fma((-2.0 * p[0]) * x[0], v[0], v[1]);
+        out[1] = p[1] * v[0];
+        break;
+    case ORC_MODEL_LOGISTIC: /* J = r (1 - 2x) */
+        out[0] = (p[0] * fma(-2.0, x[0], 1.0)) * v[0];
         break;
     default:
         for (int i = 0; i < n; ++i) out[i] = NAN;
@@ -267,6 +283,17 @@ void orc_jvp_f32(int model, int n, const double* xd, const double* pd, const flo
             out[i] = fmaf(v[ip1] - v[im2], x[im1], fmaf(dx, v[im1], -p[1] * v[i]));
         }
         break;
+    case ORC_MODEL_HENON: {
+        const float j00 = (float)((-2.0 * pd[0]) * xd[0]);
+        out[0] = fmaf(j00, v[0], v[1]);
+        out[1] = p[1] * v[0];
+        break;
+    }
+    case ORC_MODEL_LOGISTIC: {
+        const float j = (float)(pd[0] * fma(-2.0, xd[0], 1.0));
+        out[0] = j * v[0];
+        break;
+    }
     default:
         for (int i = 0; i < n; ++i) out[i] = NAN;
     }
@@ -905,6 +932,140 @@ static void point_fd_ref(const orc_desc* d, const double* p, double* x, double* 
     *steps = (uint64_t)(n_tr + n_sl + n_av);
 }
 
+/* ------------------------------------------------------------------------ */
+/* maps (ModelType::map): one iteration is one "step", time counts iterations */
+/* ------------------------------------------------------------------------ */
+static void map_step(int model, int n, double* x, const double* p) {
+    double xn[ORC_MAX_DIM];
+    orc_rhs(model, n, x, p, xn);
+    memcpy(x, xn, sizeof(double) * n);
+}
+
+/* V <- J(x) V for k row vectors (StepMAPVAR, cuda/tools/mapstep.dp.cpp:12-18);
+   the mixed form takes the product in binary32 */
+static void map_tangent(int model, int n, int k, const double* x, const double* p, int mixed,
+                        double* V) {
+    double out[ORC_MAX_DIM];
+    float vf[ORC_MAX_DIM], of[ORC_MAX_DIM];
+    for (int r = 0; r < k; ++r) {
+        if (mixed) {
+            for (int i = 0; i < n; ++i) vf[i] = (float)V[r * n + i];
+            orc_jvp_f32(model, n, x, p, vf, of);
+            for (int i = 0; i < n; ++i) V[r * n + i] = (double)of[i];
+        } else {
+            orc_jvp(model, n, x, p, V + r * n, out);
+            memcpy(V + r * n, out, sizeof(double) * n);
+        }
+    }
+}
+
+/* the sweep's method for maps: R = t_renorm iterations per renormalisation
+   (MGS, Kahan-summed logs), the transient from the start */
+static void point_map_var(const orc_desc* d, const double* p, double* x, double* lam,
+                          uint32_t* status, uint32_t* n_acc_out) {
+    const int n = d->dim, k = d->n_exps;
+    const int mixed = d->precision == ORC_PREC_MIXED;
+    const int64_t R = llround(d->t_renorm / d->h);
+    const int64_t n_tr = llround((d->t_transient + d->t_slave) / d->t_renorm);
+    const int64_t n_av = llround(d->t_average / d->t_renorm);
+    double V[ORC_MAX_DIM * ORC_MAX_DIM], sum[ORC_MAX_DIM], comp[ORC_MAX_DIM], ln[ORC_MAX_DIM];
+    for (int r = 0; r < k; ++r) {
+        sum[r] = comp[r] = 0.0;
+        for (int i = 0; i < n; ++i) V[r * n + i] = (r == i) ? 1.0 : 0.0;
+    }
+    uint32_t st = ORC_OK;
+    uint64_t n_acc = 0;
+    /* a diverging map orbit overflows within a few iterations (x ~ x^2), so
+       the escape test runs after every iteration */
+    for (int64_t m = 0; m < n_tr + n_av && st == ORC_OK; ++m) {
+        for (int64_t s = 0; s < R; ++s) {
+            map_tangent(d->model, n, k, x, p, mixed, V);
+            map_step(d->model, n, x, p);
+            ++n_acc;
+            if (escaped(n, x, d->escape_radius)) { st = ORC_ESCAPED; break; }
+        }
+        if (st != ORC_OK) break;
+        orc_mgs(n, k, 0, V, ln);
+        if (m + 1 > n_tr)
+            for (int r = 0; r < k; ++r) kahan_add(&sum[r], &comp[r], ln[r]);
+    }
+    const double t_avg = (double)n_av * (double)R * d->h;
+    for (int r = 0; r < k; ++r) lam[r] = (st == ORC_OK) ? sum[r] / t_avg : NAN;
+    *status = st;
+    *n_acc_out = (uint32_t)n_acc;
+}
+
+/* reference VAR mode for maps: cuda/lyapunov.dp.cpp:182-222,224-225 */
+static void point_map_var_ref(const orc_desc* d, const double* p, double* x, double* lam,
+                              uint64_t* steps) {
+    const int n = d->dim, k = d->n_exps;
+    const double eps = d->eps;
+    const int64_t n_tr = orc_loop_count(0.0, d->t_transient, 1.0);
+    const int64_t n_sl = orc_loop_count(0.0, d->t_slave, 1.0);
+    const int64_t n_av = orc_loop_count(0.0, d->t_average, 1.0);
+    double V[ORC_MAX_DIM * ORC_MAX_DIM], L[ORC_MAX_DIM];
+    for (int r = 0; r < k; ++r) L[r] = 0.0;
+    for (int64_t s = 0; s < n_tr; ++s) map_step(d->model, n, x, p);
+    for (int q = 0; q < k * n; ++q) V[q] = 0.0;
+    for (int r = 0; r < k; ++r) V[r * n + r] = V[r * n + r] + eps;
+    for (int64_t s = 0; s < n_sl; ++s) {
+        map_tangent(d->model, n, k, x, p, 0, V);
+        map_step(d->model, n, x, p);
+        orc_cgs_ref(n, k, V);
+        orc_normalize_ref(n, k, V, eps);
+    }
+    for (int64_t s = 0; s < n_av; ++s) {
+        map_tangent(d->model, n, k, x, p, 0, V);
+        map_step(d->model, n, x, p);
+        orc_cgs_ref(n, k, V);
+        for (int r = 0; r < k; ++r) L[r] = L[r] + orc_log(norm_ref(n, V + r * n) / eps);
+        orc_normalize_ref(n, k, V, eps);
+    }
+    for (int r = 0; r < k; ++r) lam[r] = L[r] / d->t_average;
+    *steps = (uint64_t)(n_tr + n_sl + n_av);
+}
+
+/* reference FD (two-trajectory) mode for maps: cuda/lyapunov.dp.cpp:70-122,127-128 */
+static void point_map_fd_ref(const orc_desc* d, const double* p, double* x, double* lam,
+                             uint64_t* steps) {
+    const int n = d->dim, k = d->n_exps;
+    const double eps = d->eps;
+    const int64_t n_tr = orc_loop_count(0.0, d->t_transient, 1.0);
+    const int64_t n_sl = orc_loop_count(0.0, d->t_slave, 1.0);
+    const int64_t n_av = orc_loop_count(0.0, d->t_average, 1.0);
+    double S[ORC_MAX_DIM * ORC_MAX_DIM], L[ORC_MAX_DIM];
+    for (int r = 0; r < k; ++r) L[r] = 0.0;
+    for (int64_t s = 0; s < n_tr; ++s) map_step(d->model, n, x, p);
+    for (int r = 0; r < k; ++r) {
+        memcpy(S + r * n, x, sizeof(double) * n);
+        S[r * n + r] = S[r * n + r] + eps;
+    }
+    for (int64_t s = 0; s < n_sl; ++s) {
+        map_step(d->model, n, x, p);
+        for (int r = 0; r < k; ++r) {
+            map_step(d->model, n, S + r * n, p);
+            for (int i = 0; i < n; ++i) S[r * n + i] = S[r * n + i] - x[i];
+        }
+        orc_cgs_ref(n, k, S);
+        orc_normalize_ref(n, k, S, eps);
+        for (int r = 0; r < k; ++r)
+            for (int i = 0; i < n; ++i) S[r * n + i] = S[r * n + i] + x[i];
+    }
+    for (int64_t s = 0; s < n_av; ++s) {
+        map_step(d->model, n, x, p);
+        for (int r = 0; r < k; ++r) map_step(d->model, n, S + r * n, p);
+        for (int r = 0; r < k; ++r)
+            for (int i = 0; i < n; ++i) S[r * n + i] = S[r * n + i] - x[i];
+        orc_cgs_ref(n, k, S);
+        for (int r = 0; r < k; ++r) L[r] = L[r] + orc_log(norm_ref(n, S + r * n) / eps);
+        orc_normalize_ref(n, k, S, eps);
+        for (int r = 0; r < k; ++r)
+            for (int i = 0; i < n; ++i) S[r * n + i] = S[r * n + i] + x[i];
+    }
+    for (int r = 0; r < k; ++r) lam[r] = L[r] / d->t_average;
+    *steps = (uint64_t)(n_tr + n_sl + n_av);
+}
+
 void orc_point(const orc_desc* d, int64_t idx, const double* params_in, const double* inits_in,
                double* lam, uint32_t* status, uint32_t* n_acc, uint32_t* n_rej) {
     double p[ORC_MAX_PARAMS], x[ORC_MAX_DIM];
@@ -912,6 +1073,26 @@ void orc_point(const orc_desc* d, int64_t idx, const double* params_in, const do
     orc_point_init(d, idx, p, inits_in, x);
     uint64_t steps = 0;
     *n_rej = 0;
+    if (orc_is_map(d->model)) {
+        switch (d->method) {
+        case ORC_METHOD_VAR:
+            point_map_var(d, p, x, lam, status, n_acc);
+            return;
+        case ORC_METHOD_VAR_REF: point_map_var_ref(d, p, x, lam, &steps); break;
+        case ORC_METHOD_FD_REF: point_map_fd_ref(d, p, x, lam, &steps); break;
+        default:
+            for (int r = 0; r < d->n_exps; ++r) lam[r] = NAN;
+            *status = ORC_NONFINITE;
+            *n_acc = 0;
+            return;
+        }
+        uint32_t st = ORC_OK;
+        for (int r = 0; r < d->n_exps; ++r)
+            if (!isfinite(lam[r])) st = ORC_NONFINITE;
+        *status = st;
+        *n_acc = (uint32_t)steps;
+        return;
+    }
     switch (d->method) {
     case ORC_METHOD_VAR:
         point_var(d, p, x, lam, status, n_acc, n_rej);

@@ -7,6 +7,10 @@
  *   - reference VAR mode  : cuda/lyapunov.dp.cpp:133-226 (deviceLESpectrumVARDverk_)
  *   - reference QR mode   : cuda/lyapunov.dp.cpp:229-259 (deviceLESpectrumQR_)
  *   - reference FD mode   : cuda/lyapunov.dp.cpp:5-129   (deviceLESpectrumDverk_)
+ *   - the map branches of the VAR and FD modes: cuda/lyapunov.dp.cpp:182-222 and
+ *     70-122, StepMAP/StepMAPVAR (cuda/tools/mapstep.dp.cpp:5-18); the
+ *     reference registers no map model, so the two maps here (Henon,
+ *     logistic) are this build's
  *   - DVERK 6(5) tableau  : cuda/tools/dverk.dp.cpp:7-80
  *   - Dopri5 + controller : cuda/tools/dopri.dp.cpp:7-140
  *   - CGS / normalise / QR: cuda/tools/linalg.dp.cpp:5-128
@@ -31,7 +35,10 @@ extern "C" {
 #define ORC_MAX_DIM 32
 #define ORC_MAX_PARAMS 8
 
-enum { ORC_MODEL_SHIMIZU = 0, ORC_MODEL_LORENZ = 1, ORC_MODEL_ROSSLER = 2, ORC_MODEL_L96 = 3 };
+enum { ORC_MODEL_SHIMIZU = 0, ORC_MODEL_LORENZ = 1, ORC_MODEL_ROSSLER = 2, ORC_MODEL_L96 = 3,
+       ORC_MODEL_HENON = 4, ORC_MODEL_LOGISTIC = 5 };
+/* maps: orc_rhs returns the next state f(x) and orc_jvp the product J(x) v */
+int orc_is_map(int model);
 enum { ORC_METHOD_VAR = 0, ORC_METHOD_VAR_REF = 1, ORC_METHOD_QR_REF = 2, ORC_METHOD_FD_REF = 3 };
 enum { ORC_INT_DVERK = 0, ORC_INT_DOPRI5 = 1 };
 enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };

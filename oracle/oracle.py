@@ -21,6 +21,7 @@ MAX_DIM = 32
 MAX_PARAMS = 8
 
 MODEL_SHIMIZU, MODEL_LORENZ, MODEL_ROSSLER, MODEL_L96 = 0, 1, 2, 3
+MODEL_HENON, MODEL_LOGISTIC = 4, 5
 METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF = 0, 1, 2, 3
 INT_DVERK, INT_DOPRI5 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1

@@ -162,7 +162,7 @@ int main() {
     test_task_init_and_output();
     // models registered at start-up, sorted as the reference's std::map
     auto& st = Singleton<ModelFactoryRegistry::Storage>::Instance();
-    CHECK(st._data.size() == 6 && st._data.begin()->first == "lorenz");
+    CHECK(st._data.size() == 8 && st._data.begin()->first == "henon");
     CHECK(st._data.at("shimizu")._param_number == 2);
     std::printf(g_fail ? "FAILED %d\n" : "OK\n", g_fail);
     return g_fail ? 1 : 0;

@@ -33,10 +33,14 @@ def test_library_exports_every_declared_symbol():
 
 def test_model_table_and_lookup():
     names = [m.name.decode() for m in L.models()]
-    assert names == ["shimizu", "lorenz", "rossler", "lorenz96_8", "lorenz96_16", "lorenz96_32"]
+    assert names == ["shimizu", "lorenz", "rossler", "lorenz96_8", "lorenz96_16", "lorenz96_32",
+                     "henon", "logistic"]
     info = L.model_lookup("shimizu")
     assert (info.model, info.dim, info.n_params, info.type) == (L.MODEL_SHIMIZU, 3, 2, L.TYPE_FLOW)
     assert L.model_lookup("lorenz96_16").dim == 16
+    info = L.model_lookup("henon")
+    assert (info.model, info.dim, info.n_params, info.type) == (L.MODEL_HENON, 2, 2, L.TYPE_MAP)
+    assert L.model_lookup("logistic").type == L.TYPE_MAP
     with pytest.raises(L.LyapError) as e:
         L.model_lookup("lorenz4d")
     assert e.value.code == L.ERR_NOT_FOUND
@@ -142,3 +146,24 @@ def test_batch_record_layout():
     assert L.BatchClassRec.count.offset == ctypes.sizeof(L.SweepDesc)
     assert ctypes.sizeof(L.BatchClassRec) == ctypes.sizeof(L.SweepDesc) + 7 * 8
     assert ctypes.sizeof(L.BatchTuning) == 20
+
+
+def test_finalize_maps():
+    from chaos_oneapi_amd import workloads as W
+
+    d = L.finalize(L.make_desc(W.henon(n=4)))
+    assert d.h == 1.0 and d.steps_per_renorm == 10
+    assert d.n_renorm_transient == 100 and d.n_renorm_average == 10000
+    spec = W.henon(n=4)
+    spec.update(method=L.METHOD_VAR_REF, eps=1e-8, t_transient=100.5, t_average=1000.0)
+    d = L.finalize(L.make_desc(spec))
+    assert (d.ref_steps_transient, d.ref_steps_average) == (101, 1000)
+    bad = W.henon(n=4)
+    bad.update(adaptive=1, rtol=1e-9, atol=1e-9)
+    with pytest.raises(L.LyapError, match="no adaptive step"):
+        L.finalize(L.make_desc(bad))
+    qr = W.logistic(n=4)
+    qr.update(method=L.METHOD_QR_REF)
+    with pytest.raises(L.LyapError, match="no map branch") as e:
+        L.finalize(L.make_desc(qr))
+    assert e.value.code == L.ERR_UNSUPPORTED

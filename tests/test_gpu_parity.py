@@ -141,3 +141,20 @@ def test_l96_wave_drain_matches_oracle(gpu_ctx, n, ppw, integ, adaptive, prec):
         gpu_ctx.set_option(f"l96_ppw_{n}", 0)
         gpu_ctx.set_option("l96_max_blocks", 0)
     assert_match(g, o, TOL_FP64, exact=True)
+
+
+@pytest.mark.parametrize("model", ["henon", "logistic"])
+@pytest.mark.parametrize("method,prec", [(L.METHOD_VAR, L.PREC_FP64), (L.METHOD_VAR, L.PREC_MIXED),
+                                         (L.METHOD_VAR_REF, L.PREC_FP64),
+                                         (L.METHOD_FD_REF, L.PREC_FP64)])
+def test_maps_match_oracle(gpu_ctx, model, method, prec):
+    """Map-type models: the sweep's VAR method (FP64 and binary32 tangent
+    products) and the reference's VAR/FD map branches; escaping Henon orbits
+    included (a up to 1.42, b up to 0.35)."""
+    if model == "henon":
+        spec = W.henon(n=12, precision=prec, t_transient=100.0, t_average=2000.0)
+    else:
+        spec = W.logistic(n=96, precision=prec, t_transient=100.0, t_average=2000.0)
+    spec.update(method=method, eps=1e-8)
+    g, o = run_both(gpu_ctx, spec)
+    assert_match(g, o, TOL_FP64, exact=True)

@@ -1,0 +1,44 @@
+"""Surrogate-guided refinement end to end on a small plane: what the trained
+network gains over running the method everywhere, at what agreement.
+
+Rossler (b=0.2) on a 256x256 (a, c) grid with shortened times, mixed form; a
+64x64 training lattice, calibrated std multiplier, 1 % calibration and 1 %
+verification subsamples (FP64 judge).  The gain is the fraction of points the
+surrogate answers (no integration); the bar is the class agreement of those
+answers with the FP64 judge on the held-out verification points and that the
+final map's computed points equal a full sweep's."""
+import numpy as np
+import pytest
+import torch
+
+from chaos_oneapi_amd import lyap as L
+from chaos_oneapi_amd import refine as R
+from chaos_oneapi_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_refinement_gain_and_agreement(gpu_ctx):
+    spec = W.rossler_dverk(256, precision=L.PREC_MIXED, t_transient=50.0, t_average=500.0)
+    print("case refine rossler 256x256 short times", flush=True)
+    res = R.refine(gpu_ctx, spec, train_n=64, epochs=60, batch=1024, infer_batch=65536)
+    st = res.stats
+    print({k: v for k, v in st.items() if k != "losses"}, flush=True)
+    print("timings", res.timings, flush=True)
+    # the gain: points the surrogate answered, with no integration
+    answered = 1.0 - st["rerun_fraction"]
+    print(f"surrogate answered {answered:.3%} of the plane (kappa {st['kappa']})", flush=True)
+    assert st["train_on_lattice"]
+    assert st["losses"][-1] < st["losses"][0]
+    assert st["class_agreement"] >= 0.995
+    if st["surrogate_answered_verify"] >= 50:
+        assert st["surrogate_class_agreement"] >= 0.99
+    # computed points are the method's own values: equal to a plain sweep of the plane
+    desc = L.finalize(L.make_desc(spec))
+    lam, status, _, _ = gpu_ctx.run(desc)
+    torch.cuda.synchronize()
+    lam, status = lam.cpu().numpy(), status.cpu().numpy()
+    comp = res.rerun
+    np.testing.assert_array_equal(res.status[comp], status[comp])
+    ok = comp & (status == 0)
+    np.testing.assert_array_equal(res.lam[ok], lam[ok])

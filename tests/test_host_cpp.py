@@ -31,7 +31,8 @@ def run_cli(cwd, *args):
 def test_cli_lists_models_and_reports_missing_config(tmp_path):
     r = run_cli(tmp_path)
     assert r.returncode == 1
-    assert 'Model[0]: "lorenz"  Dimension: 3 Number of parameters: 3' in r.stdout
+    assert 'Model[0]: "henon"  Dimension: 2 Number of parameters: 2' in r.stdout
+    assert 'Model[2]: "lorenz"  Dimension: 3 Number of parameters: 3' in r.stdout
     assert "error: config file: config.json not found" in r.stdout
 
 
