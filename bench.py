@@ -13,6 +13,8 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
       FP64 judge on a seeded 1 % verification subsample.
   rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
   lorenz-64     Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01; one mixed-form sweep.
+  lorenz96-16   Lorenz-96 N=16 on a 256x256 (F, d) grid, adaptive Dopri5 rtol=1e-9; the
+      tile shape (waves per point or points per wave) picked by the autotuner first.
   batch-mix     2^20 points (--batch-points) drawn as 40% Lorenz, 30% Rossler, 20%
       Lorenz-96 N=8, 8% N=16, 2% N=32, adaptive Dopri5 rtol=1e-9, mixed form; a
       step is one replay of the captured persistent batched launch, tuned first
@@ -115,6 +117,11 @@ def bench_sweep(args, ctx, spec, wl_name):
 
     desc = L.finalize(L.make_desc(spec))
     n = desc.n1 * desc.n2
+    tuned = None
+    if desc.model == L.MODEL_L96:
+        from chaos_oneapi_amd import autotune as A
+
+        tuned = A.tune_l96(ctx, desc, cache=A.Cache(None))  # sets the tile shape on ctx
     stream = torch.cuda.current_stream()
     dev = torch.device("cuda", 0)
     bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
@@ -154,7 +161,12 @@ def bench_sweep(args, ctx, spec, wl_name):
         "mean_steps_per_point": float(na_c.mean()), "mean_rejects_per_point": float(nr_c.mean()),
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
-    rec["roofline"] = roofline(achieved, "k_var_lane (mixed) over the grid")
+    if tuned is not None:
+        rec["config"]["l96_tile"] = list(tuned.best)
+        rec["config"]["l96_tuning_ms_sample"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}
+    kname = "k_var_lane" if desc.model != L.MODEL_L96 else (
+        "k_l96_wave" if tuned is not None and tuned.best[0] == "ppw" else "k_l96")
+    rec["roofline"] = roofline(achieved, kname + " (mixed) over the grid")
     if not args.no_cpu:
         cpu, parity = cpu_baseline(desc, lam_c, lam_p, args.cpu_budget, args.cpu_threads)
         rec["cpu_baseline"] = cpu
@@ -201,7 +213,8 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
         "rerun_fraction": st["rerun_fraction"], "train_points": st["train_points"],
         "verify_points": st["verify"], "calibration_points": st["calib"],
-        "std_multiplier_kappa": st["kappa"], "calibration_agreement": st["calib_agreement"],
+        "std_multiplier_kappa": st["kappa"], "escape_answers": st["escape_answers"],
+        "calibration_agreement": st["calib_agreement"],
         "max_dlambda1_chosen_vs_fp64_verify": st.get("max_dlambda1_chosen_vs_fp64"),
         "class_agreement_verify": st.get("class_agreement"),
         "surrogate_answered_verify": st.get("surrogate_answered_verify"),
@@ -383,6 +396,8 @@ def main():
             spec, name = W.rossler_dverk(512, precision=L.PREC_MIXED), "rossler-512x512-dverk65-adaptive"
         elif args.workload == "lorenz-64":
             spec, name = W.lorenz_fixed(64, precision=L.PREC_MIXED), "lorenz-64x64-dopri5-fixed"
+        elif args.workload == "lorenz96-16":
+            spec, name = W.lorenz96_dopri(256, 16, precision=L.PREC_MIXED), "lorenz96-16-256x256-dopri5-adaptive"
         elif args.workload == "batch-mix":
             spec, name = None, f"batch-mix-{args.batch_points}-dopri5-adaptive-1e-9"
         else:

@@ -187,10 +187,12 @@ def batch_kwargs(best) -> dict:
     return dict(layout=layout, occupancy3=bool(occ3), waves_per_cu=wpc, order=order, gather=gather)
 
 
-def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 256, cache: Cache | None = None,
+def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 16384, cache: Cache | None = None,
              force: bool = False, apply: bool = True, timer=None) -> TuneResult:
     """Pick the Lorenz-96 tile shape of a single sweep over `desc`'s grid and
-    (apply=True) set it on ctx."""
+    (apply=True) set it on ctx.  The sample keeps at least min_points points
+    (the whole grid when smaller): fewer than a few per wave slot and the
+    launch is timed by its slowest points, not by the shape."""
     import torch
 
     cache = cache if cache is not None else Cache()

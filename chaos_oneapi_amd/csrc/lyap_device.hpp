@@ -446,40 +446,94 @@ struct Logistic {  // (r): x' = r x (1 - x)
 // step-size control (constants and update rule of cuda/tools/dopri.dp.cpp:9-12,
 // 100-139; the exponent is 1/q - 0.75*beta with q the embedded order + 1)
 // ---------------------------------------------------------------------------
+// Step-control log and exp: the det_log / det_exp reductions with shorter
+// series (about 1e-10 relative, ample for a step-size factor).  ctrl_exp is
+// only asked for the range that survives the factor limits of ctrl_update, so
+// its argument is clamped to [-1.6, 2.5] first.
+LYAP_HD double ctrl_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return __builtin_nan("");
+    if (x == 0.0) return -kInf;
+    if (x == kInf) return kInf;
+    uint64_t b = bits_of(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    int k;
+    if (e == 0) {
+        x = x * 18014398509481984.0;
+        b = bits_of(x);
+        e = (int)((b >> 52) & 0x7ff);
+        k = e - 1023 - 54;
+    } else {
+        k = e - 1023;
+    }
+    double m = double_of((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        k = k + 1;
+    }
+    const double f = m - 1.0;
+    const double s = f / (2.0 + f);
+    const double z = s * s;
+    double r = sconst(2.0 / 11.0);
+    r = fma(r, z, sconst(2.0 / 9.0));
+    r = fma(r, z, sconst(2.0 / 7.0));
+    r = fma(r, z, sconst(2.0 / 5.0));
+    r = fma(r, z, sconst(2.0 / 3.0));
+    r = r * z;
+    const double lg = f - s * (f - r);
+    const double dk = (double)k;
+    return fma(dk, sconst(kLn2Hi), fma(dk, sconst(kLn2Lo), lg));
+}
+
+LYAP_HD double ctrl_exp(double x) {
+    x = x < -1.6 ? -1.6 : x;
+    x = x > 2.5 ? 2.5 : x;
+    const double kd = floor(x * 1.4426950408889634 + 0.5);
+    const double r = fma(-kd, sconst(kLn2Lo), fma(-kd, sconst(kLn2Hi), x));
+    double p = sconst(1.0 / 40320.0);
+    p = fma(p, r, sconst(1.0 / 5040.0));
+    p = fma(p, r, sconst(1.0 / 720.0));
+    p = fma(p, r, sconst(1.0 / 120.0));
+    p = fma(p, r, sconst(1.0 / 24.0));
+    p = fma(p, r, sconst(1.0 / 6.0));
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return p * double_of((uint64_t)(1023 + (int)kd) << 52);
+}
+
 struct Ctrl {
-    double fac_old;
-    double log_fac_old;  // det_log(fac_old), carried from the step that set it
+    double log_fac_old;  // ctrl_log of facOld (floored at 1e-4)
     bool rejected;
 };
 
-// lerr = det_log(err).  det_pow(x, y) = det_exp(y * det_log(x)), so carrying
-// the log of fac_old from the accepted step that produced it (it is err, or
-// the 1e-4 floor with log_floor = det_log(1e-4)) gives the same bits as
-// recomputing det_pow(fac_old, beta).
+// dopriStep's PI rule (cuda/tools/dopri.dp.cpp:109-139): fac = err^expo /
+// facOld^beta limited to [1/10, 1/0.2] after division by the safety factor,
+// h_new = h / fac; after a rejection h_new = h / min(5, err^expo / 0.9).  Here
+// in the equivalent form h_new = h * r, r = clamp(0.9 * exp(-(expo*lerr -
+// beta*log facOld)), 0.2, 10): one exponential and no division.
+// lerr = ctrl_log(err), log_floor = ctrl_log(1e-4).
 template <int INTEG>
 LYAP_D bool ctrl_update(Ctrl& c, double err, double lerr, double log_floor, double h_try,
                         double h_max, double& h_next) {
-    constexpr double kSafe = 0.9, kFacc1 = 1.0 / 0.2, kFacc2 = 1.0 / 10.0, kBeta = 0.04;
-    const double fac11 = det_exp(Tab<INTEG>::expo * lerr);
-    double fac = fac11 / det_exp(kBeta * c.log_fac_old);
-    const double q = fac / kSafe;
-    fac = kFacc1 < q ? kFacc1 : q;
-    fac = kFacc2 > fac ? kFacc2 : fac;
-    double hn = h_try / fac;
+    constexpr double kSafe = 0.9, kBeta = 0.04;
     if (err <= 1.0) {
-        const bool floor = !(err > 1e-4);
-        c.fac_old = floor ? 1e-4 : err;
-        c.log_fac_old = floor ? log_floor : lerr;
+        const double e = fma(-kBeta, c.log_fac_old, Tab<INTEG>::expo * lerr);
+        double r = kSafe * ctrl_exp(-e);
+        r = r < 0.2 ? 0.2 : r;
+        r = r > 10.0 ? 10.0 : r;
+        double hn = h_try * r;
+        c.log_fac_old = err > 1e-4 ? lerr : log_floor;
         if (fabs(hn) > h_max) hn = h_max;
         if (c.rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
         c.rejected = false;
         h_next = hn;
         return true;
     }
-    const double q2 = fac11 / kSafe;
-    hn = h_try / (kFacc1 < q2 ? kFacc1 : q2);
+    double r = kSafe * ctrl_exp(-(Tab<INTEG>::expo * lerr));
+    r = r < 0.2 ? 0.2 : r;
     c.rejected = true;
-    h_next = hn;
+    h_next = h_try * r;
     return false;
 }
 

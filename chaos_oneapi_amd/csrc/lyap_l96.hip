@@ -55,7 +55,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
     const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
                                : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
     const int n_exps = d.n_exps;
-    const double log_floor = det_log(1e-4);
+    const double log_floor = ctrl_log(1e-4);
 
     for (;;) {
         if (tid == 0) s_item = atomicAdd(io.counter, 1ULL);
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
         double t = 0.0, h = dp->h;
         int64_t mr = 0, step_in = 0;
         uint32_t nacc = 0, nrej = 0;
-        Ctrl ctrl{1e-4, log_floor, false};
+        Ctrl ctrl{log_floor, false};
         uint32_t st = LYAP_PT_OK;
         double k1 = 0.0;
         if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                         break;
                     }
                     double h_next;
-                    accept = ctrl_update<INTEG>(ctrl, err, det_log(err), log_floor, h_try, h_max,
+                    accept = ctrl_update<INTEG>(ctrl, err, ctrl_log(err), log_floor, h_try, h_max,
                                                 h_next);
                     accept = wave_uniform(accept);
                     h = h_next;

@@ -185,13 +185,13 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
     const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
                                : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
     const int n_exps = d.n_exps;
-    const double log_floor = det_log(1e-4);
+    const double log_floor = ctrl_log(1e-4);
 
     double x = 0.0, V[E], sum = 0.0, comp = 0.0, t = 0.0, h = 0.0, k1 = 0.0, F = 0.0, dd = 0.0;
     T dT = T(0);
     int64_t mr = 0, step_in = 0, pos = 0;
     uint32_t nacc = 0, nrej = 0;
-    Ctrl ctrl{1e-4, log_floor, false};
+    Ctrl ctrl{log_floor, false};
     bool active = false, exhausted = false;
 #pragma unroll
     for (int m = 0; m < E; ++m) V[m] = 0.0;
@@ -218,7 +218,6 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 h = dp->h;
                 mr = step_in = 0;
                 nacc = nrej = 0;
-                ctrl.fac_old = 1e-4;
                 ctrl.log_fac_old = log_floor;
                 ctrl.rejected = false;
                 if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
@@ -321,7 +320,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     accept = false;
                 } else {
                     double h_next;
-                    accept = ctrl_update<INTEG>(ctrl, err, det_log(err), log_floor, h_try, h_max,
+                    accept = ctrl_update<INTEG>(ctrl, err, ctrl_log(err), log_floor, h_try, h_max,
                                                 h_next);
                     accept = slot_uniform(accept);
                     h = h_next;

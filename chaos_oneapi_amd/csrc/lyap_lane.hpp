@@ -183,14 +183,14 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
                                : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
     const int n_exps = d.n_exps;
-    const double log_floor = det_log(1e-4);
+    const double log_floor = ctrl_log(1e-4);
 
     double x[N], V[K][N], sum[K], comp[K], p[P], k1[N];
     float pf[P];
     double t = 0.0, h = 0.0;
     int64_t m = 0, step_in = 0, item = -1;
     uint32_t nacc = 0, nrej = 0;
-    Ctrl ctrl{1e-4, log_floor, false};
+    Ctrl ctrl{log_floor, false};
     bool active = false, exhausted = false;
     // a lane that reached a renormalisation boundary waits (pending) until
     // io.gather lanes of the wave wait, or every active lane does; then they
@@ -237,7 +237,6 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 m = 0;
                 step_in = 0;
                 nacc = nrej = 0;
-                ctrl.fac_old = 1e-4;
                 ctrl.log_fac_old = log_floor;
                 ctrl.rejected = false;
                 if (INTEG == LYAP_INT_DOPRI5 && ADAPT) M::rhs(x, p, k1);
@@ -305,7 +304,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                         finished = true;
                     } else {
                         double h_next;
-                        const double lerr = det_log(err);
+                        const double lerr = ctrl_log(err);
                         const bool acc =
                             ctrl_update<INTEG>(ctrl, err, lerr, log_floor, h_try, h_max, h_next);
                         h = h_next;

@@ -64,33 +64,42 @@ class RefineResult:
 KAPPAS = (1.0, 1.5, 2.0, 3.0, 5.0, 8.0, 16.0)
 
 
-def answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi):
+def answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes=True):
     """Points the surrogate answers at std multiplier kappa: confidently
-    escaping, or decided (not escaping), away from lambda1 = 0 and with
-    kappa * std below the threshold.  Works on numpy arrays and torch tensors."""
-    escaping = pbad >= bad_hi
+    escaping (when escapes is set), or decided (not escaping), away from
+    lambda1 = 0 and with kappa * std below the threshold.  Works on numpy
+    arrays and torch tensors."""
     settled = (kappa * std1 <= std_thr) & (abs(mu1) >= lam_thr) & (pbad <= bad_lo)
-    return escaping | settled
+    return ((pbad >= bad_hi) | settled) if escapes else settled
 
 
 def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target):
-    """Smallest kappa in KAPPAS whose answerable calibration points agree with
-    their computed class on at least `target` of them (inf: answer nothing).
-    pred [n,5] numpy, lam/status the method's values at the same points."""
+    """The answer rule with the most answered calibration points among those
+    whose answers agree with the computed class on at least `target` of
+    them: kappa in KAPPAS, with or without the escape answers (which kappa
+    does not scale).  Returns (kappa, escapes, agreement, answered); kappa inf:
+    the surrogate answers nothing.  pred [n,5] numpy, lam/status the method's
+    values at the same points."""
     import numpy as np
 
     computed = classify(lam[:, 0], status)
     sur_status = np.where(pred[:, 4] >= 0.5, 2, 0)
     sur = classify(pred[:, 0], sur_status)
     finite = np.isfinite(pred[:, 0])
-    for kappa in KAPPAS:
-        a = answerable(pred[:, 0], pred[:, 3], pred[:, 4], kappa, std_thr, lam_thr, bad_lo, bad_hi) & finite
-        if not a.any():
-            return kappa, 1.0, 0
-        agree = float(np.mean(sur[a] == computed[a]))
-        if agree >= target:
-            return kappa, agree, int(a.sum())
-    return float("inf"), 1.0, 0
+    best = (float("inf"), False, 1.0, 0)
+    for escapes in (True, False):
+        for kappa in KAPPAS:
+            a = answerable(pred[:, 0], pred[:, 3], pred[:, 4], kappa, std_thr, lam_thr, bad_lo,
+                           bad_hi, escapes) & finite
+            n_a = int(a.sum())
+            if n_a == 0:
+                break
+            agree = float(np.mean(sur[a] == computed[a]))
+            if agree >= target:
+                if n_a > best[3]:
+                    best = (kappa, escapes, agree, n_a)
+                break  # larger kappa only answers fewer points
+    return best
 
 
 def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4096,
@@ -195,7 +204,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     #    calibration subsample agree with the method (the verification
     #    subsample stays out of it), then the rerun set: every point the
     #    surrogate does not answer and that step 1 did not compute
-    kappa, cal_agree, cal_answered = calibrate_kappa(
+    kappa, escapes, cal_agree, cal_answered = calibrate_kappa(
         pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
         bad_lo, bad_hi, calib_target)
     tm["calibrate"] = time.perf_counter() - t
@@ -203,7 +212,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     if kappa == float("inf"):
         flagged = torch.ones(n, dtype=torch.bool, device=dev)
     else:
-        flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi)
+        flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes)
         flagged |= ~torch.isfinite(mu1)
     vmask = torch.zeros(n, dtype=torch.bool, device=dev)
     vmask[verify] = True
@@ -235,10 +244,11 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     rerun_mask = flagged | vmask | cmask | tmask
     tm["total_chosen"] = time.perf_counter() - t_all
 
-    n_rerun = int(rerun_idx.numel()) + n_ver + n_cal + (n_t if lattice else 0)
+    n_rerun = int(rerun_mask.sum().item())  # distinct points the method computed
     stats = dict(points=n, train_points=n_t, rerun=n_rerun,
                  flagged=int(flagged.sum().item()), verify=n_ver, calib=n_cal,
-                 kappa=kappa, calib_agreement=cal_agree, calib_answered=cal_answered,
+                 kappa=kappa, escape_answers=bool(escapes), calib_agreement=cal_agree,
+                 calib_answered=cal_answered,
                  train_on_lattice=bool(lattice),
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),

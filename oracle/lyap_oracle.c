@@ -613,39 +613,95 @@ double orc_err_norm(int n, int tree, const double* ev, const double* x, const do
     return sqrt(s / (double)n);
 }
 
+/* Step-control log and exp: the reductions of orc_log / orc_exp with shorter
+   series (about 1e-10 relative, ample for a step-size factor).  ctrl_exp is
+   only asked for the range that survives the factor limits below, so its
+   argument is clamped to [-1.6, 2.5] first. */
+double orc_ctrl_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return NAN;
+    if (x == 0.0) return -INFINITY;
+    if (x == INFINITY) return INFINITY;
+    uint64_t b = as_u64(x);
+    int e = (int)((b >> 52) & 0x7ff), k;
+    if (e == 0) {
+        x = x * 18014398509481984.0;
+        b = as_u64(x);
+        e = (int)((b >> 52) & 0x7ff);
+        k = e - 1023 - 54;
+    } else {
+        k = e - 1023;
+    }
+    double m = as_f64((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
+    if (m > 1.4142135623730951) {
+        m = m * 0.5;
+        k = k + 1;
+    }
+    const double f = m - 1.0, s = f / (2.0 + f), z = s * s;
+    double r = 2.0 / 11.0;
+    r = fma(r, z, 2.0 / 9.0);
+    r = fma(r, z, 2.0 / 7.0);
+    r = fma(r, z, 2.0 / 5.0);
+    r = fma(r, z, 2.0 / 3.0);
+    r = r * z;
+    const double lg = f - s * (f - r), dk = (double)k;
+    return fma(dk, LN2_HI, fma(dk, LN2_LO, lg));
+}
+
+double orc_ctrl_exp(double x) {
+    x = x < -1.6 ? -1.6 : x;
+    x = x > 2.5 ? 2.5 : x;
+    const double kd = floor(x * 1.4426950408889634 + 0.5);
+    const double r = fma(-kd, LN2_LO, fma(-kd, LN2_HI, x));
+    double p = 1.0 / 40320.0;
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return p * as_f64((uint64_t)(1023 + (int)kd) << 52);
+}
+
 typedef struct {
-    double fac_old;
+    double log_fac_old; /* log of facOld, floored at 1e-4 */
     int rejected;
 } ctrl_t;
 
-static const double CTRL_SAFE = 0.9, CTRL_FACC1 = 1.0 / 0.2, CTRL_FACC2 = 1.0 / 10.0,
-                    CTRL_BETA = 0.04;
+static const double CTRL_SAFE = 0.9, CTRL_BETA = 0.04;
 
 static double ctrl_expo(int integ) {
     return integ == ORC_INT_DVERK ? (1.0 / 6.0 - 0.75 * 0.04) : (1.0 / 5.0 - 0.75 * 0.04);
 }
 
-/* returns 1 on accept; *h_next is the step for the next attempt */
-static int ctrl_update(ctrl_t* c, int integ, double err, double h_try, double h_max,
-                       double* h_next) {
-    double fac11 = orc_pow(err, ctrl_expo(integ));
-    double fac = fac11 / orc_pow(c->fac_old, CTRL_BETA);
-    double q = fac / CTRL_SAFE;
-    fac = CTRL_FACC1 < q ? CTRL_FACC1 : q;
-    fac = CTRL_FACC2 > fac ? CTRL_FACC2 : fac;
-    double hn = h_try / fac;
+/* dopriStep's PI rule (cuda/tools/dopri.dp.cpp:109-139): fac = err^expo /
+   facOld^beta, limited to [1/10, 1/0.2] after division by the safety factor,
+   h_new = h / fac; after a rejection h_new = h / min(5, err^expo / 0.9).  Here
+   in the equivalent form h_new = h * r, r = clamp(0.9 * exp(-(expo*lerr -
+   beta*log facOld)), 0.2, 10): one exponential and no division.  lerr is
+   orc_ctrl_log(err).  Returns 1 on accept; *h_next is the next attempt. */
+static int ctrl_update(ctrl_t* c, int integ, double err, double lerr, double log_floor,
+                       double h_try, double h_max, double* h_next) {
+    const double expo = ctrl_expo(integ);
     if (err <= 1.0) {
-        c->fac_old = err > 1e-4 ? err : 1e-4;
+        const double e = fma(-CTRL_BETA, c->log_fac_old, expo * lerr);
+        double r = CTRL_SAFE * orc_ctrl_exp(-e);
+        r = r < 0.2 ? 0.2 : r;
+        r = r > 10.0 ? 10.0 : r;
+        double hn = h_try * r;
+        c->log_fac_old = err > 1e-4 ? lerr : log_floor;
         if (fabs(hn) > h_max) hn = h_max;
         if (c->rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
         c->rejected = 0;
         *h_next = hn;
         return 1;
     }
-    double q2 = fac11 / CTRL_SAFE;
-    hn = h_try / (CTRL_FACC1 < q2 ? CTRL_FACC1 : q2);
+    double r = CTRL_SAFE * orc_ctrl_exp(-(expo * lerr));
+    r = r < 0.2 ? 0.2 : r;
     c->rejected = 1;
-    *h_next = hn;
+    *h_next = h_try * r;
     return 0;
 }
 
@@ -759,7 +815,8 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
         }
     } else {
         t_avg_exact = (double)n_av * d->t_renorm;
-        ctrl_t c = {1e-4, 0};
+        const double log_floor = orc_ctrl_log(1e-4);
+        ctrl_t c = {log_floor, 0};
         double t = 0.0, h = d->h;
         int64_t m = 0;
         double t_next = (double)(m + 1) * d->t_renorm;
@@ -777,7 +834,8 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             double err = orc_err_norm(n, tree, ev, x, xn, d->rtol, d->atol);
             if (!(err < INFINITY)) { st = ORC_NONFINITE; break; }
             double h_next;
-            int acc = ctrl_update(&c, integ, err, h_try, d->h_max, &h_next);
+            int acc = ctrl_update(&c, integ, err, orc_ctrl_log(err), log_floor, h_try, d->h_max,
+                                  &h_next);
             h = h_next;
             if (!acc) { ++n_rej; continue; }
             for (int r = 0; r < k; ++r) {

@@ -39,6 +39,10 @@ enum { ORC_MODEL_SHIMIZU = 0, ORC_MODEL_LORENZ = 1, ORC_MODEL_ROSSLER = 2, ORC_M
        ORC_MODEL_HENON = 4, ORC_MODEL_LOGISTIC = 5 };
 /* maps: orc_rhs returns the next state f(x) and orc_jvp the product J(x) v */
 int orc_is_map(int model);
+/* the step controller's log and exp (shorter series; exp clamps its argument
+   to [-1.6, 2.5], the range that survives the step-factor limits) */
+double orc_ctrl_log(double x);
+double orc_ctrl_exp(double x);
 enum { ORC_METHOD_VAR = 0, ORC_METHOD_VAR_REF = 1, ORC_METHOD_QR_REF = 2, ORC_METHOD_FD_REF = 3 };
 enum { ORC_INT_DVERK = 0, ORC_INT_DOPRI5 = 1 };
 enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };

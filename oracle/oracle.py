@@ -75,6 +75,8 @@ def lib():
         L.orc_log.restype = d; L.orc_log.argtypes = [d]
         L.orc_exp.restype = d; L.orc_exp.argtypes = [d]
         L.orc_pow.restype = d; L.orc_pow.argtypes = [d, d]
+        L.orc_ctrl_log.restype = d; L.orc_ctrl_log.argtypes = [d]
+        L.orc_ctrl_exp.restype = d; L.orc_ctrl_exp.argtypes = [d]
         L.orc_uniform.restype = d; L.orc_uniform.argtypes = [u64, i64, i64]
         L.orc_loop_count.restype = i64; L.orc_loop_count.argtypes = [d, d, d]
         L.orc_rhs.argtypes = [i, i, P, P, P]

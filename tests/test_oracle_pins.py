@@ -34,6 +34,21 @@ def test_log_exp_against_libm():
         assert abs(L.orc_pow(x, y) - x ** y) <= 2e-15 * x ** y
 
 
+def test_step_control_log_exp():
+    """The controller's shorter-series log/exp: ~1e-10 relative against libm;
+    exp clamps its argument to [-1.6, 2.5] (the range the factor limits keep)."""
+    L = O.lib()
+    rng = np.random.default_rng(3)
+    for x in np.concatenate([10.0 ** rng.uniform(-300, 300, 2000), [1e-4, 1.0, 0.5, 2.0, 5e-324]]):
+        assert abs(L.orc_ctrl_log(x) - math.log(x)) <= 2e-10 * max(1.0, abs(math.log(x)))
+    for x in rng.uniform(-1.6, 2.5, 2000):
+        # degree-8 Taylor remainder on |r| <= ln2/2: e^(ln2/2) (ln2/2)^9 / 9! = 3.0e-10
+        assert abs(L.orc_ctrl_exp(x) / math.exp(x) - 1.0) < 3.0e-10
+    assert L.orc_ctrl_exp(-50.0) == L.orc_ctrl_exp(-1.6)
+    assert L.orc_ctrl_exp(1e300) == L.orc_ctrl_exp(2.5)
+    assert L.orc_ctrl_log(0.0) == -math.inf
+
+
 def test_loop_count_matches_float_loop():
     # cuda/lyapunov.dp.cpp iterates `for (double t = t0; t < T; t += h)`
     def brute(t0, t1, h):

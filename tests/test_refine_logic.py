@@ -35,8 +35,8 @@ def test_calibration_raises_kappa_until_answers_agree():
     mu1[wrong] = -0.05
     lam = np.stack([lam1, np.zeros(n), np.full(n, -1.0)], 1)
     st = np.zeros(n, dtype=np.uint32)
-    kappa, agree, answered = R.calibrate_kappa(pred_rows(mu1, std, np.zeros(n)), lam, st,
-                                               target=0.9995, **THR)
+    kappa, escapes, agree, answered = R.calibrate_kappa(pred_rows(mu1, std, np.zeros(n)), lam, st,
+                                                        target=0.9995, **THR)
     assert kappa == 1.5 and agree == 1.0  # 1.5*std <= 0.01 <=> std <= 0.00667 (no wrong one)
     assert answered == int(np.sum(std <= 0.01 / 1.5))
 
@@ -46,8 +46,22 @@ def test_calibration_gives_up_when_nothing_agrees():
     lam = np.tile([0.1, 0.0, -1.0], (n, 1))
     st = np.zeros(n, dtype=np.uint32)
     p = pred_rows(np.full(n, -0.1), np.zeros(n), np.zeros(n))  # confidently wrong everywhere
-    kappa, agree, answered = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    kappa, escapes, agree, answered = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
     assert kappa == float("inf") and answered == 0
+
+
+def test_calibration_drops_wrong_escape_answers():
+    """Escape answers are not scaled by kappa: when confidently-escaping
+    predictions are wrong, the rule answers without them."""
+    rng = np.random.default_rng(1)
+    n = 10000
+    lam = np.tile([0.1, 0.0, -1.0], (n, 1))
+    st = np.zeros(n, dtype=np.uint32)
+    pbad = np.zeros(n)
+    pbad[:50] = 0.99  # predicted escaping, but they are chaotic and bounded
+    p = pred_rows(np.full(n, 0.1), rng.uniform(0, 0.005, n), pbad)
+    kappa, escapes, agree, answered = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    assert kappa == 1.0 and not escapes and agree == 1.0 and answered == n - 50
 
 
 def test_classify():
