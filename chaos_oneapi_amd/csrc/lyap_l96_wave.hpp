@@ -46,18 +46,18 @@ LYAP_D double group_max(double v) {
 
 LYAP_D bool wave_uniform(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
 
-// row rotation by DPP: lane i of each 16-lane row receives lane (i - R) mod 16
+// row rotation by DPP: lane i of each 16-lane row receives lane (i - R) mod 16.
+// Every lane has a valid source, so the move needs no "old" value (mov_dpp).
 template <int R>
 LYAP_D float dpp_ror(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x120 + R,
-                                                                 0xF, 0xF, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x120 + R, 0xF, 0xF,
+                                                              true));
 }
 template <int R>
 LYAP_D double dpp_ror(double v) {
     const uint64_t b = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, 0x120 + R, 0xF, 0xF, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), 0x120 + R, 0xF, 0xF,
-                                                              false);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, 0x120 + R, 0xF, 0xF, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), 0x120 + R, 0xF, 0xF, true);
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
@@ -328,42 +328,57 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 }
             }
             if (accept) {
-                // tangent step of the lane's E vectors, J at every stage state
+                // tangent step of the lane's E vectors, J at every stage state;
+                // NV vectors advance through the stages together so that one
+                // vector's lane moves fill the others' wait states
                 const T hT = (T)h_try;
                 const RowB<TB, T, S> rbt;
+                constexpr int NV = E % 2 == 0 ? 2 : 1;
 #pragma unroll
-                for (int m = 0; m < E; ++m) {
-                    const T v0 = (T)V[m];
-                    T kv[S];
+                for (int m0 = 0; m0 < E; m0 += NV) {
+                    T v0[NV], kv[NV][S];
+#pragma unroll
+                    for (int q2 = 0; q2 < NV; ++q2) v0[q2] = (T)V[m0 + q2];
 #pragma unroll
                     for (int s = 0; s < S; ++s) {
                         if (!TB::for_solution(s)) {
-                            kv[s] = T(0);
+#pragma unroll
+                            for (int q2 = 0; q2 < NV; ++q2) kv[q2][s] = T(0);
                             continue;
                         }
                         const RowA<TB, T, S> row(s);
-                        T acc = T(0);
-                        bool any = false;
+                        T varg[NV];
+#pragma unroll
+                        for (int q2 = 0; q2 < NV; ++q2) {
+                            T acc = T(0);
+                            bool any = false;
+#pragma unroll
+                            for (int j = 0; j < S; ++j) {
+                                if (j >= s || TB::a(s, j) == 0.0) continue;
+                                acc = any ? fma(row.c[j], kv[q2][j], acc) : row.c[j] * kv[q2][j];
+                                any = true;
+                            }
+                            varg[q2] = any ? fma(hT, acc, v0[q2]) : v0[q2];
+                        }
+#pragma unroll
+                        for (int q2 = 0; q2 < NV; ++q2) {
+                            const T vp1 = ring.p1(varg[q2]), vm1 = ring.m1(varg[q2]), vm2 = ring.m2(varg[q2]);
+                            kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg[q2]));
+                        }
+                    }
+#pragma unroll
+                    for (int q2 = 0; q2 < NV; ++q2) {
+                        T accv = T(0);
+                        bool anyv = false;
 #pragma unroll
                         for (int j = 0; j < S; ++j) {
-                            if (j >= s || TB::a(s, j) == 0.0) continue;
-                            acc = any ? fma(row.c[j], kv[j], acc) : row.c[j] * kv[j];
-                            any = true;
+                            if (TB::b(j) == 0.0) continue;
+                            accv = anyv ? fma(rbt.c[j], kv[q2][j], accv) : rbt.c[j] * kv[q2][j];
+                            anyv = true;
                         }
-                        const T varg = any ? fma(hT, acc, v0) : v0;
-                        const T vp1 = ring.p1(varg), vm1 = ring.m1(varg), vm2 = ring.m2(varg);
-                        kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
+                        if constexpr (MIXED) V[m0 + q2] = V[m0 + q2] + (double)(hT * accv);
+                        else V[m0 + q2] = fma(h_try, accv, V[m0 + q2]);
                     }
-                    T accv = T(0);
-                    bool anyv = false;
-#pragma unroll
-                    for (int j = 0; j < S; ++j) {
-                        if (TB::b(j) == 0.0) continue;
-                        accv = anyv ? fma(rbt.c[j], kv[j], accv) : rbt.c[j] * kv[j];
-                        anyv = true;
-                    }
-                    if constexpr (MIXED) V[m] = V[m] + (double)(hT * accv);
-                    else V[m] = fma(h_try, accv, V[m]);
                 }
                 x = xn;
                 if (INTEG == LYAP_INT_DOPRI5 && ADAPT) k1 = k[S - 1];
