@@ -38,6 +38,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -213,8 +214,10 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
         "rerun_fraction": st["rerun_fraction"], "train_points": st["train_points"],
         "verify_points": st["verify"], "calibration_points": st["calib"],
-        "std_multiplier_kappa": st["kappa"], "escape_answers": st["escape_answers"],
-        "calibration_agreement": st["calib_agreement"],
+        "std_multiplier_kappa": st["kappa"] if math.isfinite(st["kappa"]) else None,
+        "escape_answers": st["escape_answers"], "calibration_agreement": st["calib_agreement"],
+        "calibration_table": [[k if math.isfinite(k) else None, e, a, w]
+                              for k, e, a, w in st["calib_table"]],
         "max_dlambda1_chosen_vs_fp64_verify": st.get("max_dlambda1_chosen_vs_fp64"),
         "class_agreement_verify": st.get("class_agreement"),
         "surrogate_answered_verify": st.get("surrogate_answered_verify"),

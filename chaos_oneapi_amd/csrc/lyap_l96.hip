@@ -8,8 +8,9 @@
 //   - neighbour couplings x_{i+1}, x_{i-1}, x_{i-2} (and of each vector) are
 //     moves inside the N-lane group (DPP row rotations for N = 16, lane
 //     shuffles otherwise);
-//   - the error norm and the Jacobian-norm step cap are xor-butterfly
-//     reductions over the group (offsets N/2 .. 1), identical in every lane;
+//   - the error norm and the Jacobian-norm step cap are butterfly reductions
+//     over the group (Ring::sum / Ring::max: the xor butterfly with offsets
+//     N/2 .. 1, by DPP rotations for N = 16), identical in every lane;
 //   - renormalisation stages the N x N deviation matrix in LDS, transposed so
 //     lane r owns vector r, and runs right-looking MGS with one barrier per
 //     vector (double-buffered q_j).
@@ -99,7 +100,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                 {
                     const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
                     const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
-                    const double jn = group_max<N>(row);
+                    const double jn = ring.max(row);
                     if (h_try * jn > 1.0) h_try = 1.0 / jn;
                 }
                 if (t_next - t <= 1.0625 * h_try) {
@@ -166,7 +167,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                     const double ax = fabs(x), an = fabs(xn);
                     const double sk = fma(rtol, ax > an ? ax : an, atol);
                     const double q = ev / sk;
-                    const double err = sqrt(group_sum<N>(q * q) / (double)N);
+                    const double err = sqrt(ring.sum(q * q) / (double)N);
                     if (wave_uniform(!(err < kInf))) {
                         st = LYAP_PT_NONFINITE;
                         break;

@@ -10,8 +10,8 @@
 // P = 64/N: no replication).
 //   - neighbour couplings are moves inside the N-lane group (Ring: DPP row
 //     rotations for N = 8 and 16, lane shuffles for N = 32);
-//   - the error norm and the Jacobian-norm cap are xor butterflies over the
-//     group, bit-identical in every lane of the group;
+//   - the error norm and the Jacobian-norm cap are butterflies over the group
+//     (Ring::sum / Ring::max), bit-identical in every lane of the group;
 //   - renormalisation stages the slot's N x N deviation matrix in the wave's
 //     LDS region, transposed so lane u < N owns vector u, and runs the same
 //     right-looking MGS as the block kernel (sequential dot products) on the
@@ -26,23 +26,6 @@
 #include "lyap_lane.hpp"
 
 namespace lyap {
-
-template <int N>
-LYAP_D double group_sum(double v) {
-#pragma unroll
-    for (int o = N / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o);
-    return v;
-}
-
-template <int N>
-LYAP_D double group_max(double v) {
-#pragma unroll
-    for (int o = N / 2; o >= 1; o >>= 1) {
-        const double w = __shfl_xor(v, o);
-        v = v > w ? v : w;
-    }
-    return v;
-}
 
 LYAP_D bool wave_uniform(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
 
@@ -95,6 +78,38 @@ struct Ring {
         if constexpr (kDpp) return dpp_ror<2 * CS>(v);
         else return __shfl(v, src_m2);
     }
+    // Butterfly sum / max over the group, identical in every lane and equal
+    // bit for bit to the xor butterfly with component offsets N/2, ..., 1:
+    // after the level with offset o the partial results repeat with period o
+    // (in components), so the value o components away by rotation is the xor
+    // partner's value, and a + b == b + a.
+    LYAP_D double sum(double v) const {
+        if constexpr (kDpp) {
+            if constexpr (N >= 16) v = v + dpp_ror<8 * CS>(v);
+            v = v + dpp_ror<4 * CS>(v);
+            v = v + dpp_ror<2 * CS>(v);
+            v = v + dpp_ror<1 * CS>(v);
+            return v;
+        } else {
+#pragma unroll
+            for (int o = N / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o * CS);
+            return v;
+        }
+    }
+    LYAP_D double max(double v) const {
+        auto mx = [](double a, double b) { return a > b ? a : b; };
+        if constexpr (kDpp) {
+            if constexpr (N >= 16) v = mx(v, dpp_ror<8 * CS>(v));
+            v = mx(v, dpp_ror<4 * CS>(v));
+            v = mx(v, dpp_ror<2 * CS>(v));
+            v = mx(v, dpp_ror<1 * CS>(v));
+            return v;
+        } else {
+#pragma unroll
+            for (int o = N / 2; o >= 1; o >>= 1) v = mx(v, __shfl_xor(v, o * CS));
+            return v;
+        }
+    }
 };
 
 // Lanes of the wave-level drain: lane -> (group q, component c).  N >= 16:
@@ -113,25 +128,6 @@ struct WaveMap {
         else return 0x5555ULL << (16 * (q / CS) + q % CS);
     }
 };
-
-// butterfly reductions over a group whose components sit CS lanes apart (the
-// pairings, hence the order of the additions, are those of group_sum<N>)
-template <int N, int CS>
-LYAP_D double group_sum_s(double v) {
-#pragma unroll
-    for (int o = N / 2; o >= 1; o >>= 1) v = v + __shfl_xor(v, o * CS);
-    return v;
-}
-
-template <int N, int CS>
-LYAP_D double group_max_s(double v) {
-#pragma unroll
-    for (int o = N / 2; o >= 1; o >>= 1) {
-        const double w = __shfl_xor(v, o * CS);
-        v = v > w ? v : w;
-    }
-    return v;
-}
 
 // orders this wave's LDS writes before its later LDS reads (cross-lane)
 LYAP_D void wave_lds_sync() {
@@ -245,7 +241,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 t_next = (double)(mr + 1) * t_renorm;
                 const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
                 const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
-                const double jn = group_max_s<N, CS>(row);
+                const double jn = ring.max(row);
                 if (h_try * jn > 1.0) h_try = 1.0 / jn;
                 if (slot_uniform(t_next - t <= 1.0625 * h_try)) {
                     h_try = t_next - t;
@@ -313,7 +309,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 const double ax = fabs(x), an = fabs(xn);
                 const double sk = fma(rtol, ax > an ? ax : an, atol);
                 const double qe = ev / sk;
-                const double err = sqrt(group_sum_s<N, CS>(qe * qe) / (double)N);
+                const double err = sqrt(ring.sum(qe * qe) / (double)N);
                 if (slot_uniform(!(err < kInf))) {
                     st = LYAP_PT_NONFINITE;
                     finished = true;

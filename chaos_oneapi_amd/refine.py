@@ -10,9 +10,10 @@ Pipeline, all on the device through liblyap.so:
      lambda1 above std_thr), near the chaos boundary (|mu1| < lam_thr), or
      where the escape probability is not decided, plus on seeded random
      calibration and verification subsamples (calib_frac, verify_frac of the
-     grid, disjoint); kappa >= 1 is the smallest multiplier at which the
-     surrogate's answers on the calibration subsample agree with the method's
-     class on calib_target of them (the predicted std alone is not calibrated);
+     grid, disjoint); the answer rule (std multiplier kappa >= 1, escape
+     answers on or off) is the one that answers the most calibration points
+     while the map's class agreement there stays >= calib_target (the
+     predicted std alone is not calibrated);
   5. the plain FP64 method judges the verification subsample.
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch, and the FP64 judge runs on a side stream
@@ -74,32 +75,34 @@ def answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes
 
 
 def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target):
-    """The answer rule with the most answered calibration points among those
-    whose answers agree with the computed class on at least `target` of
-    them: kappa in KAPPAS, with or without the escape answers (which kappa
-    does not scale).  Returns (kappa, escapes, agreement, answered); kappa inf:
-    the surrogate answers nothing.  pred [n,5] numpy, lam/status the method's
-    values at the same points."""
+    """The answer rule that answers the most calibration points while keeping
+    the final map's class agreement on the calibration subsample at or above
+    `target` (the method's own points agree by construction, so this bounds
+    the wrong answers: wrong <= (1 - target) * n).  Rules: kappa in KAPPAS,
+    with or without the escape answers (which kappa does not scale).
+    Returns (kappa, escapes, agreement, answered, table); kappa inf: the
+    surrogate answers nothing.  pred [n,5] numpy, lam/status the method's
+    values at the same points; table lists (kappa, escapes, answered, wrong)."""
     import numpy as np
 
+    n = len(pred)
     computed = classify(lam[:, 0], status)
     sur_status = np.where(pred[:, 4] >= 0.5, 2, 0)
     sur = classify(pred[:, 0], sur_status)
     finite = np.isfinite(pred[:, 0])
+    budget = (1.0 - target) * n
     best = (float("inf"), False, 1.0, 0)
+    table = []
     for escapes in (True, False):
         for kappa in KAPPAS:
             a = answerable(pred[:, 0], pred[:, 3], pred[:, 4], kappa, std_thr, lam_thr, bad_lo,
                            bad_hi, escapes) & finite
             n_a = int(a.sum())
-            if n_a == 0:
-                break
-            agree = float(np.mean(sur[a] == computed[a]))
-            if agree >= target:
-                if n_a > best[3]:
-                    best = (kappa, escapes, agree, n_a)
-                break  # larger kappa only answers fewer points
-    return best
+            wrong = int(np.sum(sur[a] != computed[a]))
+            table.append((kappa, escapes, n_a, wrong))
+            if n_a > 0 and wrong <= budget and n_a > best[3]:
+                best = (kappa, escapes, 1.0 - wrong / n, n_a)
+    return best + (table,)
 
 
 def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4096,
@@ -204,7 +207,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     #    calibration subsample agree with the method (the verification
     #    subsample stays out of it), then the rerun set: every point the
     #    surrogate does not answer and that step 1 did not compute
-    kappa, escapes, cal_agree, cal_answered = calibrate_kappa(
+    kappa, escapes, cal_agree, cal_answered, cal_table = calibrate_kappa(
         pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
         bad_lo, bad_hi, calib_target)
     tm["calibrate"] = time.perf_counter() - t
@@ -248,7 +251,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     stats = dict(points=n, train_points=n_t, rerun=n_rerun,
                  flagged=int(flagged.sum().item()), verify=n_ver, calib=n_cal,
                  kappa=kappa, escape_answers=bool(escapes), calib_agreement=cal_agree,
-                 calib_answered=cal_answered,
+                 calib_answered=cal_answered, calib_table=cal_table,
                  train_on_lattice=bool(lattice),
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),

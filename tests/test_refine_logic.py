@@ -35,10 +35,13 @@ def test_calibration_raises_kappa_until_answers_agree():
     mu1[wrong] = -0.05
     lam = np.stack([lam1, np.zeros(n), np.full(n, -1.0)], 1)
     st = np.zeros(n, dtype=np.uint32)
-    kappa, escapes, agree, answered = R.calibrate_kappa(pred_rows(mu1, std, np.zeros(n)), lam, st,
-                                                        target=0.9995, **THR)
-    assert kappa == 1.5 and agree == 1.0  # 1.5*std <= 0.01 <=> std <= 0.00667 (no wrong one)
+    kappa, escapes, agree, answered, table = R.calibrate_kappa(
+        pred_rows(mu1, std, np.zeros(n)), lam, st, target=0.9995, **THR)
+    # kappa 1 answers the wrong points (std in (0.008, 0.01]): ~20 % of them, far
+    # over the budget of 10 wrong answers; 1.5*std <= 0.01 <=> std <= 0.00667 has none
+    assert kappa == 1.5 and agree == 1.0
     assert answered == int(np.sum(std <= 0.01 / 1.5))
+    assert len(table) == 2 * len(R.KAPPAS)
 
 
 def test_calibration_gives_up_when_nothing_agrees():
@@ -46,7 +49,7 @@ def test_calibration_gives_up_when_nothing_agrees():
     lam = np.tile([0.1, 0.0, -1.0], (n, 1))
     st = np.zeros(n, dtype=np.uint32)
     p = pred_rows(np.full(n, -0.1), np.zeros(n), np.zeros(n))  # confidently wrong everywhere
-    kappa, escapes, agree, answered = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
     assert kappa == float("inf") and answered == 0
 
 
@@ -60,8 +63,21 @@ def test_calibration_drops_wrong_escape_answers():
     pbad = np.zeros(n)
     pbad[:50] = 0.99  # predicted escaping, but they are chaotic and bounded
     p = pred_rows(np.full(n, 0.1), rng.uniform(0, 0.005, n), pbad)
-    kappa, escapes, agree, answered = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
     assert kappa == 1.0 and not escapes and agree == 1.0 and answered == n - 50
+
+
+def test_calibration_spends_the_wrong_answer_budget():
+    """A few wrong answers are acceptable when the map's agreement stays at the
+    target: 4 wrong of 10000 calibration points is within 0.05 %."""
+    n = 10000
+    lam = np.tile([0.1, 0.0, -1.0], (n, 1))
+    st = np.zeros(n, dtype=np.uint32)
+    mu1 = np.full(n, 0.1)
+    mu1[:4] = -0.1  # confidently wrong
+    p = pred_rows(mu1, np.full(n, 0.001), np.zeros(n))
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    assert kappa == 1.0 and answered == n and abs(agree - 0.9996) < 1e-12
 
 
 def test_classify():
