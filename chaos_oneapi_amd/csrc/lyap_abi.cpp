@@ -117,6 +117,7 @@ struct lyap_ctx {
     int lane_max_blocks = 0;
     int l96_max_blocks = 0;
     int lane_gather = 0;
+    int item_order = 0;  // 0: items spread over the grid; 1: in list order
     unsigned long long* counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -293,7 +294,7 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
     hipStream_t s = (hipStream_t)stream;
     lyap::SweepIO io{first,    count,    d_idx_list, d_params,   d_inits,
                      d_lambda, d_status, d_n_acc,    d_n_rej,    c->counter,
-                     spread_stride(count), c->lane_gather, 0};
+                     c->item_order ? 1 : spread_stride(count), c->lane_gather, 0};
     if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     if (lane_model(fd.model)) {
         e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks);
@@ -360,6 +361,11 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
     if (k == "lane_gather") {
         if (value < 0 || value > 64) return fail(LYAP_ERR_INVALID, "lane_gather must be in [0, 64]");
         c->lane_gather = (int)value;
+        return LYAP_SUCCESS;
+    }
+    if (k == "item_order") {
+        if (value != 0 && value != 1) return fail(LYAP_ERR_INVALID, "item_order must be 0 or 1");
+        c->item_order = (int)value;
         return LYAP_SUCCESS;
     }
     if (k == "l96_max_blocks") {

@@ -91,10 +91,11 @@ LYAP_D void batch_body(const BatchClass* __restrict__ cls, int n_cls, double* ld
         const lyap_sweep_desc d = hot_fields(bc->d);
         const SweepIO io = uniform_io(bc->io);
         auto getd = [bc] { return &bc->d; };
+        auto getio = [bc] { return &bc->io; };
         switch ((int)uni((uint32_t)bc->kind)) {
-            case BATCH_SHIMIZU: lane_drain<Shimizu, INTEG, ADAPT, MIXED>(d, getd, io); break;
-            case BATCH_LORENZ: lane_drain<Lorenz, INTEG, ADAPT, MIXED>(d, getd, io); break;
-            case BATCH_ROSSLER: lane_drain<Rossler, INTEG, ADAPT, MIXED>(d, getd, io); break;
+            case BATCH_SHIMIZU: lane_drain<Shimizu, INTEG, ADAPT, MIXED>(d, getd, io, getio); break;
+            case BATCH_LORENZ: lane_drain<Lorenz, INTEG, ADAPT, MIXED>(d, getd, io, getio); break;
+            case BATCH_ROSSLER: lane_drain<Rossler, INTEG, ADAPT, MIXED>(d, getd, io, getio); break;
             case BATCH_L96_8: l96_wave_drain<8, L.p8, INTEG, ADAPT, MIXED>(d, getd, io, lds); break;
             case BATCH_L96_16: l96_wave_drain<16, L.p16, INTEG, ADAPT, MIXED>(d, getd, io, lds); break;
             case BATCH_L96_32: l96_wave_drain<32, L.p32, INTEG, ADAPT, MIXED>(d, getd, io, lds); break;

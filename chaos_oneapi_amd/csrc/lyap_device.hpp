@@ -32,10 +32,17 @@ LYAP_HD double double_of(uint64_t u) { return __builtin_bit_cast(double, u); }
 // take no 64-bit literal, so FP64 constants live in SGPR pairs; left alone,
 // the compiler hoists all of them out of the step loop, overflows the SGPR
 // budget and spills them into VGPR lanes (a v_readlane per use).  Passing the
-// value through an empty asm keeps it a fresh s_mov pair at its use site.
+// value through an empty asm keeps it a fresh s_mov pair at its use site.  The
+// two halves go through separate 32-bit asm operands: with one 64-bit operand
+// the register coalescer merges halves of different constants and spills the
+// pairs anyway (about 5x more v_readlane/v_writelane in the lane kernels).
 LYAP_HD double sconst(double v) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    __asm__ volatile("" : "+s"(v));
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+    __asm__ volatile("" : "+s"(lo));
+    __asm__ volatile("" : "+s"(hi));
+    v = __builtin_bit_cast(double, ((uint64_t)hi << 32) | lo);
 #endif
     return v;
 }
@@ -614,8 +621,22 @@ struct SweepIO {
 // position of work item `item`: a multiplicative permutation of [0, count)
 // (stride coprime to count) that spreads consecutive items over the grid, so
 // the last items taken are not one expensive region of the parameter plane
-LYAP_D int64_t item_position(const SweepIO& io, int64_t item) {
-    return (int64_t)(((unsigned __int128)item * (unsigned __int128)io.stride) % (unsigned __int128)io.count);
+LYAP_D int64_t item_position(int64_t item, int64_t stride, int64_t count) {
+    return (int64_t)(((unsigned __int128)item * (unsigned __int128)stride) % (unsigned __int128)count);
+}
+LYAP_D int64_t item_position(const SweepIO& io, int64_t item) { return item_position(item, io.stride, io.count); }
+
+// The IO record of a kernel whose arguments are (lyap_sweep_desc, SweepIO, ...):
+// it follows the descriptor in the kernarg segment; read through an opaque
+// pointer like kernarg_desc().
+using KernargIO = const __attribute__((address_space(4))) SweepIO;
+static_assert(sizeof(lyap_sweep_desc) % alignof(SweepIO) == 0, "SweepIO follows the descriptor unpadded");
+LYAP_D KernargIO* kernarg_io() {
+    const __attribute__((address_space(4))) char* b =
+        (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+    KernargIO* p = (KernargIO*)(b + sizeof(lyap_sweep_desc));
+    __asm__ volatile("" : "+s"(p));
+    return p;
 }
 
 }  // namespace lyap

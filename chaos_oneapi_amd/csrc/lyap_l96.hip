@@ -262,7 +262,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
 #pragma unroll
                     for (int c = 1; c < N; ++c) s2 = fma(vr[c], vr[c], s2);
                     const double nrm = sqrt(s2);
-                    ln = det_log(nrm);
+                    ln = nrm;  // its log is taken after the loop, by every row lane at once
                     const double inv = 1.0 / nrm;
 #pragma unroll
                     for (int c = 0; c < N; ++c) {
@@ -287,6 +287,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < E; ++m) V[m] = sV[(gid + NG * m) * N + i];
+            if (tid < K) ln = det_log(ln);
             ++mr;
             if (mr > n_tr && tid < K) kahan_add(sum, comp, ln);
             if (mr == m_total) break;

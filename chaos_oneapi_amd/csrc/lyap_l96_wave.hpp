@@ -409,7 +409,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
 #pragma unroll 8
                             for (int c = 1; c < N; ++c) s2 = fma(row[c], row[c], s2);
                             const double nrm = sqrt(s2);
-                            ln = det_log(nrm);
+                            ln = nrm;  // its log is taken after the loop, by every row lane at once
                             const double inv = 1.0 / nrm;
 #pragma unroll 8
                             for (int c = 0; c < N; ++c) row[c] = row[c] * inv;
@@ -427,6 +427,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     wave_lds_sync();
 #pragma unroll
                     for (int m = 0; m < E; ++m) V[m] = sV[(vg + G * m) * RS + i];
+                    if (u < K) ln = det_log(ln);
                     ++mr;
                     if (mr > n_tr && u < K) kahan_add(sum, comp, ln);
                     if (mr == m_total) finished = true;

@@ -166,10 +166,12 @@ LYAP_D int64_t grab_items(bool want, unsigned long long* counter) {
 
 // Persistent per-lane loop over the points of one sweep: lanes that finish a
 // point take the next item from io.counter until the items are exhausted and
-// every lane of the wave is done.  getd() returns the descriptor to read the
-// per-point set-up fields from (an opaque pointer, so they are not kept live).
-template <class M, int INTEG, bool ADAPT, bool MIXED, class GetDesc>
-LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io) {
+// every lane of the wave is done.  getd() and getio() return the descriptor and
+// the IO record to read the per-point set-up and output fields from (opaque
+// pointers, so those fields are not kept live in SGPRs across the step loop);
+// io supplies only the fields every iteration reads (counter, count, gather).
+template <class M, int INTEG, bool ADAPT, bool MIXED, class GetDesc, class GetIO>
+LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io, GetIO getio) {
     constexpr int N = M::N, P = M::P, K = M::N, S = Tab<INTEG>::S;
     using TX = typename std::conditional<MIXED, float, double>::type;
     // scalars the step loop reads; everything else of the descriptor is
@@ -180,10 +182,12 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     const int64_t max_steps = d.max_steps, spr = d.steps_per_renorm;
     const double rtol = d.rtol, atol = d.atol, h_min = d.h_min, h_max = d.h_max;
     const double t_renorm = d.t_renorm, esc = d.escape_radius, h_fixed = d.h;
-    const double t_avg = ADAPT ? (double)d.n_renorm_average * d.t_renorm
-                               : (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
-    const int n_exps = d.n_exps;
     const double log_floor = ctrl_log(1e-4);
+    // io gives the fields read every iteration (work counter, count, gather);
+    // the others are re-read through getio() when a lane takes or finishes a
+    // point
+    unsigned long long* const counter = io.counter;
+    const int64_t count = io.count;
 
     double x[N], V[K][N], sum[K], comp[K], p[P], k1[N];
     float pf[P];
@@ -200,31 +204,39 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     const int gather = io.gather;
 
     auto write_out = [&](uint32_t st) {
+        const auto dp = getd();
+        const auto iop = getio();
+        const double t_avg = ADAPT ? (double)dp->n_renorm_average * dp->t_renorm
+                                   : (double)dp->n_renorm_average * (double)dp->steps_per_renorm * dp->h;
+        const int n_exps = dp->n_exps;
+        double* const lam = iop->lambda;
         for (int r = 0; r < n_exps; ++r) {
             double s = 0.0;
 #pragma unroll
             for (int q = 0; q < K; ++q)
                 if (q == r) s = sum[q];
-            io.lambda[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
+            lam[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
         }
-        if (io.status) io.status[item] = st;
-        if (io.n_acc) io.n_acc[item] = nacc;
-        if (io.n_rej) io.n_rej[item] = nrej;
+        if (iop->status) iop->status[item] = st;
+        if (iop->n_acc) iop->n_acc[item] = nacc;
+        if (iop->n_rej) iop->n_rej[item] = nrej;
     };
 
     for (;;) {
-        const int64_t got = grab_items(!active && !exhausted, io.counter);
+        const int64_t got = grab_items(!active && !exhausted, counter);
         if (!active && !exhausted) {
-            if (got < io.count) {
+            if (got < count) {
                 const auto dp = getd();
-                item = item_position(io, got);
-                const int64_t idx = io.idx_list ? io.idx_list[item] : io.first + item;
-                point_params<P>(*dp, idx, io.params, p);
+                const auto iop = getio();
+                item = item_position(got, iop->stride, count);
+                const int64_t idx = iop->idx_list ? iop->idx_list[item] : iop->first + item;
+                point_params<P>(*dp, idx, iop->params, p);
 #pragma unroll
                 for (int j = 0; j < P; ++j) pf[j] = (float)p[j];
+                const double* const inits = iop->inits;
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                    x[i] = io.inits ? io.inits[idx * N + i] : init_component(*dp, idx, i, p);
+                    x[i] = inits ? inits[idx * N + i] : init_component(*dp, idx, i, p);
 #pragma unroll
                 for (int r = 0; r < K; ++r) {
                     sum[r] = 0.0;

@@ -32,6 +32,7 @@ constexpr int kOut = 5;       // mu1, mu2, mu3, log-variance, escape logit
 constexpr int kOutPad = 16;   // head rows padded to one MFMA tile
 constexpr int kLayers = 4;
 constexpr int kMaxSplits = 16;  // split-K of the weight-gradient products
+constexpr int kColsumChunks = 32;  // row chunks of the bias-gradient sums
 
 LYAP_HD uint16_t f2bf(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -229,24 +230,26 @@ __global__ void k_head_loss(const float* __restrict__ Z, const double* __restric
     if (loss && b < n_valid) atomicAdd(loss, l * inv_n);
 }
 
-// db[j] = sum_b dZ[b][j]: one block of 64 columns x 16 row groups; thread
-// (c, g) sums rows g, g+16, ... in order (coalesced across c), then the 16
-// partial sums are added in a fixed tree, so the result is deterministic
-__global__ __launch_bounds__(1024) void k_colsum(const float* __restrict__ dZ, int n, int H,
-                                                 float* __restrict__ db) {
-    __shared__ float part[16][64];
+// Partial column sums of dZ [n][H] over row chunks: block (x, r) covers
+// columns 64x .. 64x+63 and rows [r*rc, min(n, (r+1)*rc)); thread (c, g) sums
+// rows g, g+4, ... of the chunk in order (coalesced across c), the four row
+// groups are added in a fixed tree and the chunk's sum goes to part[r][j].
+// k_sum_splits then adds the chunks in order, so db is deterministic.
+__global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ dZ, int n, int H, int rc,
+                                                     float* __restrict__ part) {
+    __shared__ float red[4][64];
     const int c = (int)(threadIdx.x & 63), g = (int)(threadIdx.x >> 6);
     const int j = (int)blockIdx.x * 64 + c;
+    const int r = (int)blockIdx.y;
+    const int lo = r * rc, hi = min(n, lo + rc);
     float s = 0.f;
     if (j < H)
-        for (int b = g; b < n; b += 16) s += dZ[(size_t)b * H + j];
-    part[g][c] = s;
+        for (int b = lo + g; b < hi; b += 4) s += dZ[(size_t)b * H + j];
+    red[g][c] = s;
     __syncthreads();
-    for (int w = 8; w >= 1; w >>= 1) {
-        if (g < w) part[g][c] += part[g + w][c];
-        __syncthreads();
-    }
-    if (g == 0 && j < H) db[j] = part[0][c];
+    if (g < 2) red[g][c] += red[g + 2][c];
+    __syncthreads();
+    if (g == 0 && j < H) part[(size_t)r * H + j] = red[0][c] + red[1][c];
 }
 
 // dZ (bf16, row-major) -> fp32 for the bias reduction of hidden layers
@@ -337,7 +340,7 @@ struct lyap_mlp {
     float* dh = nullptr;
     float* dzf = nullptr;
     float* loss = nullptr;
-    float* split_ws = nullptr;  // kMaxSplits x (largest weight matrix) partial products
+    float* split_ws = nullptr;  // split-K partial products / bias-gradient chunk sums
     int64_t step = 0;
     std::vector<void*> allocs;
 };
@@ -405,9 +408,12 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->loss, 4);
-    size_t max_w = 0;
-    for (int l = 0; l < kLayers; ++l) max_w = std::max(max_w, (size_t)dims[l + 1] * dims[l]);
-    rc |= mlp_alloc(m, (void**)&m->split_ws, (size_t)kMaxSplits * max_w * 4);
+    size_t max_w = 0, max_d = 0;
+    for (int l = 0; l < kLayers; ++l) {
+        max_w = std::max(max_w, (size_t)dims[l + 1] * dims[l]);
+        max_d = std::max(max_d, (size_t)dims[l + 1]);
+    }
+    rc |= mlp_alloc(m, (void**)&m->split_ws, std::max((size_t)kMaxSplits * max_w, (size_t)kColsumChunks * max_d) * 4);
     if (rc != LYAP_SUCCESS) {
         lyap_mlp_destroy(m);
         return LYAP_ERR_HIP;
@@ -498,6 +504,21 @@ static void gemm_wgrad(lyap_mlp* m, hipStream_t s, const uint16_t* A, const uint
                        mn, C);
 }
 
+// bias gradient db[j] = sum_b dZ[b][j] (after the weight-gradient product of
+// the same layer has consumed split_ws): up to kColsumChunks row chunks summed
+// in parallel, then added in chunk order
+static void colsum(lyap_mlp* m, hipStream_t s, const float* dZ, int n, int H, float* db) {
+    int chunks = n / 128;
+    if (chunks > kColsumChunks) chunks = kColsumChunks;
+    if (chunks < 1) chunks = 1;
+    const int rc = ((n + chunks - 1) / chunks + 3) / 4 * 4;
+    chunks = (n + rc - 1) / rc;
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((H + 63) / 64), (unsigned)chunks), dim3(256), 0, s, dZ, n, H,
+                       rc, m->split_ws);
+    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, s, m->split_ws, chunks,
+                       (int64_t)H, db);
+}
+
 static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s) {
     const auto& c = m->cfg;
     const int* D = m->dims;
@@ -526,8 +547,7 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
     for (int l = kLayers - 1; l >= 0; --l) {
         // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T
         gemm_wgrad(m, s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad);
-        hipLaunchKernelGGL(k_colsum, dim3((D[l + 1] + 63) / 64), dim3(1024), 0, s, m->dzf, n_pad, D[l + 1],
-                           m->grad + m->off_b[l]);
+        colsum(m, s, m->dzf, n_pad, D[l + 1], m->grad + m->off_b[l]);
         if (l == 0) break;
         // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU
         gemm(s, m->dz, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);

@@ -25,7 +25,7 @@ namespace lyap {
 // ---------------------------------------------------------------------------
 template <class M, int INTEG, bool ADAPT, bool MIXED>
 __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const SweepIO io) {
-    lane_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io);
+    lane_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
 }
 
 // ---------------------------------------------------------------------------

@@ -15,6 +15,10 @@ Pipeline, all on the device through liblyap.so:
      while the map's class agreement there stays >= calib_target (the
      predicted std alone is not calibrated);
   5. the plain FP64 method judges the verification subsample.
+The rerun takes its points longest first: a point's expected cost is the
+step count of its nearest training-lattice point, and the kernel takes the
+sorted list in order (context option item_order), so the launch does not end
+waiting on a few long points.
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch, and the FP64 judge runs on a side stream
 (with a context of its own) concurrently with steps 1-3; the rerun of step 4
@@ -105,12 +109,12 @@ def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target)
     return best + (table,)
 
 
-def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4096,
+def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 4096,
            lr: float = 1e-3, seed: int = 7, verify_frac: float = 0.01, std_thr: float = 0.01,
            lam_thr: float = 0.02, bad_lo: float = 0.02, bad_hi: float = 0.98,
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
-           calib_target: float = 0.9995) -> RefineResult:
+           calib_target: float = 0.9995, longest_first: bool = True) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -157,7 +161,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
         ar = torch.arange(train_n, device=dev, dtype=torch.int64)
         train_idx = ((ar * s1)[:, None] * fd.n2 + (ar * s2)[None, :]).reshape(-1)
         first = torch.cat([train_idx, calib, verify]).contiguous()
-        lam_f, st_f, _, _ = ctx.run(fd, idx_list=first)
+        lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
         lam_t, st_t = lam_f[:n_t], st_f[:n_t]
         p_t = grid_params(fd, train_idx)
     else:
@@ -168,12 +172,13 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
         lam_t, st_t, _, _ = ctx.run(td)
         p_t = grid_params(td, torch.arange(n_t, device=dev))
         first = torch.cat([calib, verify]).contiguous()
-        lam_f, st_f, _, _ = ctx.run(fd, idx_list=first)
+        lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
         lam_f = torch.cat([lam_t, lam_f])
         st_f = torch.cat([st_t, st_f])
     lam_c, st_c = lam_f[n_t:n_t + n_cal], st_f[n_t:n_t + n_cal]
     lam_v, st_v = lam_f[n_t + n_cal:], st_f[n_t + n_cal:]
     t = mark_stream("train_map", t, main)
+    first_kernel_ms = ctx.kernel_ms()
 
     # 2. training (one shuffle per epoch; batches are contiguous slices)
     ok = st_t == 0
@@ -225,9 +230,24 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
     if lattice:
         tmask[train_idx] = True
     rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask & ~tmask).flatten().to(torch.int64).contiguous()
-    lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
+    ordered = longest_first and lattice and rerun_idx.numel() > 0
+    if ordered:
+        # longest first: each point's expected cost is the step count of its
+        # nearest lattice point, and the kernel takes the list in order, so the
+        # launch ends on short points instead of waiting for a long one
+        cost_t = (na_f[:n_t].to(torch.int64) + nr_f[:n_t].to(torch.int64))
+        j1 = torch.clamp(torch.round(torch.div(rerun_idx, fd.n2, rounding_mode="floor") / s1), 0, train_n - 1)
+        j2 = torch.clamp(torch.round(torch.remainder(rerun_idx, fd.n2) / s2), 0, train_n - 1)
+        cost = cost_t[(j1 * train_n + j2).to(torch.int64)]
+        rerun_idx = rerun_idx[torch.sort(cost, descending=True, stable=True).indices].contiguous()
+        ctx.set_option("item_order", 1)
+    try:
+        lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
+    finally:
+        if ordered:
+            ctx.set_option("item_order", 0)
     t = mark_stream("rerun", t, main)
-    rerun_kernel_ms = ctx.kernel_ms()
+    rerun_kernel_ms = ctx.kernel_ms() if rerun_idx.numel() else 0.0
     torch.cuda.synchronize(dev)
     judge_ctx.close()
 
@@ -255,6 +275,10 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 40, batch: int = 4
                  train_on_lattice=bool(lattice),
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),
+                 rerun_longest_first=bool(ordered),
+                 first_kernel_ms=first_kernel_ms, first_points=int(first.numel()),
+                 first_acc_steps=float(na_f.double().sum().item()),
+                 first_rej_steps=float(nr_f.double().sum().item()),
                  rerun_acc_steps=float(na_r.double().sum().item()),
                  rerun_rej_steps=float(nr_r.double().sum().item()))
     if judge:

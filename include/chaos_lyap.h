@@ -163,6 +163,11 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *   "lane_gather": 3-D kernels: lanes at a renormalisation wait until this many
  *       lanes of the wave wait, then renormalise together (0 or 1 = at once;
  *       results are the same bits either way)
+ *   "item_order": order in which the persistent kernels take the points of a
+ *       sweep: 0 (default) spread over the grid (item i is point i*s mod count,
+ *       s ~ count/phi coprime to count), 1 in order (point i, i.e. the order of
+ *       idx_list: a list sorted by decreasing expected cost schedules the
+ *       longest points first); results are the same bits either way
  */
 int lyap_ctx_set_option(lyap_ctx* ctx, const char* key, int64_t value);
 
