@@ -519,18 +519,19 @@ struct Ctrl {
 // h_new = h / fac; after a rejection h_new = h / min(5, err^expo / 0.9).  Here
 // in the equivalent form h_new = h * r, r = clamp(0.9 * exp(-(expo*lerr -
 // beta*log facOld)), 0.2, 10): one exponential and no division.
-// lerr = ctrl_log(err), log_floor = ctrl_log(1e-4).
+// ok: err <= 1; above_floor: err > 1e-4; lerr = log(err) (ctrl_log);
+// log_floor = ctrl_log(1e-4).
 template <int INTEG>
-LYAP_D bool ctrl_update(Ctrl& c, double err, double lerr, double log_floor, double h_try,
+LYAP_D bool ctrl_update(Ctrl& c, bool ok, bool above_floor, double lerr, double log_floor, double h_try,
                         double h_max, double& h_next) {
     constexpr double kSafe = 0.9, kBeta = 0.04;
-    if (err <= 1.0) {
+    if (ok) {
         const double e = fma(-kBeta, c.log_fac_old, Tab<INTEG>::expo * lerr);
         double r = kSafe * ctrl_exp(-e);
         r = r < 0.2 ? 0.2 : r;
         r = r > 10.0 ? 10.0 : r;
         double hn = h_try * r;
-        c.log_fac_old = err > 1e-4 ? lerr : log_floor;
+        c.log_fac_old = above_floor ? lerr : log_floor;
         if (fabs(hn) > h_max) hn = h_max;
         if (c.rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
         c.rejected = false;
@@ -586,6 +587,37 @@ LYAP_D void kahan_add(double& s, double& c, double v) {
     const double t = s + y;
     c = (t - s) - y;
     s = t;
+}
+
+// Log-norm accumulation of the sweep's method.  The norms of up to
+// kLogWindow consecutive renormalisations are multiplied into a product kept
+// as a mantissa in [1, 2) and an exact binary exponent; the log of each
+// window's product, log(mant) + ex*ln2, is Kahan-added to the sum.  One log
+// per window instead of one per renormalisation (the product rounds once per
+// factor, far below the log's own error).  A mantissa that is zero,
+// subnormal, infinite or NaN keeps its exponent field, so its log is what the
+// log of the norm itself would have been.
+constexpr int kLogWindow = 16;
+
+LYAP_HD double split_exponent(double m, int32_t& ex) {
+    const uint64_t b = bits_of(m);
+    const int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0 || e == 0x7ff) return m;
+    ex += e - 1023;
+    return double_of((b & 0x800fffffffffffffULL) | 0x3ff0000000000000ULL);
+}
+
+LYAP_HD double log_scaled(double mant, int32_t ex) {
+    const double dk = (double)ex;
+    return fma(dk, sconst(kLn2Hi), fma(dk, sconst(kLn2Lo), det_log(mant)));
+}
+
+// one vector's share: push a norm; flush adds the window's log to (s, c)
+LYAP_HD void lognorm_push(double& mant, int32_t& ex, double nrm) { mant = split_exponent(mant * nrm, ex); }
+LYAP_HD void lognorm_flush(double& mant, int32_t& ex, double& s, double& c) {
+    kahan_add(s, c, log_scaled(mant, ex));
+    mant = 1.0;
+    ex = 0;
 }
 
 // The descriptor as it sits in the kernarg segment (the first kernel argument,

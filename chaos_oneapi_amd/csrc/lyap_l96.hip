@@ -76,6 +76,9 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
 #pragma unroll
         for (int m = 0; m < E; ++m) V[m] = (gid + NG * m == i) ? 1.0 : 0.0;
         double sum = 0.0, comp = 0.0;
+        double lmant = 1.0;  // open log window of row tid's norms (tid < K)
+        int32_t lex = 0;
+        int lcnt = 0;
         double t = 0.0, h = dp->h;
         int64_t mr = 0, step_in = 0;
         uint32_t nacc = 0, nrej = 0;
@@ -173,8 +176,8 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                         break;
                     }
                     double h_next;
-                    accept = ctrl_update<INTEG>(ctrl, err, ctrl_log(err), log_floor, h_try, h_max,
-                                                h_next);
+                    accept = ctrl_update<INTEG>(ctrl, err <= 1.0, err > 1e-4, ctrl_log(err), log_floor,
+                                                h_try, h_max, h_next);
                     accept = wave_uniform(accept);
                     h = h_next;
                     if (!accept) {
@@ -251,7 +254,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
             for (int m = 0; m < E; ++m) sV[(gid + NG * m) * N + i] = V[m];
             __syncthreads();
             double vr[N];
-            double ln = 0.0;
+            double vn = 0.0;
             if (tid < K) {
 #pragma unroll
                 for (int c = 0; c < N; ++c) vr[c] = sV[tid * N + c];
@@ -262,7 +265,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
 #pragma unroll
                     for (int c = 1; c < N; ++c) s2 = fma(vr[c], vr[c], s2);
                     const double nrm = sqrt(s2);
-                    ln = nrm;  // its log is taken after the loop, by every row lane at once
+                    vn = nrm;  // accumulated after the loop, by every row lane at once
                     const double inv = 1.0 / nrm;
 #pragma unroll
                     for (int c = 0; c < N; ++c) {
@@ -287,9 +290,14 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < E; ++m) V[m] = sV[(gid + NG * m) * N + i];
-            if (tid < K) ln = det_log(ln);
             ++mr;
-            if (mr > n_tr && tid < K) kahan_add(sum, comp, ln);
+            if (mr > n_tr && tid < K) {
+                lognorm_push(lmant, lex, vn);
+                if (++lcnt == kLogWindow || mr == m_total) {
+                    lognorm_flush(lmant, lex, sum, comp);
+                    lcnt = 0;
+                }
+            }
             if (mr == m_total) break;
         }
 

@@ -184,6 +184,9 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
     const double log_floor = ctrl_log(1e-4);
 
     double x = 0.0, V[E], sum = 0.0, comp = 0.0, t = 0.0, h = 0.0, k1 = 0.0, F = 0.0, dd = 0.0;
+    double lmant = 1.0;  // open log window of row u's norms (lanes u < K)
+    int32_t lex = 0;
+    int lcnt = 0;
     T dT = T(0);
     int64_t mr = 0, step_in = 0, pos = 0;
     uint32_t nacc = 0, nrej = 0;
@@ -210,6 +213,9 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
 #pragma unroll
                 for (int m = 0; m < E; ++m) V[m] = (vg + G * m == i) ? 1.0 : 0.0;
                 sum = comp = 0.0;
+                lmant = 1.0;
+                lex = 0;
+                lcnt = 0;
                 t = 0.0;
                 h = dp->h;
                 mr = step_in = 0;
@@ -316,8 +322,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     accept = false;
                 } else {
                     double h_next;
-                    accept = ctrl_update<INTEG>(ctrl, err, ctrl_log(err), log_floor, h_try, h_max,
-                                                h_next);
+                    accept = ctrl_update<INTEG>(ctrl, err <= 1.0, err > 1e-4, ctrl_log(err), log_floor,
+                                                h_try, h_max, h_next);
                     accept = slot_uniform(accept);
                     h = h_next;
                     if (!accept) ++nrej;
@@ -402,14 +408,14 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     for (int m = 0; m < E; ++m) sV[(vg + G * m) * RS + i] = V[m];
                     wave_lds_sync();
                     double* row = sV + u * RS;
-                    double ln = 0.0;
+                    double vn = 0.0;
                     for (int j = 0; j < K; ++j) {
                         if (u == j) {
                             double s2 = row[0] * row[0];
 #pragma unroll 8
                             for (int c = 1; c < N; ++c) s2 = fma(row[c], row[c], s2);
                             const double nrm = sqrt(s2);
-                            ln = nrm;  // its log is taken after the loop, by every row lane at once
+                            vn = nrm;  // accumulated after the loop, by every row lane at once
                             const double inv = 1.0 / nrm;
 #pragma unroll 8
                             for (int c = 0; c < N; ++c) row[c] = row[c] * inv;
@@ -427,9 +433,14 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     wave_lds_sync();
 #pragma unroll
                     for (int m = 0; m < E; ++m) V[m] = sV[(vg + G * m) * RS + i];
-                    if (u < K) ln = det_log(ln);
                     ++mr;
-                    if (mr > n_tr && u < K) kahan_add(sum, comp, ln);
+                    if (mr > n_tr && u < K) {
+                        lognorm_push(lmant, lex, vn);
+                        if (++lcnt == kLogWindow || mr == m_total) {
+                            lognorm_flush(lmant, lex, sum, comp);
+                            lcnt = 0;
+                        }
+                    }
                     if (mr == m_total) finished = true;
                 }
             }

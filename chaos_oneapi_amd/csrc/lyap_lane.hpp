@@ -103,9 +103,10 @@ LYAP_D void tangent_step(const typename M::template J<TX> (&X)[Tab<INTEG>::S], c
     }
 }
 
-// modified Gram-Schmidt on K rows of length N (sequential fma dot products)
+// modified Gram-Schmidt on K rows of length N (sequential fma dot products);
+// nrm[j] = row j's norm before its normalisation
 template <int K, int N>
-LYAP_D void mgs_rows(double (&V)[K][N], double* ln) {
+LYAP_D void mgs_rows(double (&V)[K][N], double* nrm_out) {
 #pragma unroll
     for (int j = 0; j < K; ++j) {
 #pragma unroll
@@ -120,27 +121,42 @@ LYAP_D void mgs_rows(double (&V)[K][N], double* ln) {
 #pragma unroll
         for (int i = 1; i < N; ++i) s = fma(V[j][i], V[j][i], s);
         const double nrm = sqrt(s);
-        ln[j] = det_log(nrm);
+        nrm_out[j] = nrm;
         const double inv = 1.0 / nrm;
 #pragma unroll
         for (int i = 0; i < N; ++i) V[j][i] = V[j][i] * inv;
     }
 }
 
+// The error norm squared with one division:
+//   err^2 = (1/N) sum_i (ev_i / sk_i)^2 = sum_i ev_i^2 w_i / (N prod_j A_j),
+// A_j = sk_j^2, w_i = prod_{j != i} A_j as prefix x suffix products (a factor 1
+// left out).  Three divisions, a square root and a division by N become one
+// division; the 3-D flows only (products of N - 1 squared scales in range).
 template <int N>
-LYAP_D double err_norm_seq(const double* ev, const double* x, const double* xn, double rtol,
-                           double atol) {
-    double q[N];
+LYAP_D double err2_seq(const double* ev, const double* x, const double* xn, double rtol, double atol) {
+    double A[N], pre[N], suf[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const double ax = fabs(x[i]), an = fabs(xn[i]);
         const double sk = fma(rtol, ax > an ? ax : an, atol);
-        q[i] = ev[i] / sk;
+        A[i] = sk * sk;
     }
-    double s = q[0] * q[0];
+    pre[0] = 1.0;
 #pragma unroll
-    for (int i = 1; i < N; ++i) s = fma(q[i], q[i], s);
-    return sqrt(s / (double)N);
+    for (int i = 1; i < N; ++i) pre[i] = (i == 1) ? A[0] : pre[i - 1] * A[i - 1];
+    suf[N - 1] = 1.0;
+#pragma unroll
+    for (int i = N - 2; i >= 0; --i) suf[i] = (i == N - 2) ? A[N - 1] : A[i + 1] * suf[i + 1];
+    double num = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const double w = (i == 0) ? suf[0] : ((i == N - 1) ? pre[N - 1] : pre[i] * suf[i]);
+        const double e2 = ev[i] * ev[i];
+        num = (i == 0) ? e2 * w : fma(e2, w, num);
+    }
+    const double den = (double)N * ((N == 1) ? A[0] : pre[N - 1] * A[N - 1]);
+    return num / den;
 }
 
 template <int N>
@@ -189,7 +205,9 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     unsigned long long* const counter = io.counter;
     const int64_t count = io.count;
 
-    double x[N], V[K][N], sum[K], comp[K], p[P], k1[N];
+    double x[N], V[K][N], sum[K], comp[K], lmant[K], p[P], k1[N];
+    int32_t lex[K];
+    int lcnt = 0;  // renormalisations in the open log window
     float pf[P];
     double t = 0.0, h = 0.0;
     int64_t m = 0, step_in = 0, item = -1;
@@ -241,9 +259,12 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 for (int r = 0; r < K; ++r) {
                     sum[r] = 0.0;
                     comp[r] = 0.0;
+                    lmant[r] = 1.0;
+                    lex[r] = 0;
 #pragma unroll
                     for (int i = 0; i < N; ++i) V[r][i] = (r == i) ? 1.0 : 0.0;
                 }
+                lcnt = 0;
                 t = 0.0;
                 h = dp->h;
                 m = 0;
@@ -264,12 +285,17 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 const int np = __popcll(pm);
                 if (np >= gather || np == __popcll(__ballot(active))) {
                     if (pending) {
-                        double ln[K];
-                        mgs_rows<K, N>(V, ln);
+                        double nrm[K];
+                        mgs_rows<K, N>(V, nrm);
                         ++m;
                         if (m > n_tr) {
 #pragma unroll
-                            for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
+                            for (int r = 0; r < K; ++r) lognorm_push(lmant[r], lex[r], nrm[r]);
+                            if (++lcnt == kLogWindow || m == m_total) {
+#pragma unroll
+                                for (int r = 0; r < K; ++r) lognorm_flush(lmant[r], lex[r], sum[r], comp[r]);
+                                lcnt = 0;
+                            }
                         }
                         pending = false;
                         if (m == m_total) {
@@ -310,15 +336,15 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
 #pragma unroll
                     for (int i = 0; i < N; ++i) k1_next[i] = k1[i];
                     base_step<M, INTEG, true, TX>(x, p, h_try, xn, ev, X, k1_next);
-                    const double err = err_norm_seq<N>(ev, x, xn, rtol, atol);
-                    if (!(err < kInf)) {
+                    const double e2 = err2_seq<N>(ev, x, xn, rtol, atol);
+                    if (!(e2 < kInf)) {
                         st = LYAP_PT_NONFINITE;
                         finished = true;
                     } else {
                         double h_next;
-                        const double lerr = ctrl_log(err);
-                        const bool acc =
-                            ctrl_update<INTEG>(ctrl, err, lerr, log_floor, h_try, h_max, h_next);
+                        const double lerr = 0.5 * ctrl_log(e2);
+                        const bool acc = ctrl_update<INTEG>(ctrl, e2 <= 1.0, e2 > 1e-8, lerr, log_floor,
+                                                            h_try, h_max, h_next);
                         h = h_next;
                         if (!acc) {
                             ++nrej;
@@ -375,7 +401,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
 
 // Persistent per-lane loop of the sweep's method for maps: per iteration
 // V <- J(x) V (binary32 products in the mixed form), x <- f(x); MGS and the
-// Kahan-summed logs every steps_per_renorm iterations; the escape test after
+// windowed log-norm accumulation every steps_per_renorm iterations; the escape test after
 // every iteration (a diverging map overflows within a few iterations).
 template <class M, bool MIXED, class GetDesc>
 LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io) {
@@ -388,7 +414,9 @@ LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io)
     const double t_avg = (double)d.n_renorm_average * (double)d.steps_per_renorm * d.h;
     const int n_exps = d.n_exps;
 
-    double x[N], V[K][N], sum[K], comp[K], p[P];
+    double x[N], V[K][N], sum[K], comp[K], lmant[K], p[P];
+    int32_t lex[K];
+    int lcnt = 0;
     float pf[P];
     int64_t m = 0, step_in = 0, item = -1;
     uint32_t nacc = 0;
@@ -410,9 +438,12 @@ LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io)
 #pragma unroll
                 for (int r = 0; r < K; ++r) {
                     sum[r] = comp[r] = 0.0;
+                    lmant[r] = 1.0;
+                    lex[r] = 0;
 #pragma unroll
                     for (int i = 0; i < N; ++i) V[r][i] = (r == i) ? 1.0 : 0.0;
                 }
+                lcnt = 0;
                 m = step_in = 0;
                 nacc = 0;
                 active = true;
@@ -448,12 +479,17 @@ LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io)
             finished = true;
         } else if (++step_in == spr) {
             step_in = 0;
-            double ln[K];
-            mgs_rows<K, N>(V, ln);
+            double nrm[K];
+            mgs_rows<K, N>(V, nrm);
             ++m;
             if (m > n_tr) {
 #pragma unroll
-                for (int r = 0; r < K; ++r) kahan_add(sum[r], comp[r], ln[r]);
+                for (int r = 0; r < K; ++r) lognorm_push(lmant[r], lex[r], nrm[r]);
+                if (++lcnt == kLogWindow || m == m_total) {
+#pragma unroll
+                    for (int r = 0; r < K; ++r) lognorm_flush(lmant[r], lex[r], sum[r], comp[r]);
+                    lcnt = 0;
+                }
             }
             if (m == m_total) finished = true;
         }

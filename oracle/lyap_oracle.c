@@ -510,8 +510,8 @@ static double dot_ns(int n, int tree, const double* a, const double* b) {
 }
 
 /* modified Gram-Schmidt on k row vectors of length n; rows end unit length;
-   log_norms[j] = log of row j's norm before its normalisation */
-void orc_mgs(int n, int k, int tree, double* V, double* log_norms) {
+   norms[j] = row j's norm before its normalisation */
+static void mgs_norms(int n, int k, int tree, double* V, double* norms) {
     for (int j = 0; j < k; ++j) {
         double* vj = V + j * n;
         for (int r = 0; r < j; ++r) {
@@ -520,10 +520,39 @@ void orc_mgs(int n, int k, int tree, double* V, double* log_norms) {
             for (int i = 0; i < n; ++i) vj[i] = fma(-c, vr[i], vj[i]);
         }
         double nrm = sqrt(dot_ns(n, tree, vj, vj));
-        log_norms[j] = orc_log(nrm);
+        norms[j] = nrm;
         double inv = 1.0 / nrm;
         for (int i = 0; i < n; ++i) vj[i] = vj[i] * inv;
     }
+}
+
+/* the same, with log_norms[j] = log of row j's norm */
+void orc_mgs(int n, int k, int tree, double* V, double* log_norms) {
+    mgs_norms(n, k, tree, V, log_norms);
+    for (int j = 0; j < k; ++j) log_norms[j] = orc_log(log_norms[j]);
+}
+
+/* Log-norm accumulation of the sweep's method.  The norms of up to
+   ORC_LOG_WINDOW consecutive renormalisations are multiplied into a product
+   kept as a mantissa in [1, 2) and an exact binary exponent; the log of each
+   window's product, log(mant) + ex*ln2, is Kahan-added to the sum.  A
+   mantissa that is zero, subnormal, infinite or NaN keeps its exponent field
+   (its log is then -inf, a finite value, +inf or NaN, as the log of the
+   norm itself would have been). */
+double orc_split_exponent(double m, int32_t* ex) {
+    uint64_t b;
+    memcpy(&b, &m, 8);
+    const int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0 || e == 0x7ff) return m;
+    *ex += e - 1023;
+    b = (b & 0x800fffffffffffffULL) | 0x3ff0000000000000ULL;
+    memcpy(&m, &b, 8);
+    return m;
+}
+
+double orc_log_scaled(double mant, int32_t ex) {
+    const double dk = (double)ex;
+    return fma(dk, LN2_HI, fma(dk, LN2_LO, orc_log(mant)));
 }
 
 /* cuda/tools/linalg.dp.cpp:5-10 */
@@ -613,6 +642,34 @@ double orc_err_norm(int n, int tree, const double* ev, const double* x, const do
     return sqrt(s / (double)n);
 }
 
+/* The 3-D flows' error norm, squared, with one division:
+     err^2 = (1/n) sum_i (ev_i / sk_i)^2 = sum_i ev_i^2 w_i / (n prod_j A_j),
+   A_j = sk_j^2, w_i = prod_{j != i} A_j as prefix x suffix products (a factor
+   1 is left out, which is exact).  For small n only (the products of n - 1
+   squared scales must stay in range). */
+double orc_err2_seq(int n, const double* ev, const double* x, const double* xn, double rtol,
+                    double atol) {
+    double A[ORC_MAX_DIM], pre[ORC_MAX_DIM], suf[ORC_MAX_DIM];
+    if (n <= 0) return 0.0;
+    for (int i = 0; i < n; ++i) {
+        double ax = fabs(x[i]), an = fabs(xn[i]);
+        double sk = fma(rtol, ax > an ? ax : an, atol);
+        A[i] = sk * sk;
+    }
+    pre[0] = 1.0;
+    for (int i = 1; i < n; ++i) pre[i] = (i == 1) ? A[0] : pre[i - 1] * A[i - 1];
+    suf[n - 1] = 1.0;
+    for (int i = n - 2; i >= 0; --i) suf[i] = (i == n - 2) ? A[n - 1] : A[i + 1] * suf[i + 1];
+    double num = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double w = (i == 0) ? suf[0] : ((i == n - 1) ? pre[n - 1] : pre[i] * suf[i]);
+        const double e2 = ev[i] * ev[i];
+        num = (i == 0) ? e2 * w : fma(e2, w, num);
+    }
+    const double den = (double)n * ((n == 1) ? A[0] : pre[n - 1] * A[n - 1]);
+    return num / den;
+}
+
 /* Step-control log and exp: the reductions of orc_log / orc_exp with shorter
    series (about 1e-10 relative, ample for a step-size factor).  ctrl_exp is
    only asked for the range that survives the factor limits below, so its
@@ -682,16 +739,17 @@ static double ctrl_expo(int integ) {
    in the equivalent form h_new = h * r, r = clamp(0.9 * exp(-(expo*lerr -
    beta*log facOld)), 0.2, 10): one exponential and no division.  lerr is
    orc_ctrl_log(err).  Returns 1 on accept; *h_next is the next attempt. */
-static int ctrl_update(ctrl_t* c, int integ, double err, double lerr, double log_floor,
+/* ok: err <= 1; above_floor: err > 1e-4; lerr = log(err) */
+static int ctrl_update(ctrl_t* c, int integ, int ok, int above_floor, double lerr, double log_floor,
                        double h_try, double h_max, double* h_next) {
     const double expo = ctrl_expo(integ);
-    if (err <= 1.0) {
+    if (ok) {
         const double e = fma(-CTRL_BETA, c->log_fac_old, expo * lerr);
         double r = CTRL_SAFE * orc_ctrl_exp(-e);
         r = r < 0.2 ? 0.2 : r;
         r = r > 10.0 ? 10.0 : r;
         double hn = h_try * r;
-        c->log_fac_old = err > 1e-4 ? lerr : log_floor;
+        c->log_fac_old = above_floor ? lerr : log_floor;
         if (fabs(hn) > h_max) hn = h_max;
         if (c->rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
         c->rejected = 0;
@@ -765,6 +823,34 @@ static void kahan_add(double* s, double* c, double v) {
     *s = t;
 }
 
+typedef struct {
+    double sum[ORC_MAX_DIM], comp[ORC_MAX_DIM], mant[ORC_MAX_DIM];
+    int32_t ex[ORC_MAX_DIM];
+    int cnt;
+} lnacc_t;
+
+static void lnacc_init(lnacc_t* a, int k) {
+    for (int r = 0; r < k; ++r) {
+        a->sum[r] = a->comp[r] = 0.0;
+        a->mant[r] = 1.0;
+        a->ex[r] = 0;
+    }
+    a->cnt = 0;
+}
+
+/* one renormalisation of the averaging phase; `last` closes the window */
+static void lnacc_push(lnacc_t* a, int k, const double* nrm, int last) {
+    for (int r = 0; r < k; ++r) a->mant[r] = orc_split_exponent(a->mant[r] * nrm[r], &a->ex[r]);
+    if (++a->cnt == ORC_LOG_WINDOW || last) {
+        for (int r = 0; r < k; ++r) {
+            kahan_add(&a->sum[r], &a->comp[r], orc_log_scaled(a->mant[r], a->ex[r]));
+            a->mant[r] = 1.0;
+            a->ex[r] = 0;
+        }
+        a->cnt = 0;
+    }
+}
+
 /* added method: coupled variational RK, MGS on exact time boundaries */
 static void point_var(const orc_desc* d, const double* p, double* x, double* lam, uint32_t* status,
                       uint32_t* n_acc_out, uint32_t* n_rej_out) {
@@ -775,13 +861,14 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
     const int64_t n_av = llround(d->t_average / d->t_renorm);
     const int64_t m_total = n_tr + n_av;
     double V[ORC_MAX_DIM * ORC_MAX_DIM];
-    double sum[ORC_MAX_DIM], comp[ORC_MAX_DIM], ln[ORC_MAX_DIM];
+    double nrm[ORC_MAX_DIM];
+    lnacc_t lna;
     double xn[ORC_MAX_DIM], ev[ORC_MAX_DIM], sx[8 * ORC_MAX_DIM];
     for (int r = 0; r < k; ++r)
         for (int i = 0; i < n; ++i) {
             V[r * n + i] = (r == i) ? 1.0 : 0.0;
         }
-    for (int r = 0; r < k; ++r) sum[r] = comp[r] = 0.0;
+    lnacc_init(&lna, k);
     uint32_t st = ORC_OK;
     uint64_t n_acc = 0, n_rej = 0;
     double t_avg_exact;
@@ -789,7 +876,7 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
 /* MGS dot products are sequential for every model (the GPU stages the
    deviation matrix in LDS and gives each vector to one lane); the step's
    error norm uses the butterfly for Lorenz-96 */
-#define ORC_RENORM() orc_mgs(n, k, 0, V, ln)
+#define ORC_RENORM() mgs_norms(n, k, 0, V, nrm)
 
     if (!d->adaptive) {
         const int64_t R = llround(d->t_renorm / d->h);
@@ -810,8 +897,7 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if (st == ORC_OK && escaped(n, x, d->escape_radius)) st = ORC_ESCAPED;
             if (st != ORC_OK) break;
             ORC_RENORM();
-            if (m + 1 > n_tr)
-                for (int r = 0; r < k; ++r) kahan_add(&sum[r], &comp[r], ln[r]);
+            if (m + 1 > n_tr) lnacc_push(&lna, k, nrm, m + 1 == m_total);
         }
     } else {
         t_avg_exact = (double)n_av * d->t_renorm;
@@ -831,11 +917,25 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if (t_next - t <= 1.0625 * h_try) { h_try = t_next - t; clipped = 1; }
             if (h_try < d->h_min) { st = ORC_STEP_UNDERFLOW; break; }
             orc_rk_step(d->model, integ, n, x, p, h_try, xn, ev, sx);
-            double err = orc_err_norm(n, tree, ev, x, xn, d->rtol, d->atol);
-            if (!(err < INFINITY)) { st = ORC_NONFINITE; break; }
+            /* Lorenz-96: the norm (butterfly sum); the 3-D flows: its square
+               (one division), compared with 1 and 1e-8, log halved */
+            int ok, above;
+            double lerr;
+            if (tree) {
+                const double err = orc_err_norm(n, tree, ev, x, xn, d->rtol, d->atol);
+                if (!(err < INFINITY)) { st = ORC_NONFINITE; break; }
+                ok = err <= 1.0;
+                above = err > 1e-4;
+                lerr = orc_ctrl_log(err);
+            } else {
+                const double e2 = orc_err2_seq(n, ev, x, xn, d->rtol, d->atol);
+                if (!(e2 < INFINITY)) { st = ORC_NONFINITE; break; }
+                ok = e2 <= 1.0;
+                above = e2 > 1e-8;
+                lerr = 0.5 * orc_ctrl_log(e2);
+            }
             double h_next;
-            int acc = ctrl_update(&c, integ, err, orc_ctrl_log(err), log_floor, h_try, d->h_max,
-                                  &h_next);
+            int acc = ctrl_update(&c, integ, ok, above, lerr, log_floor, h_try, d->h_max, &h_next);
             h = h_next;
             if (!acc) { ++n_rej; continue; }
             for (int r = 0; r < k; ++r) {
@@ -849,15 +949,14 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if (clipped) {
                 ORC_RENORM();
                 ++m;
-                if (m > n_tr)
-                    for (int r = 0; r < k; ++r) kahan_add(&sum[r], &comp[r], ln[r]);
+                if (m > n_tr) lnacc_push(&lna, k, nrm, m == m_total);
                 if (m == m_total) break;
                 t_next = (double)(m + 1) * d->t_renorm;
             }
         }
     }
 #undef ORC_RENORM
-    for (int r = 0; r < k; ++r) lam[r] = (st == ORC_OK) ? sum[r] / t_avg_exact : NAN;
+    for (int r = 0; r < k; ++r) lam[r] = (st == ORC_OK) ? lna.sum[r] / t_avg_exact : NAN;
     *status = st;
     *n_acc_out = (uint32_t)n_acc;
     *n_rej_out = (uint32_t)n_rej;
@@ -1018,7 +1117,7 @@ static void map_tangent(int model, int n, int k, const double* x, const double* 
 }
 
 /* the sweep's method for maps: R = t_renorm iterations per renormalisation
-   (MGS, Kahan-summed logs), the transient from the start */
+   (MGS, windowed log-norm accumulation), the transient from the start */
 static void point_map_var(const orc_desc* d, const double* p, double* x, double* lam,
                           uint32_t* status, uint32_t* n_acc_out) {
     const int n = d->dim, k = d->n_exps;
@@ -1026,11 +1125,11 @@ static void point_map_var(const orc_desc* d, const double* p, double* x, double*
     const int64_t R = llround(d->t_renorm / d->h);
     const int64_t n_tr = llround((d->t_transient + d->t_slave) / d->t_renorm);
     const int64_t n_av = llround(d->t_average / d->t_renorm);
-    double V[ORC_MAX_DIM * ORC_MAX_DIM], sum[ORC_MAX_DIM], comp[ORC_MAX_DIM], ln[ORC_MAX_DIM];
-    for (int r = 0; r < k; ++r) {
-        sum[r] = comp[r] = 0.0;
+    double V[ORC_MAX_DIM * ORC_MAX_DIM], nrm[ORC_MAX_DIM];
+    lnacc_t lna;
+    lnacc_init(&lna, k);
+    for (int r = 0; r < k; ++r)
         for (int i = 0; i < n; ++i) V[r * n + i] = (r == i) ? 1.0 : 0.0;
-    }
     uint32_t st = ORC_OK;
     uint64_t n_acc = 0;
     /* a diverging map orbit overflows within a few iterations (x ~ x^2), so
@@ -1043,12 +1142,11 @@ static void point_map_var(const orc_desc* d, const double* p, double* x, double*
             if (escaped(n, x, d->escape_radius)) { st = ORC_ESCAPED; break; }
         }
         if (st != ORC_OK) break;
-        orc_mgs(n, k, 0, V, ln);
-        if (m + 1 > n_tr)
-            for (int r = 0; r < k; ++r) kahan_add(&sum[r], &comp[r], ln[r]);
+        mgs_norms(n, k, 0, V, nrm);
+        if (m + 1 > n_tr) lnacc_push(&lna, k, nrm, m + 1 == n_tr + n_av);
     }
     const double t_avg = (double)n_av * (double)R * d->h;
-    for (int r = 0; r < k; ++r) lam[r] = (st == ORC_OK) ? sum[r] / t_avg : NAN;
+    for (int r = 0; r < k; ++r) lam[r] = (st == ORC_OK) ? lna.sum[r] / t_avg : NAN;
     *status = st;
     *n_acc_out = (uint32_t)n_acc;
 }

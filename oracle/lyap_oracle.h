@@ -19,7 +19,8 @@
  * plus the method the sweep adds on top of the reference (coupled variational
  * RK with the Jacobian at every stage state, modified Gram-Schmidt on exact
  * time boundaries, adaptive DVERK/Dopri5, FP32 tangent stages with FP64
- * renormalisation and Kahan-compensated log sums).
+ * renormalisation, norm products over windows of 16 renormalisations and
+ * Kahan-compensated sums of the windows' logs).
  *
  * The descriptor below has the same memory layout as lyap_sweep_desc in
  * include/chaos_lyap.h so one Python dictionary can fill both.
@@ -103,12 +104,23 @@ void orc_frozen_tangent_step(int model, int n, const double* x_frozen, const dou
 
 /* orthogonalisation kernels */
 void orc_mgs(int n, int k, int tree, double* V /* k rows of n */, double* log_norms);
+/* log-norm accumulation of the sweep's method: norms of up to ORC_LOG_WINDOW
+   consecutive renormalisations multiply into mant * 2^ex (mant in [1, 2),
+   orc_split_exponent moves the binary exponent of a positive normal value
+   into *ex), and log(mant * 2^ex) = orc_log_scaled(mant, ex) of each window
+   is Kahan-summed */
+#define ORC_LOG_WINDOW 16
+double orc_split_exponent(double m, int32_t* ex);
+double orc_log_scaled(double mant, int32_t ex);
 void orc_cgs_ref(int n, int k, double* V /* k rows of n */);
 void orc_normalize_ref(int n, int k, double* V, double eps);
 void orc_qr_ref(int n, double* Q /* n x n row-major, columns are vectors */, double* Rdiag);
 
 /* controller (reference dopriStep constants) */
 double orc_err_norm(int n, int tree, const double* err_vec, const double* x, const double* x_new,
+                    double rtol, double atol);
+/* the 3-D flows' form: the error norm squared with one division (small n) */
+double orc_err2_seq(int n, const double* err_vec, const double* x, const double* x_new,
                     double rtol, double atol);
 
 /* point parameters / initial state */

@@ -87,11 +87,17 @@ def lib():
         L.orc_tangent_step.argtypes = [i, i, i, P, P, d, P]
         L.orc_frozen_tangent_step.argtypes = [i, i, P, P, d, P]
         L.orc_mgs.argtypes = [i, i, i, P, P]
+        L.orc_split_exponent.restype = d
+        L.orc_split_exponent.argtypes = [d, ctypes.POINTER(ctypes.c_int32)]
+        L.orc_log_scaled.restype = d
+        L.orc_log_scaled.argtypes = [d, ctypes.c_int32]
         L.orc_cgs_ref.argtypes = [i, i, P]
         L.orc_normalize_ref.argtypes = [i, i, P, d]
         L.orc_qr_ref.argtypes = [i, P, P]
         L.orc_err_norm.restype = d
         L.orc_err_norm.argtypes = [i, i, P, P, P, d, d]
+        L.orc_err2_seq.restype = d
+        L.orc_err2_seq.argtypes = [i, P, P, P, d, d]
         L.orc_point_params.argtypes = [P, i64, P, P]
         L.orc_point_init.argtypes = [P, i64, P, P, P]
         L.orc_point.argtypes = [P, i64, P, P, P, P, P, P]
@@ -170,6 +176,22 @@ def mgs(V, tree=False):
     ln = np.empty(k)
     lib().orc_mgs(n, k, int(tree), _ptr(V), _ptr(ln))
     return V, ln
+
+
+LOG_WINDOW = 16  # ORC_LOG_WINDOW
+
+
+def split_exponent(m, ex=0):
+    """(mant, ex): a positive normal m as mant * 2^(ex - ex_in), mant in [1, 2);
+    other values come back unchanged with ex unchanged."""
+    e = ctypes.c_int32(int(ex))
+    mant = lib().orc_split_exponent(float(m), ctypes.byref(e))
+    return mant, e.value
+
+
+def log_scaled(mant, ex):
+    """log(mant * 2^ex) as the sweep's method takes it per log window."""
+    return lib().orc_log_scaled(float(mant), int(ex))
 
 
 def cgs_ref(V):

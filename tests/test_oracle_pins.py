@@ -49,6 +49,59 @@ def test_step_control_log_exp():
     assert L.orc_ctrl_log(0.0) == -math.inf
 
 
+def test_windowed_log_norm_accumulation():
+    """The sweep's method multiplies the norms of up to LOG_WINDOW
+    renormalisations into mant * 2^ex and takes one log per window.  Pins:
+    split_exponent is exact (mant in [1, 2), mant * 2^ex == the product, by
+    ldexp) and leaves zero/subnormal/inf/NaN alone; log_scaled is log(mant) +
+    ex ln2 to ~1 ulp (math.log, math.ldexp); a window's log equals the sum
+    of the logs of its norms (math.fsum) to ~1e-15 relative."""
+    rng = np.random.default_rng(5)
+    for m in np.concatenate([10.0 ** rng.uniform(-307, 307, 3000), [1.0, 2.0, 1.5, 0.75, 2.2250738585072014e-308]]):
+        mant, ex = O.split_exponent(m, 7)
+        assert 1.0 <= mant < 2.0
+        assert math.ldexp(mant, ex - 7) == m
+    for v in (0.0, 5e-324, 1e-310, math.inf):
+        assert O.split_exponent(v, 3) == (v, 3)
+    mant, ex = O.split_exponent(math.nan, 3)
+    assert math.isnan(mant) and ex == 3
+    for _ in range(2000):
+        mant, ex = rng.uniform(1.0, 2.0), int(rng.integers(-3000, 3000))
+        ref = math.log(mant) + ex * math.log(2.0)
+        assert abs(O.log_scaled(mant, ex) - ref) <= 2.5e-16 * max(1.0, abs(ref))
+    # a window of norms, as the kernels accumulate them
+    for _ in range(200):
+        norms = np.exp(rng.uniform(-8.0, 8.0, O.LOG_WINDOW))
+        mant, ex = 1.0, 0
+        for nv in norms:
+            mant, ex = O.split_exponent(mant * nv, ex)
+        ref = math.fsum(math.log(v) for v in norms)
+        assert abs(O.log_scaled(mant, ex) - ref) <= 1e-15 * max(1.0, abs(ref)) * O.LOG_WINDOW
+
+
+def test_squared_error_norm_of_the_3d_flows():
+    """orc_err2_seq (one division) against numpy's mean of (ev/sk)^2 with
+    sk = rtol*max(|x|, |xn|) + atol, for n = 1..4 and ranges the controller
+    sees; and orc_err_norm (per-component divisions) squared agrees."""
+    L = O.lib()
+    rng = np.random.default_rng(9)
+    for _ in range(3000):
+        n = int(rng.integers(1, 5))
+        x = rng.uniform(-50, 50, n) * 10.0 ** rng.uniform(-6, 2, n)
+        xn = x + rng.normal(0, 1e-3, n)
+        ev = rng.normal(0, 1, n) * 10.0 ** rng.uniform(-16, -6, n)
+        rtol, atol = 10.0 ** rng.uniform(-12, -6), 10.0 ** rng.uniform(-12, -6)
+        sk = rtol * np.maximum(np.abs(x), np.abs(xn)) + atol
+        ref = float(np.mean((ev / sk) ** 2))
+        got = L.orc_err2_seq(n, O._ptr(O.f64(ev)), O._ptr(O.f64(x)), O._ptr(O.f64(xn)), rtol, atol)
+        assert abs(got - ref) <= 8e-16 * n * ref
+        e1 = L.orc_err_norm(n, 0, O._ptr(O.f64(ev)), O._ptr(O.f64(x)), O._ptr(O.f64(xn)), rtol, atol)
+        assert abs(e1 * e1 - got) <= 1e-14 * got
+    z = np.zeros(3)
+    assert L.orc_err2_seq(3, O._ptr(z), O._ptr(O.f64([1.0, 2.0, 3.0])), O._ptr(O.f64([1.0, 2.0, 3.0])),
+                          1e-9, 1e-9) == 0.0
+
+
 def test_loop_count_matches_float_loop():
     # cuda/lyapunov.dp.cpp iterates `for (double t = t0; t < T; t += h)`
     def brute(t0, t1, h):
