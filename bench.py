@@ -158,6 +158,8 @@ def bench_sweep(args, ctx, spec, wl_name):
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
         "max_dlambda1_chosen_vs_plain_fp64": d1,
         "points_over_1e-3": int(np.sum(np.abs(lam_c[ok, 0] - lam_p[ok, 0]) > 1e-3)),
+        "dlambda1_chosen_vs_plain_fp64_p999": float(np.quantile(np.abs(lam_c[ok, 0] - lam_p[ok, 0]), 0.999)) if ok.any() else 0.0,
+        "dlambda1_chosen_vs_plain_fp64_median": float(np.median(np.abs(lam_c[ok, 0] - lam_p[ok, 0]))) if ok.any() else 0.0,
         "classification_agreement": cls, "points_status_ok": int(ok.sum()),
         "mean_steps_per_point": float(na_c.mean()), "mean_rejects_per_point": float(nr_c.mean()),
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
@@ -214,6 +216,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
     launch_ms = [st["first_kernel_ms"], st["rerun_kernel_ms"]]
     achieved = fl / (sum(launch_ms) * 1e-3) / 1e12
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
+    dl_c = np.abs(res.lam[computed, 0] - lam_p[computed, 0])
     rec["config"] = {
         "workload": wl_name, "points": n, "chosen_form": "mixed + surrogate refinement",
         "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
@@ -228,7 +231,10 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "class_agreement_verify": st.get("class_agreement"),
         "surrogate_answered_verify": st.get("surrogate_answered_verify"),
         "surrogate_class_agreement_verify": st.get("surrogate_class_agreement"),
-        "max_dlambda1_computed_vs_plain_fp64": float(np.max(np.abs(res.lam[computed, 0] - lam_p[computed, 0]))) if computed.any() else 0.0,
+        "max_dlambda1_computed_vs_plain_fp64": float(np.max(dl_c)) if dl_c.size else 0.0,
+        "computed_points_over_1e-3_vs_plain_fp64": int(np.sum(dl_c > 1e-3)),
+        "dlambda1_computed_vs_plain_fp64_p999": float(np.quantile(dl_c, 0.999)) if dl_c.size else 0.0,
+        "rerun_lane_gather": st.get("rerun_gather"),
         "class_agreement_full_plane": float(np.mean(cls_final == cls_plain)),
         "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
@@ -271,7 +277,7 @@ def bench_batch(args, ctx, n_points, wl_name):
     classes = device_classes(L.PREC_MIXED)
     progress("classes " + ", ".join(f"{c['name']} {c['count']}" for c in mix))
     t0 = time.perf_counter()
-    tune = A.tune_batch(ctx, classes, cache=A.Cache(None))
+    tune = A.tune_batch(ctx, classes, cache=A.Cache(None), time_scale=0.1)
     tune_s = time.perf_counter() - t0
     progress(f"tuned in {tune_s:.1f} s: {tune.best} {tune.timings}")
     b = L.Batch(ctx, classes, **A.batch_kwargs(tune.best))
@@ -288,10 +294,19 @@ def bench_batch(args, ctx, n_points, wl_name):
     outs = [tuple(t.cpu().numpy() for t in o) for o in b.outputs]
     b.close()
     fl = sum(F.sweep_flops(c["desc"], o[2], o[3]) for c, o in zip(classes, outs))
-    # the same classes as separate single-class launches (default shapes)
+    # the same classes as separate single-class launches of the same kernels
+    # (the batch layout's Lorenz-96 points per wave)
     single_ms = 0.0
+    layout = A.batch_kwargs(tune.best)["layout"]
     for c, m in zip(classes, mix):
-        ctx.run(c["desc"], count=c["count"], params=c["params"])
+        dim = c["desc"].dim
+        if c["desc"].model == L.MODEL_L96:
+            ctx.set_option(f"l96_ppw_{dim}", L.batch_layout_ppw(layout, dim))
+        try:
+            ctx.run(c["desc"], count=c["count"], params=c["params"])
+        finally:
+            if c["desc"].model == L.MODEL_L96:
+                ctx.set_option(f"l96_ppw_{dim}", 0)
         single_ms += ctx.kernel_ms()
         progress(f"single-class launch {m['name']}: {ctx.kernel_ms():.0f} ms")
     # plain FP64: the same batch in FP64 (same tuning)

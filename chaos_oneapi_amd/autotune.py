@@ -11,9 +11,11 @@ Knobs, all exposed by liblyap.so:
   - single 3-D sweeps: the persistent grid size (lane_max_blocks) and the
     renormalisation gather (lane_gather).
 
-Each candidate is timed on a sample of the workload (a seeded subset of the
-points with the full integration times, so per-point costs keep their
-distribution), the fastest is kept, and the choice is cached in a JSON file
+Each candidate is timed on a sample of the workload — a seeded subset of the
+points with the full integration times (per-point costs keep their
+distribution), or, for the batched launch, every point at a fraction of the
+integration times (the launch fills the GPU as the full one does and is not
+timed by a few slow points) — the fastest is kept, and the choice is cached in a JSON file
 keyed by the device name and a signature of the workload.  Every candidate
 computes the same results bit for bit (the layouts change which lanes do the
 work, not the arithmetic), so tuning never changes the output.
@@ -144,18 +146,40 @@ def sample_classes(classes, frac: float = 1 / 16, min_points: int = 256, seed: i
     return out
 
 
+def scaled_classes(classes, time_scale: float):
+    """The same points with the transient and averaging times scaled by
+    time_scale (the descriptors re-finalised: renormalisation counts follow,
+    t_renorm and the step control stay).  Per-step work and its spread over the
+    points are unchanged, so a launch shape that wins on the scaled batch wins
+    on the full one, while the scaled batch still fills the GPU as the full one
+    does — a subset of the points would be timed by its few slowest points."""
+    out = []
+    for c in classes:
+        d = L.SweepDesc.from_buffer_copy(c["desc"])
+        d.t_transient *= time_scale
+        d.t_slave *= time_scale
+        d.t_average *= time_scale
+        s = dict(c)
+        s["desc"] = L.finalize(d)
+        out.append(s)
+    return out
+
+
 def tune_batch(ctx, classes, frac: float = 1 / 16, min_points: int = 256, repeats: int = 1,
                cache: Cache | None = None, candidates=BATCH_CANDIDATES, force: bool = False,
-               timer=None) -> TuneResult:
+               timer=None, time_scale: float | None = None) -> TuneResult:
     """Pick the batched-launch tuning for `classes` (the Batch class list).
 
+    With time_scale the candidates run on every point at time_scale times the
+    integration time (scaled_classes); without it, on a seeded subset of the
+    points (sample_classes: frac of each class, at least MIN_SAMPLE_TOTAL).
     timer(cand, sample) -> ms may replace the GPU timing (host-side tests)."""
     cache = cache if cache is not None else Cache()
     key = batch_key(device_name(ctx.device) if timer is None else "test", classes)
     hit = cache.get(key)
     if hit and not force:
         return TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
-    sample = sample_classes(classes, frac, min_points)
+    sample = scaled_classes(classes, time_scale) if time_scale else sample_classes(classes, frac, min_points)
     timings = {}
     for cand in candidates:
         try:

@@ -651,4 +651,12 @@ int lyap_batch_destroy(lyap_batch* b) {
     return LYAP_SUCCESS;
 }
 
+int lyap_batch_layout_ppw(int layout, int dim, int* ppw) {
+    if (!ppw) return fail(LYAP_ERR_INVALID, "lyap_batch_layout_ppw: NULL output");
+    const int v = lyap::batch_points_per_wave(layout, dim);
+    if (v <= 0) return fail(LYAP_ERR_INVALID, "no batch layout %d", layout);
+    *ppw = v;
+    return LYAP_SUCCESS;
+}
+
 }  // extern "C"

@@ -141,10 +141,12 @@ __device__ __forceinline__ float gelu_grad(float x) {
 }
 
 // features[b][j] = sin(2 pi (B_j . u)), features[b][F+j] = cos(...); row- and column-major bf16
+// (loss, when given, is cleared for the loss reduction of the same pass)
 __global__ void k_fourier(const double* __restrict__ p, int n, int n_valid, const float* __restrict__ fb,
                           int F, double lo1, double inv1, double lo2, double inv2,
-                          uint16_t* __restrict__ X, uint16_t* __restrict__ Xt) {
+                          uint16_t* __restrict__ X, uint16_t* __restrict__ Xt, float* __restrict__ loss) {
     const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (loss && b == 0) *loss = 0.f;
     if (b >= n) return;
     float u0 = 0.f, u1 = 0.f;
     if (b < n_valid) {
@@ -174,15 +176,35 @@ __global__ void k_gelu_fwd(const float* __restrict__ Z, int n, int H, uint16_t* 
     Ht[(size_t)j * n + b] = h;
 }
 
-// dZ = dH * gelu'(Z) in row- and column-major bf16
-__global__ void k_gelu_bwd(const float* __restrict__ dH, const float* __restrict__ Z, int n, int H,
-                           uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * H) return;
-    const int b = (int)(idx / H), j = (int)(idx % H);
-    const uint16_t g = f2bf(dH[idx] * gelu_grad(Z[idx]));
-    dZr[idx] = g;
-    dZt[(size_t)j * n + b] = g;
+// dZ = dH * gelu'(Z) in row- and column-major bf16, and the partial column
+// sums of dZ (as rounded to bf16) for the bias gradient: block (x, r) covers
+// columns 64x .. 64x+63 and rows [r*rc, min(n, (r+1)*rc)); thread (c, g) takes
+// rows g, g+4, ... of the chunk in order, the four row groups are added in a
+// fixed tree and the chunk's sum goes to part[r][j] (k_sum_splits adds the
+// chunks in order, so the bias gradient is deterministic).
+__global__ __launch_bounds__(256) void k_gelu_bwd_colsum(const float* __restrict__ dH,
+                                                         const float* __restrict__ Z, int n, int H, int rc,
+                                                         uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                                                         float* __restrict__ part) {
+    __shared__ float red[4][64];
+    const int c = (int)(threadIdx.x & 63), g = (int)(threadIdx.x >> 6);
+    const int j = (int)blockIdx.x * 64 + c;
+    const int r = (int)blockIdx.y;
+    const int lo = r * rc, hi = min(n, lo + rc);
+    float s = 0.f;
+    if (j < H)
+        for (int b = lo + g; b < hi; b += 4) {
+            const size_t idx = (size_t)b * H + j;
+            const uint16_t v = f2bf(dH[idx] * gelu_grad(Z[idx]));
+            dZr[idx] = v;
+            dZt[(size_t)j * n + b] = v;
+            s += bf2f(v);
+        }
+    red[g][c] = s;
+    __syncthreads();
+    if (g < 2) red[g][c] += red[g + 2][c];
+    __syncthreads();
+    if (g == 0 && j < H) part[(size_t)r * H + j] = red[0][c] + red[1][c];
 }
 
 // loss and head gradient: Gaussian NLL on mu1 with the learned log-variance,
@@ -252,23 +274,42 @@ __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ d
     if (g == 0 && j < H) part[(size_t)r * H + j] = red[0][c] + red[1][c];
 }
 
-// dZ (bf16, row-major) -> fp32 for the bias reduction of hidden layers
-__global__ void k_bf_to_f(const uint16_t* __restrict__ a, float* __restrict__ b, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) b[i] = bf2f(a[i]);
-}
+// where each layer's weight matrix [O][I] sits in the parameter vector, and
+// its BF16 copies (Wt [O][I], Wn [I][O])
+struct WeightMap {
+    int64_t off[kLayers];
+    int O[kLayers], I[kLayers];
+    uint16_t* wt[kLayers];
+    uint16_t* wn[kLayers];
+};
 
-__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, const uint8_t* __restrict__ trainable, int64_t n, float lr,
-                       float b1, float b2, float eps, float c1, float c2) {
+// Adam on every trainable parameter, then the new value of every weight entry
+// issued to its BF16 copies (one pass over the parameters)
+__global__ void k_adam_issue(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                             float* __restrict__ v, const uint8_t* __restrict__ trainable, int64_t n,
+                             float lr, float b1, float b2, float eps, float c1, float c2, WeightMap wm) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !trainable[i]) return;
-    const float gi = g[i];
-    const float mi = b1 * m[i] + (1.f - b1) * gi;
-    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
-    m[i] = mi;
-    v[i] = vi;
-    w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    if (i >= n) return;
+    float wi = w[i];
+    if (trainable[i]) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        wi -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+    }
+#pragma unroll
+    for (int l = 0; l < kLayers; ++l) {
+        const int64_t k = i - wm.off[l];
+        if (k >= 0 && k < (int64_t)wm.O[l] * wm.I[l]) {
+            const int o = (int)(k / wm.I[l]), c = (int)(k % wm.I[l]);
+            const uint16_t bv = f2bf(wi);
+            wm.wt[l][k] = bv;
+            wm.wn[l][(size_t)c * wm.O[l] + o] = bv;
+        }
+    }
 }
 
 // master [O][I] fp32 -> Wt [O][I] bf16 and Wn [I][O] bf16
@@ -341,6 +382,7 @@ struct lyap_mlp {
     float* dzf = nullptr;
     float* loss = nullptr;
     float* split_ws = nullptr;  // split-K partial products / bias-gradient chunk sums
+    float* col_ws = nullptr;    // bias-gradient chunk sums of the hidden layers
     int64_t step = 0;
     std::vector<void*> allocs;
 };
@@ -356,8 +398,8 @@ static int mlp_alloc(lyap_mlp* m, void** p, size_t bytes) {
 
 int lyap_mlp_destroy(lyap_mlp* m) {
     if (!m) return LYAP_SUCCESS;
-    hipSetDevice(m->device);
-    for (void* p : m->allocs) hipFree(p);
+    (void)hipSetDevice(m->device);
+    for (void* p : m->allocs) (void)hipFree(p);
     delete m;
     return LYAP_SUCCESS;
 }
@@ -406,7 +448,7 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     rc |= mlp_alloc(m, (void**)&m->dz, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dz_t, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
-    rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * H * 4);
+    rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * kOutPad * 4);  // the head's dZ in fp32
     rc |= mlp_alloc(m, (void**)&m->loss, 4);
     size_t max_w = 0, max_d = 0;
     for (int l = 0; l < kLayers; ++l) {
@@ -414,6 +456,7 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
         max_d = std::max(max_d, (size_t)dims[l + 1]);
     }
     rc |= mlp_alloc(m, (void**)&m->split_ws, std::max((size_t)kMaxSplits * max_w, (size_t)kColsumChunks * max_d) * 4);
+    rc |= mlp_alloc(m, (void**)&m->col_ws, (size_t)kColsumChunks * max_d * 4);
     if (rc != LYAP_SUCCESS) {
         lyap_mlp_destroy(m);
         return LYAP_ERR_HIP;
@@ -507,24 +550,30 @@ static void gemm_wgrad(lyap_mlp* m, hipStream_t s, const uint16_t* A, const uint
 // bias gradient db[j] = sum_b dZ[b][j] (after the weight-gradient product of
 // the same layer has consumed split_ws): up to kColsumChunks row chunks summed
 // in parallel, then added in chunk order
+static void colsum_chunks(int n, int* chunks, int* rc) {
+    int c = n / 128;
+    if (c > kColsumChunks) c = kColsumChunks;
+    if (c < 1) c = 1;
+    *rc = ((n + c - 1) / c + 3) / 4 * 4;
+    *chunks = (n + *rc - 1) / *rc;
+}
+
 static void colsum(lyap_mlp* m, hipStream_t s, const float* dZ, int n, int H, float* db) {
-    int chunks = n / 128;
-    if (chunks > kColsumChunks) chunks = kColsumChunks;
-    if (chunks < 1) chunks = 1;
-    const int rc = ((n + chunks - 1) / chunks + 3) / 4 * 4;
-    chunks = (n + rc - 1) / rc;
+    int chunks, rc;
+    colsum_chunks(n, &chunks, &rc);
     hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((H + 63) / 64), (unsigned)chunks), dim3(256), 0, s, dZ, n, H,
                        rc, m->split_ws);
     hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, s, m->split_ws, chunks,
                        (int64_t)H, db);
 }
 
-static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s) {
+// clear_loss: the loss accumulator is cleared by the first kernel (training)
+static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s, bool clear_loss) {
     const auto& c = m->cfg;
     const int* D = m->dims;
     hipLaunchKernelGGL(k_fourier, dim3((n_pad + 255) / 256), dim3(256), 0, s, d_p, n_pad, n_valid, m->fourier,
                        c.n_fourier, c.lo1, 1.0 / (c.hi1 - c.lo1), c.lo2, 1.0 / (c.hi2 - c.lo2), m->act[0],
-                       m->act_t[0]);
+                       m->act_t[0], clear_loss ? m->loss : nullptr);
     for (int l = 0; l < kLayers; ++l) {
         gemm(s, m->act[l], m->wt[l], m->z[l], m->master + m->off_b[l], n_pad, D[l + 1], D[l], 0.f);
         if (l + 1 < kLayers) {
@@ -544,17 +593,23 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
     hipLaunchKernelGGL(k_head_loss, dim3((n_pad + 255) / 256), dim3(256), 0, s, m->z[kLayers - 1], d_lam,
                        d_status, n_pad, n_valid, m->norm, m->norm + 3, 1.0f / (float)n_valid, m->dz, m->dz_t,
                        m->dzf, m->loss);
+    int chunks, rc;
+    colsum_chunks(n_pad, &chunks, &rc);
     for (int l = kLayers - 1; l >= 0; --l) {
         // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T
         gemm_wgrad(m, s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad);
-        colsum(m, s, m->dzf, n_pad, D[l + 1], m->grad + m->off_b[l]);
+        // bias gradient: the head's from its fp32 dZ; a hidden layer's from the
+        // chunk sums its GELU backward pass left in col_ws
+        if (l == kLayers - 1)
+            colsum(m, s, m->dzf, n_pad, D[l + 1], m->grad + m->off_b[l]);
+        else
+            hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((D[l + 1] + 255) / 256)), dim3(256), 0, s, m->col_ws,
+                               chunks, (int64_t)D[l + 1], m->grad + m->off_b[l]);
         if (l == 0) break;
         // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU
         gemm(s, m->dz, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);
-        const int64_t tot = (int64_t)n_pad * D[l];
-        hipLaunchKernelGGL(k_gelu_bwd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m->dh, m->z[l - 1],
-                           n_pad, D[l], m->dz, m->dz_t);
-        hipLaunchKernelGGL(k_bf_to_f, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m->dz, m->dzf, tot);
+        hipLaunchKernelGGL(k_gelu_bwd_colsum, dim3((unsigned)((D[l] + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
+                           m->dh, m->z[l - 1], n_pad, D[l], rc, m->dz, m->dz_t, m->col_ws);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "mlp backward: %s", hipGetErrorString(e));
@@ -576,8 +631,7 @@ int lyap_mlp_backward(lyap_mlp* m, const double* d_p, const double* d_lambda, co
     hipStream_t s = (hipStream_t)stream;
     MLP_HIP(hipSetDevice(m->device));
     const int n_pad = (int)((n + 63) / 64 * 64);
-    MLP_HIP(hipMemsetAsync(m->loss, 0, 4, s));
-    if ((rc = forward(m, d_p, n_pad, (int)n, s))) return rc;
+    if ((rc = forward(m, d_p, n_pad, (int)n, s, true))) return rc;
     if ((rc = backward(m, d_lambda, d_status, n_pad, (int)n, s))) return rc;
     if (d_loss) MLP_HIP(hipMemcpyAsync(d_loss, m->loss, 4, hipMemcpyDeviceToDevice, s));
     return LYAP_SUCCESS;
@@ -591,13 +645,17 @@ int lyap_mlp_train_step(lyap_mlp* m, const double* d_p, const double* d_lambda, 
     const auto& c = m->cfg;
     m->step += 1;
     const float c1 = 1.f - std::pow(c.beta1, (float)m->step), c2 = 1.f - std::pow(c.beta2, (float)m->step);
-    hipLaunchKernelGGL(k_adam, dim3((unsigned)((m->n_params + 255) / 256)), dim3(256), 0, s, m->master, m->grad,
-                       m->adam_m, m->adam_v, m->trainable, m->n_params, c.lr, c.beta1, c.beta2, c.eps, c1, c2);
+    WeightMap wm;
     for (int l = 0; l < kLayers; ++l) {
-        const int nw = m->dims[l + 1] * m->dims[l];
-        hipLaunchKernelGGL(k_issue_bf16, dim3((nw + 255) / 256), dim3(256), 0, s, m->master + m->off_w[l],
-                           m->dims[l + 1], m->dims[l], m->wt[l], m->wn[l]);
+        wm.off[l] = m->off_w[l];
+        wm.O[l] = m->dims[l + 1];
+        wm.I[l] = m->dims[l];
+        wm.wt[l] = m->wt[l];
+        wm.wn[l] = m->wn[l];
     }
+    hipLaunchKernelGGL(k_adam_issue, dim3((unsigned)((m->n_params + 255) / 256)), dim3(256), 0, s, m->master,
+                       m->grad, m->adam_m, m->adam_v, m->trainable, m->n_params, c.lr, c.beta1, c.beta2, c.eps, c1,
+                       c2, wm);
     MLP_HIP(hipGetLastError());
     return LYAP_SUCCESS;
 }
@@ -609,7 +667,7 @@ int lyap_mlp_forward(lyap_mlp* m, const double* d_p, int64_t n, float* d_out, vo
     hipStream_t s = (hipStream_t)stream;
     MLP_HIP(hipSetDevice(m->device));
     const int n_pad = (int)((n + 63) / 64 * 64);
-    if ((rc = forward(m, d_p, n_pad, (int)n, s))) return rc;
+    if ((rc = forward(m, d_p, n_pad, (int)n, s, false))) return rc;
     hipLaunchKernelGGL(k_head_out, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->z[kLayers - 1], (int)n,
                        m->norm, m->norm + 3, d_out);
     MLP_HIP(hipGetLastError());

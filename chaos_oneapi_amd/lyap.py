@@ -34,6 +34,7 @@ EXPORTS = (
     "lyap_mlp_create", "lyap_mlp_destroy", "lyap_mlp_backward", "lyap_mlp_train_step",
     "lyap_mlp_forward", "lyap_mlp_param_count", "lyap_mlp_get", "lyap_mlp_set_params",
     "lyap_batch_create", "lyap_batch_launch", "lyap_batch_last_ms", "lyap_batch_info",
+    "lyap_batch_layout_ppw",
     "lyap_batch_destroy",
 )
 
@@ -154,6 +155,7 @@ def load(path: str = LIB_PATH):
     L.lyap_batch_launch.argtypes = [vp, vp]
     L.lyap_batch_last_ms.argtypes = [vp, P(ctypes.c_float)]
     L.lyap_batch_info.argtypes = [vp, P(i64), P(ctypes.c_int)]
+    L.lyap_batch_layout_ppw.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_int)]
     L.lyap_batch_destroy.argtypes = [vp]
     L.lyap_cuda_lyapunov.argtypes = [
         vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
@@ -276,6 +278,13 @@ class Context:
             int(dimension), int(model), _ptr(params), int(params_dim), float(eps), float(h),
             _ptr(lyap_exps), int(exps_num), float(time_skip), float(time_skip_clp),
             float(calc_time), float(add_skip_steps), ctypes.c_void_p(s.cuda_stream)))
+
+
+def batch_layout_ppw(layout: int, dim: int) -> int:
+    """Points per wave64 of a batch layout for a class of dimension dim."""
+    v = ctypes.c_int()
+    check(load().lyap_batch_layout_ppw(int(layout), int(dim), ctypes.byref(v)))
+    return int(v.value)
 
 
 class Batch:

@@ -20,6 +20,8 @@ expected cost is the step count of its nearest training-lattice point, and
 the kernel takes the sorted list in order (context option item_order).  Off
 by default: on the 2048^2 Rossler plane (3.6 M rerun points, ~18 per lane)
 it measured no gain (10.40 s against 10.36 s spread over the plane).
+rerun_gather: the lane_gather option of the rerun (2^20 points of that
+plane: 2,763 ms at 8 against 2,813 ms at 0 and 2,903 ms at 16; bit-identical).
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch, and the FP64 judge runs on a side stream
 (with a context of its own) concurrently with steps 1-3; the rerun of step 4
@@ -115,7 +117,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
            lam_thr: float = 0.02, bad_lo: float = 0.02, bad_hi: float = 0.98,
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
-           calib_target: float = 0.9995, longest_first: bool = False) -> RefineResult:
+           calib_target: float = 0.9995, longest_first: bool = False,
+           rerun_gather: int = 8) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -242,9 +245,13 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
         cost = cost_t[(j1 * train_n + j2).to(torch.int64)]
         rerun_idx = rerun_idx[torch.sort(cost, descending=True, stable=True).indices].contiguous()
         ctx.set_option("item_order", 1)
+    # the rerun keeps every lane busy for many points: lanes at a
+    # renormalisation wait until rerun_gather lanes of the wave do (same bits)
+    ctx.set_option("lane_gather", rerun_gather)
     try:
         lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
     finally:
+        ctx.set_option("lane_gather", 0)
         if ordered:
             ctx.set_option("item_order", 0)
     t = mark_stream("rerun", t, main)
@@ -276,7 +283,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
                  train_on_lattice=bool(lattice),
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),
-                 rerun_longest_first=bool(ordered),
+                 rerun_longest_first=bool(ordered), rerun_gather=int(rerun_gather),
                  first_kernel_ms=first_kernel_ms, first_points=int(first.numel()),
                  first_acc_steps=float(na_f.double().sum().item()),
                  first_rej_steps=float(nr_f.double().sum().item()),

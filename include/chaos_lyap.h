@@ -239,6 +239,9 @@ int lyap_batch_last_ms(lyap_batch* b, float* ms);
 /* persistent grid size and the number of non-empty classes */
 int lyap_batch_info(lyap_batch* b, int64_t* waves, int* n_classes);
 int lyap_batch_destroy(lyap_batch* b);
+/* points per wave64 of batch layout `layout` for a class of dimension dim
+   (Lorenz-96 N = 8, 16, 32; 64 for the per-lane 3-D classes); host only */
+int lyap_batch_layout_ppw(int layout, int dim, int* ppw);
 
 /*
  * Surrogate network (no counterpart in the reference): (p1, p2) -> 2*n_fourier

@@ -141,6 +141,17 @@ def test_batch_entry_points_reject_null():
     assert b"NULL" in lib.lyap_last_error()
 
 
+def test_batch_layout_points_per_wave():
+    # layout 0: one point per N-lane group (N = 32: one per wave); layout 1: two
+    # vector groups per point for N = 8, 16; the 3-D classes run 64 per wave
+    want = {(0, 8): 8, (0, 16): 4, (0, 32): 1, (1, 8): 4, (1, 16): 2, (1, 32): 1, (0, 3): 64, (1, 3): 64}
+    for (layout, dim), ppw in want.items():
+        assert L.batch_layout_ppw(layout, dim) == ppw
+    with pytest.raises(L.LyapError):
+        L.batch_layout_ppw(2, 16)
+    assert L.load().lyap_batch_layout_ppw(0, 16, None) == L.ERR_INVALID
+
+
 def test_batch_record_layout():
     # lyap_batch_class: the descriptor, then count and six pointers
     assert L.BatchClassRec.count.offset == ctypes.sizeof(L.SweepDesc)

@@ -111,3 +111,27 @@ def test_sample_keeps_a_minimum_total():
     s = A.sample_classes(cls, frac=1 / 16, min_points=1, min_total=2500)
     tot = sum(c["count"] for c in s)
     assert 2400 <= tot <= 2600
+
+
+def test_scaled_classes_shrink_times_and_keep_points():
+    cls = []
+    for c in W.batch_mix(2000, seed=4):
+        cls.append(dict(desc=L.finalize(L.make_desc(c["spec"])), count=c["count"],
+                        params=torch.from_numpy(c["params"]), name=c["name"]))
+    sc = A.scaled_classes(cls, 0.1)
+    for c, s in zip(cls, sc):
+        assert s["count"] == c["count"] and s["params"] is c["params"]
+        assert s["desc"].n_renorm_transient * 10 == c["desc"].n_renorm_transient
+        assert s["desc"].n_renorm_average * 10 == c["desc"].n_renorm_average
+        assert s["desc"].t_renorm == c["desc"].t_renorm and s["desc"].rtol == c["desc"].rtol
+        assert c["desc"].t_average > 1.0  # the originals are untouched
+    seen = []
+
+    def timer(cand, sample):
+        seen.append(sum(x["count"] for x in sample))
+        assert sample[0]["desc"].n_renorm_average * 10 == cls[0]["desc"].n_renorm_average
+        return 1.0 + cand[1]
+
+    r = A.tune_batch(None, cls, cache=A.Cache(None), timer=timer, time_scale=0.1)
+    assert r.best == (0, False, 0, 0, 0)
+    assert seen == [sum(c["count"] for c in cls)] * len(A.BATCH_CANDIDATES)  # every point, every candidate

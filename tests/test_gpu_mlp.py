@@ -81,3 +81,47 @@ def test_training_reduces_loss():
     print("losses", losses, flush=True)
     assert losses[-1] < losses[0] - 1.0
     net.close()
+
+
+def test_adam_step_and_bf16_issue_match_cpu():
+    """One train_step = backward at the current parameters, Adam on every
+    trainable parameter (step 1: m = (1-b1) g, v = (1-b2) g^2, bias-corrected),
+    then the new weights issued to the BF16 copies that the forward and the
+    input-gradient products read.  Pins: the updated parameters against a
+    NumPy Adam of the step's own gradient; the frozen head rows untouched; the
+    forward output and the next gradient at the updated parameters against the
+    CPU reference (a stale or transposed BF16 copy fails them)."""
+    lr, b1, b2, eps = 1e-3, 0.9, 0.999, 1e-8
+    net = L.Mlp(max_batch=256, hidden=256, n_fourier=32, fourier_sigma=4.0, seed=13, ranges=RANGES,
+                y_mean=YM, y_std=YS, lr=lr)
+    p, lam, st = batch(200, 3)
+    pd, ld, sd = dev(p, torch.float64), dev(lam, torch.float64), dev(st.view(np.int32), torch.int32)
+    w0 = net.get(0).astype(np.float64)
+    print("mlp train_step n=200", flush=True)
+    net.train_step(pd, ld, sd)
+    torch.cuda.synchronize()
+    w1 = net.get(0).astype(np.float64)
+    g = net.get(1).astype(np.float64)
+    m = (1 - b1) * g
+    v = (1 - b2) * g * g
+    ref = w0 - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    dims = [64, 256, 256, 256, 16]
+    # the head's padded rows (outputs 5..15) are frozen at zero
+    off = sum(dims[l + 1] * dims[l] + dims[l + 1] for l in range(3))
+    frozen = np.zeros_like(w0, dtype=bool)
+    wh = w0[off:off + 16 * 256].reshape(16, 256)
+    frozen[off:off + 16 * 256] = np.repeat(np.arange(16) >= 5, 256)
+    frozen[off + 16 * 256 + 5:off + 16 * 256 + 16] = True
+    assert np.all(w1[frozen] == w0[frozen]) and np.all(wh[5:] == 0.0)
+    np.testing.assert_allclose(w1[~frozen], ref[~frozen], rtol=0, atol=1e-6 * lr + 1e-7)
+    fourier = net.get(2)
+    out = torch.empty((200, 5), dtype=torch.float32, device="cuda")
+    net.forward(pd, out)
+    loss = torch.zeros(1, dtype=torch.float32, device="cuda")
+    net.backward(pd, ld, sd, loss)
+    torch.cuda.synchronize()
+    l_ref, g_ref, Z = R.forward_backward(net.get(0), fourier, p, lam, st, RANGES, YM, YS, hidden=256)
+    np.testing.assert_allclose(out.cpu().numpy()[:, 0], Z[:, 0] * YS[0] + YM[0], rtol=2e-3, atol=2e-4)
+    g2 = net.get(1).astype(np.float64)
+    assert np.linalg.norm(g2 - g_ref) <= 1e-2 * np.linalg.norm(g_ref)
+    net.close()
