@@ -122,7 +122,7 @@ def bench_sweep(args, ctx, spec, wl_name):
     if desc.model == L.MODEL_L96:
         from chaos_oneapi_amd import autotune as A
 
-        tuned = A.tune_l96(ctx, desc, cache=A.Cache(None))  # sets the tile shape on ctx
+        tuned = A.tune_l96(ctx, desc, cache=A.Cache(None), time_scale=0.1)  # sets the tile shape on ctx
     stream = torch.cuda.current_stream()
     dev = torch.device("cuda", 0)
     bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
@@ -210,10 +210,12 @@ def bench_surrogate(args, ctx, spec, wl_name):
     # per launch over the average launch duration
     per = F.per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
     renorms = F.mgs(desc.dim, desc.dim) * (desc.n_renorm_transient + desc.n_renorm_average)
-    fl = (per[0] * (st["rerun_acc_steps"] + st["first_acc_steps"])
-          + per[1] * (st["rerun_rej_steps"] + st["first_rej_steps"])
-          + renorms * (st["rerun_points"] + st["first_points"]))
+    fl = (per[0] * (st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"])
+          + per[1] * (st["rerun_rej_steps"] + st["first_rej_steps"] + st["overlap_rej_steps"])
+          + renorms * (st["rerun_points"] + st["first_points"] + st["overlap_points"]))
     launch_ms = [st["first_kernel_ms"], st["rerun_kernel_ms"]]
+    if st["overlap_points"]:
+        launch_ms.append(st["overlap_kernel_ms"])
     achieved = fl / (sum(launch_ms) * 1e-3) / 1e12
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
     dl_c = np.abs(res.lam[computed, 0] - lam_p[computed, 0])
@@ -235,13 +237,15 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "computed_points_over_1e-3_vs_plain_fp64": int(np.sum(dl_c > 1e-3)),
         "dlambda1_computed_vs_plain_fp64_p999": float(np.quantile(dl_c, 0.999)) if dl_c.size else 0.0,
         "rerun_lane_gather": st.get("rerun_gather"),
+        "overlap_points": st.get("overlap_points"),
         "class_agreement_full_plane": float(np.mean(cls_final == cls_plain)),
         "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
-    rec["roofline"] = roofline(achieved, "k_var_lane (mixed): first launch + rerun")
+    rec["roofline"] = roofline(achieved, "k_var_lane (mixed): first launch, overlap launch and rerun"
+                               if st["overlap_points"] else "k_var_lane (mixed): first launch + rerun")
     rec["roofline"]["launch_ms"] = launch_ms
-    rec["roofline"]["avg_launch_ms"] = sum(launch_ms) / 2
+    rec["roofline"]["avg_launch_ms"] = sum(launch_ms) / len(launch_ms)
     if not args.no_cpu:
         cpu, parity = cpu_baseline(desc, None, lam_p, args.cpu_budget, args.cpu_threads)
         rec["cpu_baseline"] = cpu

@@ -155,12 +155,8 @@ def scaled_classes(classes, time_scale: float):
     does — a subset of the points would be timed by its few slowest points."""
     out = []
     for c in classes:
-        d = L.SweepDesc.from_buffer_copy(c["desc"])
-        d.t_transient *= time_scale
-        d.t_slave *= time_scale
-        d.t_average *= time_scale
         s = dict(c)
-        s["desc"] = L.finalize(d)
+        s["desc"] = scaled_desc(c["desc"], time_scale)
         out.append(s)
     return out
 
@@ -211,12 +207,24 @@ def batch_kwargs(best) -> dict:
     return dict(layout=layout, occupancy3=bool(occ3), waves_per_cu=wpc, order=order, gather=gather)
 
 
+def scaled_desc(desc, time_scale: float):
+    """desc with the transient and averaging times scaled (re-finalised)."""
+    d = L.SweepDesc.from_buffer_copy(desc)
+    d.t_transient *= time_scale
+    d.t_slave *= time_scale
+    d.t_average *= time_scale
+    return L.finalize(d)
+
+
 def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 16384, cache: Cache | None = None,
-             force: bool = False, apply: bool = True, timer=None) -> TuneResult:
+             force: bool = False, apply: bool = True, timer=None,
+             time_scale: float | None = None) -> TuneResult:
     """Pick the Lorenz-96 tile shape of a single sweep over `desc`'s grid and
-    (apply=True) set it on ctx.  The sample keeps at least min_points points
-    (the whole grid when smaller): fewer than a few per wave slot and the
-    launch is timed by its slowest points, not by the shape."""
+    (apply=True) set it on ctx.  With time_scale the candidates run over the
+    whole grid at that fraction of the integration times (the launch fills
+    the GPU as the full one does); without it, on a sample of at least
+    min_points points (the whole grid when smaller): fewer than a few per
+    wave slot and the launch is timed by its slowest points, not by the shape."""
     import torch
 
     cache = cache if cache is not None else Cache()
@@ -226,7 +234,11 @@ def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 16384, cache: Ca
     if hit and not force:
         res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
     else:
-        idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 3), dtype=torch.int64)
+        if time_scale:
+            run_desc, idx = scaled_desc(desc, time_scale), None
+        else:
+            run_desc = desc
+            idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 3), dtype=torch.int64)
         timings = {}
         for cand in L96_CANDIDATES[desc.dim]:
             if timer:
@@ -234,7 +246,7 @@ def tune_l96(ctx, desc, frac: float = 1 / 16, min_points: int = 16384, cache: Ca
                 continue
             _set_l96(ctx, desc.dim, cand)
             try:
-                ctx.run(desc, idx_list=idx.to(torch.device("cuda", ctx.device)))
+                ctx.run(run_desc, idx_list=None if idx is None else idx.to(torch.device("cuda", ctx.device)))
                 timings[cand] = ctx.kernel_ms()
             finally:
                 _set_l96(ctx, desc.dim, None)

@@ -317,6 +317,17 @@ __global__ __launch_bounds__(64) void k_l96_wave(const lyap_sweep_desc d, const 
     l96_wave_drain<N, P, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, lds);
 }
 
+// the same with a register budget for 3 waves per SIMD: the layouts whose
+// compiler-chosen budget runs 2 (N = 16 with 4 points per wave: 179-187 VGPRs;
+// N = 32 with 1: 183-197) fit 168 with a few spilled registers and run 4-8 %
+// faster; the other layouts already fit
+template <int N, int P, int INTEG, bool ADAPT, bool MIXED>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_l96_wave_w3(
+    const lyap_sweep_desc d, const SweepIO io) {
+    __shared__ double lds[l96_wave_lds_doubles<N, P>()];
+    l96_wave_drain<N, P, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, lds);
+}
+
 namespace {
 
 template <class Kern>
@@ -346,8 +357,12 @@ hipError_t launch_one(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s
 template <int N, int P, int INTEG, bool ADAPT, bool MIXED>
 hipError_t launch_wave_one(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s, int num_cus,
                            int max_blocks) {
-    return launch_persistent_l96(k_l96_wave<N, P, INTEG, ADAPT, MIXED>, 64, d, io, P, s, num_cus,
-                                 max_blocks);
+    if constexpr ((N == 16 && P == 4) || (N == 32 && P == 1))
+        return launch_persistent_l96(k_l96_wave_w3<N, P, INTEG, ADAPT, MIXED>, 64, d, io, P, s, num_cus,
+                                     max_blocks);
+    else
+        return launch_persistent_l96(k_l96_wave<N, P, INTEG, ADAPT, MIXED>, 64, d, io, P, s, num_cus,
+                                     max_blocks);
 }
 
 #define LYAP_L96_VARIANTS(LAUNCH, ...)                                                    \

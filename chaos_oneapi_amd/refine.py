@@ -24,8 +24,10 @@ rerun_gather: the lane_gather option of the rerun (2^20 points of that
 plane: 2,763 ms at 8 against 2,813 ms at 0 and 2,903 ms at 16; bit-identical).
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch, and the FP64 judge runs on a side stream
-(with a context of its own) concurrently with steps 1-3; the rerun of step 4
-skips every point already computed.
+(with a context of its own) concurrently with steps 1-3.  While steps 2-4
+run, a seeded subset of the other points (overlap_points) is computed on
+another side stream with its persistent grid capped so the training kernels
+keep room; the rerun of step 4 skips every point already computed.
 The returned map takes the method's values where it ran and the surrogate's
 elsewhere.  Gates reported on the verification subsample: max |dlambda1|
 between the chosen form and FP64 on computed points, and agreement of the
@@ -118,7 +120,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, longest_first: bool = False,
-           rerun_gather: int = 8) -> RefineResult:
+           rerun_gather: int = 8, overlap_points: int = 1 << 18, overlap_frac: float = 1 / 16,
+           overlap_blocks_per_cu: int = 2) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -183,6 +186,37 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     lam_v, st_v = lam_f[n_t + n_cal:], st_f[n_t + n_cal:]
     t = mark_stream("train_map", t, main)
     first_kernel_ms = ctx.kernel_ms()
+    vmask = torch.zeros(n, dtype=torch.bool, device=dev)
+    vmask[verify] = True
+    cmask = torch.zeros(n, dtype=torch.bool, device=dev)
+    cmask[calib] = True
+    tmask = torch.zeros(n, dtype=torch.bool, device=dev)
+    if lattice:
+        tmask[train_idx] = True
+
+    # 1b. while the surrogate trains, a seeded subset of the other points
+    #     (overlap_frac of them, at most overlap_points) runs on a side stream
+    #     with a context of its own, its persistent grid capped at
+    #     overlap_blocks_per_cu 256-lane blocks per CU so the training kernels
+    #     keep room on every CU.  These points are computed whatever the
+    #     surrogate later says, so the final map does not depend on timing.
+    omask = torch.zeros(n, dtype=torch.bool, device=dev)
+    ov_ctx, ov_idx = None, None
+    rest = torch.nonzero(~(vmask | cmask | tmask)).flatten()
+    n_ov = min(int(overlap_points), int(overlap_frac * rest.numel()))
+    if n_ov > 0 and lattice:
+        go = torch.Generator(device="cpu").manual_seed(seed + 2)
+        pick = torch.randperm(int(rest.numel()), generator=go)[:n_ov].to(dev)
+        ov_idx = rest[pick].sort().values.contiguous()
+        omask[ov_idx] = True
+        ov_ctx = L.Context(ctx.device)
+        cus = torch.cuda.get_device_properties(dev).multi_processor_count
+        ov_ctx.set_option("lane_max_blocks", overlap_blocks_per_cu * cus)
+        ov_ctx.set_option("lane_gather", rerun_gather)
+        side_o = torch.cuda.Stream(dev)
+        side_o.wait_stream(main)
+        with torch.cuda.stream(side_o):
+            lam_o, st_o, na_o, nr_o = ov_ctx.run(fd, idx_list=ov_idx, stream=side_o)
 
     # 2. training (one shuffle per epoch; batches are contiguous slices)
     ok = st_t == 0
@@ -226,14 +260,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     else:
         flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes)
         flagged |= ~torch.isfinite(mu1)
-    vmask = torch.zeros(n, dtype=torch.bool, device=dev)
-    vmask[verify] = True
-    cmask = torch.zeros(n, dtype=torch.bool, device=dev)
-    cmask[calib] = True
-    tmask = torch.zeros(n, dtype=torch.bool, device=dev)
-    if lattice:
-        tmask[train_idx] = True
-    rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask & ~tmask).flatten().to(torch.int64).contiguous()
+    rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask & ~tmask & ~omask).flatten().to(torch.int64).contiguous()
     ordered = longest_first and lattice and rerun_idx.numel() > 0
     if ordered:
         # longest first: each point's expected cost is the step count of its
@@ -258,6 +285,10 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     rerun_kernel_ms = ctx.kernel_ms() if rerun_idx.numel() else 0.0
     torch.cuda.synchronize(dev)
     judge_ctx.close()
+    overlap_kernel_ms = 0.0
+    if ov_ctx is not None:
+        overlap_kernel_ms = ov_ctx.kernel_ms()
+        ov_ctx.close()
 
     # 5. assemble
     lam = pred[:, :3].to(torch.float64)
@@ -272,7 +303,10 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     if lattice:
         lam[train_idx] = lam_t
         status[train_idx] = st_t
-    rerun_mask = flagged | vmask | cmask | tmask
+    if ov_idx is not None:
+        lam[ov_idx] = lam_o
+        status[ov_idx] = st_o
+    rerun_mask = flagged | vmask | cmask | tmask | omask
     tm["total_chosen"] = time.perf_counter() - t_all
 
     n_rerun = int(rerun_mask.sum().item())  # distinct points the method computed
@@ -284,6 +318,10 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),
                  rerun_longest_first=bool(ordered), rerun_gather=int(rerun_gather),
+                 overlap_points=int(ov_idx.numel()) if ov_idx is not None else 0,
+                 overlap_kernel_ms=overlap_kernel_ms,
+                 overlap_acc_steps=float(na_o.double().sum().item()) if ov_idx is not None else 0.0,
+                 overlap_rej_steps=float(nr_o.double().sum().item()) if ov_idx is not None else 0.0,
                  first_kernel_ms=first_kernel_ms, first_points=int(first.numel()),
                  first_acc_steps=float(na_f.double().sum().item()),
                  first_rej_steps=float(nr_f.double().sum().item()),

@@ -135,3 +135,19 @@ def test_scaled_classes_shrink_times_and_keep_points():
     r = A.tune_batch(None, cls, cache=A.Cache(None), timer=timer, time_scale=0.1)
     assert r.best == (0, False, 0, 0, 0)
     assert seen == [sum(c["count"] for c in cls)] * len(A.BATCH_CANDIDATES)  # every point, every candidate
+
+
+def test_tune_l96_time_scaled_runs_the_whole_grid():
+    desc = L.finalize(L.make_desc(W.lorenz96_dopri(n=32, dim=16)))
+    sd = A.scaled_desc(desc, 0.1)
+    assert sd.n_renorm_average * 10 == desc.n_renorm_average
+    assert sd.n_renorm_transient * 10 == desc.n_renorm_transient
+    seen = []
+
+    def timer(cand, idx):
+        seen.append(idx)
+        return 1.0 if cand == ("ppw", 4) else 2.0
+
+    r = A.tune_l96(None, desc, cache=A.Cache(None), timer=timer, time_scale=0.1)
+    assert r.best == ("ppw", 4)
+    assert all(i is None for i in seen)  # no point subset: the scaled whole grid

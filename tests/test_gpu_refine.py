@@ -29,6 +29,15 @@ def test_refinement_gain_and_agreement(gpu_ctx):
     answered = 1.0 - st["rerun_fraction"]
     print(f"surrogate answered {answered:.3%} of the plane (kappa {st['kappa']})", flush=True)
     assert st["train_on_lattice"]
+    # the launch that overlaps training took its seeded 1/16 of the points that
+    # the first launch did not compute (the lattice may share a few points with
+    # the calibration and verification subsamples)
+    first = st["train_points"] + st["verify"] + st["calib"]
+    assert int((st["points"] - first) / 16) <= st["overlap_points"] <= int((st["points"] - st["train_points"]) / 16)
+    assert st["overlap_points"] > 0
+    assert int(res.rerun.sum()) == st["rerun"]
+    assert st["train_points"] + st["overlap_points"] + st["rerun_points"] <= st["rerun"]
+    assert st["rerun"] <= first + st["overlap_points"] + st["rerun_points"]
     assert st["losses"][-1] < st["losses"][0]
     assert st["class_agreement"] >= 0.995
     if st["surrogate_answered_verify"] >= 50:
