@@ -118,11 +118,15 @@ def bench_sweep(args, ctx, spec, wl_name):
 
     desc = L.finalize(L.make_desc(spec))
     n = desc.n1 * desc.n2
-    tuned = None
-    if desc.model == L.MODEL_L96:
-        from chaos_oneapi_amd import autotune as A
+    from chaos_oneapi_amd import autotune as A
 
+    tuned = tuned_lanes = None
+    if desc.model == L.MODEL_L96:
         tuned = A.tune_l96(ctx, desc, cache=A.Cache(None), time_scale=0.1)  # sets the tile shape on ctx
+    elif desc.dim == 3:
+        # persistent grid cap and renormalisation gather, set on ctx for the
+        # chosen form and the plain FP64 sweep alike
+        tuned_lanes = A.tune_lanes(ctx, desc, cache=A.Cache(None), time_scale=0.1)
     stream = torch.cuda.current_stream()
     dev = torch.device("cuda", 0)
     bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
@@ -164,9 +168,13 @@ def bench_sweep(args, ctx, spec, wl_name):
         "mean_steps_per_point": float(na_c.mean()), "mean_rejects_per_point": float(nr_c.mean()),
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
+    if tuned_lanes is not None:
+        rec["config"]["lane_shape"] = {"lane_max_blocks": int(tuned_lanes.best[0]),
+                                       "lane_gather": int(tuned_lanes.best[1])}
+        rec["config"]["lane_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned_lanes.timings.items()}
     if tuned is not None:
         rec["config"]["l96_tile"] = list(tuned.best)
-        rec["config"]["l96_tuning_ms_sample"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}
+        rec["config"]["l96_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}
     kname = "k_var_lane" if desc.model != L.MODEL_L96 else (
         "k_l96_wave" if tuned is not None and tuned.best[0] == "ppw" else "k_l96")
     rec["roofline"] = roofline(achieved, kname + " (mixed) over the grid")

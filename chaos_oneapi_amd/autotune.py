@@ -268,11 +268,13 @@ def _set_l96(ctx, dim: int, cand) -> None:
 
 def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
                cache: Cache | None = None, force: bool = False, apply: bool = True,
-               timer=None) -> TuneResult:
+               timer=None, time_scale: float | None = None) -> TuneResult:
     """Pick the launch shape of a single 3-D sweep and (apply=True) set it on
     ctx: (lane_max_blocks, lane_gather) — the persistent grid cap (0 = every
     block that fits, else that many 256-lane blocks) and how many lanes of a
-    wave gather before renormalising together."""
+    wave gather before renormalising together.  With time_scale the
+    candidates run over the whole grid at that fraction of the integration
+    times; without it, on a sample of the points."""
     import torch
 
     cache = cache if cache is not None else Cache()
@@ -284,16 +286,20 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
     if hit and not force:
         res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
     else:
-        idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 4), dtype=torch.int64)
+        if time_scale:
+            run_desc, idx = scaled_desc(desc, time_scale), None
+        else:
+            run_desc = desc
+            idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 4), dtype=torch.int64)
         timings = {}
-        for cand in ((0, 0), (0, 8), (0, 16), (2 * cus, 0)):
+        for cand in ((0, 0), (0, 8), (0, 16), (2 * cus, 0), (2 * cus, 8)):
             if timer:
                 timings[cand] = timer(cand, idx)
                 continue
             ctx.set_option("lane_max_blocks", cand[0])
             ctx.set_option("lane_gather", cand[1])
             try:
-                ctx.run(desc, idx_list=idx.to(torch.device("cuda", ctx.device)))
+                ctx.run(run_desc, idx_list=None if idx is None else idx.to(torch.device("cuda", ctx.device)))
                 timings[cand] = ctx.kernel_ms()
             finally:
                 ctx.set_option("lane_max_blocks", 0)

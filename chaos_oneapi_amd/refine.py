@@ -23,9 +23,8 @@ it measured no gain (10.40 s against 10.36 s spread over the plane).
 rerun_gather: the lane_gather option of the rerun (2^20 points of that
 plane: 2,763 ms at 8 against 2,813 ms at 0 and 2,903 ms at 16; bit-identical).
 The training map and the calibration and verification runs do not depend on
-the surrogate: they are one launch, and the FP64 judge runs on a side stream
-(with a context of its own) concurrently with steps 1-3.  While steps 2-4
-run, a seeded subset of the other points (overlap_points) is computed on
+the surrogate: they are one launch; the FP64 judge runs on a side stream (with
+a context of its own) beside the rerun of step 4.  While steps 2-4 run, a seeded subset of the other points (overlap_points) is computed on
 another side stream with its persistent grid capped so the training kernels
 keep room; the rerun of step 4 skips every point already computed.
 The returned map takes the method's values where it ran and the surrogate's
@@ -121,7 +120,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, longest_first: bool = False,
            rerun_gather: int = 8, overlap_points: int = 1 << 18, overlap_frac: float = 1 / 16,
-           overlap_blocks_per_cu: int = 2) -> RefineResult:
+           overlap_blocks_per_cu: int = 2, first_blocks_per_cu: int = 2) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -143,8 +142,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     #    lattice (stride s = (n-1) // (train_n-1) >= 1 in both directions), the
     #    training map is that lattice, so the three sets are one chosen-form
     #    launch over the fine grid (one tail) and their values enter the final
-    #    map; the FP64 judge runs concurrently on a side stream with a context
-    #    of its own (its own work counter)
+    #    map
     gv = torch.Generator(device="cpu").manual_seed(seed + 1)
     n_ver = max(1, int(round(verify_frac * n)))
     n_cal = max(1, int(round(calib_frac * n)))
@@ -152,14 +150,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     verify = perm[:n_ver].sort().values.to(dev)
     calib = perm[n_ver:n_ver + n_cal].sort().values.to(dev)
     main = torch.cuda.current_stream(dev)
-    side_j = torch.cuda.Stream(dev)
-    side_j.wait_stream(main)
-    judge_ctx = L.Context(ctx.device)
-    if judge:
-        with torch.cuda.stream(side_j):
-            pdesc = L.SweepDesc.from_buffer_copy(fd)
-            pdesc.precision = L.PREC_FP64
-            lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
+    n_cus = torch.cuda.get_device_properties(dev).multi_processor_count
     s1 = (fd.n1 - 1) // (train_n - 1) if train_n > 1 else 0
     s2 = (fd.n2 - 1) // (train_n - 1) if train_n > 1 else 0
     n_t = train_n * train_n
@@ -168,7 +159,15 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
         ar = torch.arange(train_n, device=dev, dtype=torch.int64)
         train_idx = ((ar * s1)[:, None] * fd.n2 + (ar * s2)[None, :]).reshape(-1)
         first = torch.cat([train_idx, calib, verify]).contiguous()
-        lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
+        # about one point per lane: the launch lasts as long as its slowest
+        # point, and lanes advance faster with fewer waves per SIMD (two
+        # 256-lane blocks per CU: 820 ms against 1,083 ms at three for 149,422
+        # points of the 2048^2 Rossler plane)
+        ctx.set_option("lane_max_blocks", first_blocks_per_cu * n_cus)
+        try:
+            lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
+        finally:
+            ctx.set_option("lane_max_blocks", 0)
         lam_t, st_t = lam_f[:n_t], st_f[:n_t]
         p_t = grid_params(fd, train_idx)
     else:
@@ -210,8 +209,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
         ov_idx = rest[pick].sort().values.contiguous()
         omask[ov_idx] = True
         ov_ctx = L.Context(ctx.device)
-        cus = torch.cuda.get_device_properties(dev).multi_processor_count
-        ov_ctx.set_option("lane_max_blocks", overlap_blocks_per_cu * cus)
+        ov_ctx.set_option("lane_max_blocks", overlap_blocks_per_cu * n_cus)
         ov_ctx.set_option("lane_gather", rerun_gather)
         side_o = torch.cuda.Stream(dev)
         side_o.wait_stream(main)
@@ -272,6 +270,20 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
         cost = cost_t[(j1 * train_n + j2).to(torch.int64)]
         rerun_idx = rerun_idx[torch.sort(cost, descending=True, stable=True).indices].contiguous()
         ctx.set_option("item_order", 1)
+    # the FP64 judge of the verification points: launched just before the
+    # rerun, on a side stream with a context of its own, so its waves take
+    # their slots first and the rerun's persistent grid fills the rest (run
+    # beside the first launch instead, it slowed that latency-bound launch by
+    # ~40 %)
+    judge_ctx = None
+    if judge:
+        side_j = torch.cuda.Stream(dev)
+        side_j.wait_stream(main)
+        judge_ctx = L.Context(ctx.device)
+        with torch.cuda.stream(side_j):
+            pdesc = L.SweepDesc.from_buffer_copy(fd)
+            pdesc.precision = L.PREC_FP64
+            lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
     # the rerun keeps every lane busy for many points: lanes at a
     # renormalisation wait until rerun_gather lanes of the wave do (same bits)
     ctx.set_option("lane_gather", rerun_gather)
@@ -284,7 +296,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     t = mark_stream("rerun", t, main)
     rerun_kernel_ms = ctx.kernel_ms() if rerun_idx.numel() else 0.0
     torch.cuda.synchronize(dev)
-    judge_ctx.close()
+    if judge_ctx is not None:
+        judge_ctx.close()
     overlap_kernel_ms = 0.0
     if ov_ctx is not None:
         overlap_kernel_ms = ov_ctx.kernel_ms()
