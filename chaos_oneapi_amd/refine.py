@@ -24,9 +24,12 @@ rerun_gather: the lane_gather option of the rerun (2^20 points of that
 plane: 2,763 ms at 8 against 2,813 ms at 0 and 2,903 ms at 16; bit-identical).
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch; the FP64 judge runs on a side stream (with
-a context of its own) beside the rerun of step 4.  While steps 2-4 run, a seeded subset of the other points (overlap_points) is computed on
-another side stream with its persistent grid capped so the training kernels
-keep room; the rerun of step 4 skips every point already computed.
+a context of its own) beside the rerun of step 4, which skips every point
+already computed.  Optionally (overlap_points > 0), while steps 2-4 run, a
+seeded subset of the other points is computed on another side stream with
+its persistent grid capped so the training kernels keep room.  Off by
+default: on the 2048^2 Rossler plane it saved at most 0.16 s of 9.3 s and
+made training time erratic (0.99 s, once 1.98 s, against 0.75 s alone).
 The returned map takes the method's values where it ran and the surrogate's
 elsewhere.  Gates reported on the verification subsample: max |dlambda1|
 between the chosen form and FP64 on computed points, and agreement of the
@@ -119,7 +122,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, longest_first: bool = False,
-           rerun_gather: int = 8, overlap_points: int = 1 << 18, overlap_frac: float = 1 / 16,
+           rerun_gather: int = 8, overlap_points: int = 0, overlap_frac: float = 1 / 16,
            overlap_blocks_per_cu: int = 2, first_blocks_per_cu: int = 2) -> RefineResult:
     import torch
 

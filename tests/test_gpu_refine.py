@@ -21,7 +21,9 @@ pytestmark = pytest.mark.gpu
 def test_refinement_gain_and_agreement(gpu_ctx):
     spec = W.rossler_dverk(256, precision=L.PREC_MIXED, t_transient=50.0, t_average=500.0)
     print("case refine rossler 256x256 short times", flush=True)
-    res = R.refine(gpu_ctx, spec, train_n=64, epochs=60, batch=1024, infer_batch=65536)
+    # with the optional launch that overlaps training (a seeded 1/16 of the rest)
+    res = R.refine(gpu_ctx, spec, train_n=64, epochs=60, batch=1024, infer_batch=65536,
+                   overlap_points=1 << 18)
     st = res.stats
     print({k: v for k, v in st.items() if k != "losses"}, flush=True)
     print("timings", res.timings, flush=True)
