@@ -24,8 +24,8 @@ rerun_gather: the lane_gather option of the rerun (2^20 points of that
 plane: 2,763 ms at 8 against 2,813 ms at 0 and 2,903 ms at 16; bit-identical).
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch; the FP64 judge runs on a side stream (with
-a context of its own) beside the rerun of step 4, which skips every point
-already computed.  Optionally (overlap_points > 0), while steps 2-4 run, a
+a context of its own) while the surrogate trains; the rerun of step 4 skips
+every point already computed.  Optionally (overlap_points > 0), while steps 2-4 run, a
 seeded subset of the other points is computed on another side stream with
 its persistent grid capped so the training kernels keep room.  Off by
 default: on the 2048^2 Rossler plane it saved at most 0.16 s of 9.3 s and
@@ -73,7 +73,10 @@ class RefineResult:
     stats: dict = field(default_factory=dict)
 
 
-KAPPAS = (1.0, 1.5, 2.0, 3.0, 5.0, 8.0, 16.0)
+# std multipliers the calibration tries; a fine grid near 1 lets the rule spend
+# the wrong-answer budget instead of stopping at the next coarse step (on the
+# 2048^2 Rossler plane kappa = 1 overshoots it by 2-4x while 1.5 uses a fifth)
+KAPPAS = (1.0, 1.1, 1.25, 1.5, 1.75, 2.0, 3.0, 5.0, 8.0, 16.0)
 
 
 def answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes=True):
@@ -196,6 +199,20 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
     if lattice:
         tmask[train_idx] = True
 
+    # the FP64 judge of the verification points, on a side stream with a
+    # context of its own while the surrogate trains (training's small kernels
+    # leave most of the GPU idle; beside the latency-bound first launch the
+    # judge slowed it by ~40 %, beside the rerun it cost the rerun ~0.3 s)
+    judge_ctx = None
+    if judge:
+        side_j = torch.cuda.Stream(dev)
+        side_j.wait_stream(main)
+        judge_ctx = L.Context(ctx.device)
+        with torch.cuda.stream(side_j):
+            pdesc = L.SweepDesc.from_buffer_copy(fd)
+            pdesc.precision = L.PREC_FP64
+            lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
+
     # 1b. while the surrogate trains, a seeded subset of the other points
     #     (overlap_frac of them, at most overlap_points) runs on a side stream
     #     with a context of its own, its persistent grid capped at
@@ -273,20 +290,6 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 
         cost = cost_t[(j1 * train_n + j2).to(torch.int64)]
         rerun_idx = rerun_idx[torch.sort(cost, descending=True, stable=True).indices].contiguous()
         ctx.set_option("item_order", 1)
-    # the FP64 judge of the verification points: launched just before the
-    # rerun, on a side stream with a context of its own, so its waves take
-    # their slots first and the rerun's persistent grid fills the rest (run
-    # beside the first launch instead, it slowed that latency-bound launch by
-    # ~40 %)
-    judge_ctx = None
-    if judge:
-        side_j = torch.cuda.Stream(dev)
-        side_j.wait_stream(main)
-        judge_ctx = L.Context(ctx.device)
-        with torch.cuda.stream(side_j):
-            pdesc = L.SweepDesc.from_buffer_copy(fd)
-            pdesc.precision = L.PREC_FP64
-            lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
     # the rerun keeps every lane busy for many points: lanes at a
     # renormalisation wait until rerun_gather lanes of the wave do (same bits)
     ctx.set_option("lane_gather", rerun_gather)

@@ -37,10 +37,11 @@ def test_calibration_raises_kappa_until_answers_agree():
     st = np.zeros(n, dtype=np.uint32)
     kappa, escapes, agree, answered, table = R.calibrate_kappa(
         pred_rows(mu1, std, np.zeros(n)), lam, st, target=0.9995, **THR)
-    # kappa 1 answers the wrong points (std in (0.008, 0.01]): ~20 % of them, far
-    # over the budget of 10 wrong answers; 1.5*std <= 0.01 <=> std <= 0.00667 has none
-    assert kappa == 1.5 and agree == 1.0
-    assert answered == int(np.sum(std <= 0.01 / 1.5))
+    # kappa 1 answers the wrong points (std in (0.008, 0.01]) and 1.1 those in
+    # (0.008, 0.0091], far over the budget of 10 wrong answers;
+    # 1.25*std <= 0.01 <=> std <= 0.008 has none
+    assert kappa == 1.25 and agree == 1.0
+    assert answered == int(np.sum(1.25 * std <= 0.01))
     assert len(table) == 2 * len(R.KAPPAS)
 
 
