@@ -8,7 +8,7 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
   rossler-2048-surrogate (default)  Rossler (b=0.2) over a 2048x2048 (a, c) plane,
       adaptive DVERK 6(5) rtol=atol=1e-10, transient 200, averaging 2000, MGS every
       0.5.  A step is the chosen form end to end: a 256x256 training map in the
-      mixed form, surrogate training (BF16 MFMA, Adam, 160 epochs), inference over the plane,
+      mixed form, surrogate training (BF16 MFMA, Adam, 80 epochs of 2048-point batches), inference over the plane,
       mixed-form reruns where the surrogate is uncertain or |mu1| < 0.02, and the
       FP64 judge on a seeded 1 % verification subsample.
   rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
@@ -402,7 +402,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--train-n", type=int, default=256)
-    ap.add_argument("--epochs", type=int, default=160)
+    ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--fourier-sigma", type=float, default=6.0)
     ap.add_argument("--batch-points", type=int, default=1 << 20)
     args = ap.parse_args()

@@ -119,7 +119,7 @@ def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target)
     return best + (table,)
 
 
-def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 160, batch: int = 4096,
+def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2048,
            lr: float = 1e-3, seed: int = 7, verify_frac: float = 0.01, std_thr: float = 0.01,
            lam_thr: float = 0.02, bad_lo: float = 0.02, bad_hi: float = 0.98,
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
