@@ -209,7 +209,10 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                         }
                         const T varg = any ? fma(hT, acc, v0) : v0;
                         const T vp1 = ring.p1(varg), vm1 = ring.m1(varg), vm2 = ring.m2(varg);
-                        kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
+                        if constexpr (MIXED)
+                            kv[s] = fma(vp1 - vm2, jm1[s], fma(-dT, varg, jd[s] * vm1));
+                        else
+                            kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
                     }
                     T accv = T(0);
                     bool anyv = false;

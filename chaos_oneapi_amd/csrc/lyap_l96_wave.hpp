@@ -365,7 +365,12 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
 #pragma unroll
                         for (int q2 = 0; q2 < NV; ++q2) {
                             const T vp1 = ring.p1(varg[q2]), vm1 = ring.m1(varg[q2]), vm2 = ring.m2(varg[q2]);
-                            kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg[q2]));
+                            // binary32: the jd*v(i-1) product first, so its lane move folds
+                            // into a DPP multiply (the FP64 form keeps the fused order)
+                            if constexpr (MIXED)
+                                kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(-dT, varg[q2], jd[s] * vm1));
+                            else
+                                kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg[q2]));
                         }
                     }
 #pragma unroll

@@ -280,7 +280,9 @@ void orc_jvp_f32(int model, int n, const double* xd, const double* pd, const flo
         for (int i = 0; i < n; ++i) {
             int ip1 = (i + 1) % n, im1 = (i + n - 1) % n, im2 = (i + n - 2) % n;
             const float dx = (float)(xd[ip1] - xd[im2]);
-            out[i] = fmaf(v[ip1] - v[im2], x[im1], fmaf(dx, v[im1], -p[1] * v[i]));
+            /* the dx*v(i-1) product rounded first (the kernels fold its lane
+               move into a DPP multiply) */
+            out[i] = fmaf(v[ip1] - v[im2], x[im1], fmaf(-p[1], v[i], dx * v[im1]));
         }
         break;
     case ORC_MODEL_HENON: {
