@@ -32,7 +32,7 @@ constexpr int kOut = 5;       // mu1, mu2, mu3, log-variance, escape logit
 constexpr int kOutPad = 16;   // head rows padded to one MFMA tile
 constexpr int kLayers = 4;
 constexpr int kMaxSplits = 16;  // split-K of the weight-gradient products
-constexpr int kColsumChunks = 32;  // row chunks of the bias-gradient sums
+constexpr int kColsumChunks = 64;  // row chunks of the bias-gradient sums
 
 LYAP_HD uint16_t f2bf(float f) {
     uint32_t u = __builtin_bit_cast(uint32_t, f);
@@ -551,7 +551,7 @@ static void gemm_wgrad(lyap_mlp* m, hipStream_t s, const uint16_t* A, const uint
 // the same layer has consumed split_ws): up to kColsumChunks row chunks summed
 // in parallel, then added in chunk order
 static void colsum_chunks(int n, int* chunks, int* rc) {
-    int c = n / 128;
+    int c = n / 64;  // 64-row chunks: 16 rows per thread
     if (c > kColsumChunks) c = kColsumChunks;
     if (c < 1) c = 1;
     *rc = ((n + c - 1) / c + 3) / 4 * 4;
