@@ -8,9 +8,10 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
   rossler-2048-surrogate (default)  Rossler (b=0.2) over a 2048x2048 (a, c) plane,
       adaptive DVERK 6(5) rtol=atol=1e-10, transient 200, averaging 2000, MGS every
       0.5.  A step is the chosen form end to end: a 256x256 training map in the
-      mixed form, surrogate training (BF16 MFMA, Adam, 80 epochs of 2048-point batches), inference over the plane,
-      mixed-form reruns where the surrogate is uncertain or |mu1| < 0.02, and the
-      FP64 judge on a seeded 1 % verification subsample.
+      mixed form, surrogate training (BF16 MFMA, Adam, 80 epochs of 2048-point
+      batches), inference over the plane, mixed-form reruns where the surrogate
+      is uncertain or |mu1| < 0.02, and the FP64 judge on a seeded 1 %
+      verification subsample.
   rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
   lorenz-64     Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01; one mixed-form sweep.
   lorenz96-16   Lorenz-96 N=16 on a 256x256 (F, d) grid, adaptive Dopri5 rtol=1e-9; the
@@ -213,9 +214,10 @@ def bench_surrogate(args, ctx, spec, wl_name):
     cls_final = R.classify(np.nan_to_num(res.lam[:, 0], nan=0.0), res.status)
     cls_plain = R.classify(lam_p[:, 0], st_p)
     computed = res.rerun & okp & (res.status == 0)
-    # the step's two mixed-form launches of k_var_lane (the first launch and the
-    # rerun): total algorithmic flops over total duration = the average flops
-    # per launch over the average launch duration
+    # the step's mixed-form launches of k_var_lane (the first launch and the
+    # rerun, and the optional launch beside training): total algorithmic flops
+    # over total duration = the average flops per launch over the average
+    # launch duration
     per = F.per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
     renorms = F.mgs(desc.dim, desc.dim) * (desc.n_renorm_transient + desc.n_renorm_average)
     fl = (per[0] * (st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"])
