@@ -48,6 +48,39 @@ LYAP_HD float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b <<
 // the accumulator element i of lane l is C[m0 + 4(l/16) + i][n0 + 16j + l%16].
 // M, N multiples of 16 (of 16*NB for N), K a multiple of 16.
 // ---------------------------------------------------------------------------
+// acc[j] += A-tile . B-tile_j over k in [k_lo, k_hi): four k-steps' loads are
+// issued before their MFMAs (one step's loads would otherwise wait out the L2
+// latency alone), then a step at a time for the rest; each accumulator still
+// takes the k-steps in increasing order
+template <int NB>
+__device__ __forceinline__ void mfma_k_range(const uint16_t* __restrict__ a_row,
+                                             const uint16_t* __restrict__ b_base, int K, int k_lo,
+                                             int k_hi, v4f (&acc)[NB]) {
+    int k0 = k_lo;
+    for (; k0 + 64 <= k_hi; k0 += 64) {
+        v4s a[4], b[4][NB];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a[u] = *reinterpret_cast<const v4s*>(a_row + k0 + 16 * u);
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                b[u][j] = *reinterpret_cast<const v4s*>(b_base + (size_t)(16 * j) * K + k0 + 16 * u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int j = 0; j < NB; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][j], acc[j], 0, 0, 0);
+    }
+    for (; k0 < k_hi; k0 += 16) {
+        const v4s a = *reinterpret_cast<const v4s*>(a_row + k0);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const v4s b = *reinterpret_cast<const v4s*>(b_base + (size_t)(16 * j) * K + k0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+        }
+    }
+}
+
 template <int NB>
 __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
                                                  const uint16_t* __restrict__ B, float* __restrict__ C,
@@ -64,14 +97,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
     const uint16_t* a_row = A + (size_t)(m0 + r) * K + 4 * q;
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        const v4s a = *reinterpret_cast<const v4s*>(a_row + k0);
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const v4s b = *reinterpret_cast<const v4s*>(B + (size_t)(n0 + 16 * j + r) * K + 4 * q + k0);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-        }
-    }
+    mfma_k_range<NB>(a_row, B + (size_t)(n0 + r) * K + 4 * q, K, 0, K, acc);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const int col = n0 + 16 * j + r;
@@ -106,14 +132,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt_splitk(const uint16_t* __restri
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
     const uint16_t* a_row = A + (size_t)(m0 + r) * K + 4 * q;
-    for (int k0 = k_lo; k0 < k_hi; k0 += 16) {
-        const v4s a = *reinterpret_cast<const v4s*>(a_row + k0);
-#pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const v4s b = *reinterpret_cast<const v4s*>(B + (size_t)(n0 + 16 * j + r) * K + 4 * q + k0);
-            acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
-        }
-    }
+    mfma_k_range<NB>(a_row, B + (size_t)(n0 + r) * K + 4 * q, K, k_lo, k_hi, acc);
     float* Ps = P + (size_t)split * M * N;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
