@@ -520,15 +520,17 @@ struct Ctrl {
 // in the equivalent form h_new = h * r, r = clamp(0.9 * exp(-(expo*lerr -
 // beta*log facOld)), 0.2, 10): one exponential and no division.
 // ok: err <= 1; above_floor: err > 1e-4; lerr = log(err) (ctrl_log);
-// log_floor = ctrl_log(1e-4).
+// log_floor = ctrl_log(1e-4).  Both outcomes share one ctrl_exp call on the
+// selected argument, so a wave whose lanes disagree evaluates it once.
 template <int INTEG>
 LYAP_D bool ctrl_update(Ctrl& c, bool ok, bool above_floor, double lerr, double log_floor, double h_try,
                         double h_max, double& h_next) {
     constexpr double kSafe = 0.9, kBeta = 0.04;
+    const double e_rej = Tab<INTEG>::expo * lerr;
+    const double e = ok ? fma(-kBeta, c.log_fac_old, e_rej) : e_rej;
+    double r = kSafe * ctrl_exp(-e);
+    r = r < 0.2 ? 0.2 : r;
     if (ok) {
-        const double e = fma(-kBeta, c.log_fac_old, Tab<INTEG>::expo * lerr);
-        double r = kSafe * ctrl_exp(-e);
-        r = r < 0.2 ? 0.2 : r;
         r = r > 10.0 ? 10.0 : r;
         double hn = h_try * r;
         c.log_fac_old = above_floor ? lerr : log_floor;
@@ -538,8 +540,6 @@ LYAP_D bool ctrl_update(Ctrl& c, bool ok, bool above_floor, double lerr, double 
         h_next = hn;
         return true;
     }
-    double r = kSafe * ctrl_exp(-(Tab<INTEG>::expo * lerr));
-    r = r < 0.2 ? 0.2 : r;
     c.rejected = true;
     h_next = h_try * r;
     return false;

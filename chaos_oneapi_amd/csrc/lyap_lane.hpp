@@ -210,6 +210,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     int lcnt = 0;  // renormalisations in the open log window
     float pf[P];
     double t = 0.0, h = 0.0;
+    double t_next = 0.0;  // (m + 1) * t_renorm: the next renormalisation time
     int64_t m = 0, step_in = 0, item = -1;
     uint32_t nacc = 0, nrej = 0;
     Ctrl ctrl{log_floor, false};
@@ -268,6 +269,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 t = 0.0;
                 h = dp->h;
                 m = 0;
+                t_next = t_renorm;
                 step_in = 0;
                 nacc = nrej = 0;
                 ctrl.log_fac_old = log_floor;
@@ -288,6 +290,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                         double nrm[K];
                         mgs_rows<K, N>(V, nrm);
                         ++m;
+                        t_next = (double)(m + 1) * t_renorm;
                         if (m > n_tr) {
 #pragma unroll
                             for (int r = 0; r < K; ++r) lognorm_push(lmant[r], lex[r], nrm[r]);
@@ -317,11 +320,16 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 st = LYAP_PT_MAXSTEPS;
                 finished = true;
             } else {
-                const double t_next = (double)(m + 1) * t_renorm;
                 double h_try = h;
                 bool clipped = false;
                 const double jn = M::jnorm(x, p);
-                if (h_try * jn > 1.0) h_try = 1.0 / jn;
+                // the division only when a lane of the wave needs the cap (the
+                // empty asm keeps it from being hoisted above the branch)
+                if (__any(h_try * jn > 1.0)) {
+                    double jv = jn;
+                    asm volatile("" : "+v"(jv));
+                    if (h_try * jn > 1.0) h_try = 1.0 / jv;
+                }
                 // land on the boundary when it is within 1/16 of the step, so no
                 // round-off sliver is left before it
                 if (t_next - t <= 1.0625 * h_try) {
