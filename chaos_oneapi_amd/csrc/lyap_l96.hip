@@ -267,9 +267,8 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                     double s2 = vr[0] * vr[0];
 #pragma unroll
                     for (int c = 1; c < N; ++c) s2 = fma(vr[c], vr[c], s2);
-                    const double nrm = sqrt(s2);
-                    vn = nrm;  // accumulated after the loop, by every row lane at once
-                    const double inv = 1.0 / nrm;
+                    const double inv = inv_sqrt_det(s2);
+                    vn = s2 * inv;  // accumulated after the loop, by every row lane at once
 #pragma unroll
                     for (int c = 0; c < N; ++c) {
                         vr[c] = vr[c] * inv;

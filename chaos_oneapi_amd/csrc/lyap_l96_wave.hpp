@@ -419,9 +419,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                             double s2 = row[0] * row[0];
 #pragma unroll 8
                             for (int c = 1; c < N; ++c) s2 = fma(row[c], row[c], s2);
-                            const double nrm = sqrt(s2);
-                            vn = nrm;  // accumulated after the loop, by every row lane at once
-                            const double inv = 1.0 / nrm;
+                            const double inv = inv_sqrt_det(s2);
+                            vn = s2 * inv;  // accumulated after the loop, by every row lane at once
 #pragma unroll 8
                             for (int c = 0; c < N; ++c) row[c] = row[c] * inv;
                         }

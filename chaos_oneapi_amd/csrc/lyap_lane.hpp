@@ -120,9 +120,8 @@ LYAP_D void mgs_rows(double (&V)[K][N], double* nrm_out) {
         double s = V[j][0] * V[j][0];
 #pragma unroll
         for (int i = 1; i < N; ++i) s = fma(V[j][i], V[j][i], s);
-        const double nrm = sqrt(s);
-        nrm_out[j] = nrm;
-        const double inv = 1.0 / nrm;
+        const double inv = inv_sqrt_det(s);
+        nrm_out[j] = s * inv;
 #pragma unroll
         for (int i = 0; i < N; ++i) V[j][i] = V[j][i] * inv;
     }

@@ -511,8 +511,21 @@ static double dot_ns(int n, int tree, const double* a, const double* b) {
     return s;
 }
 
-/* modified Gram-Schmidt on k row vectors of length n; rows end unit length;
-   norms[j] = row j's norm before its normalisation */
+/* 1/sqrt(s) of the sweep's renormalisations: the binary64 bit-pattern seed
+   of the classic inverse-square-root trick (relative error below 3.5 %) and
+   four Newton steps y <- y*(3/2 - (s/2)*y^2); within ~1.2 ulp of 1/sqrt(s)
+   (pinned against sqrt in tests/test_oracle_pins.py) and made of IEEE
+   multiplies and fmas only, so the GPU gives the same bits. */
+double orc_inv_sqrt(double s) {
+    double y = as_f64(0x5FE6EB50C7B537A9ULL - (as_u64(s) >> 1));
+    const double hs = 0.5 * s;
+    for (int it = 0; it < 4; ++it) y = y * fma(-hs, y * y, 1.5);
+    return y;
+}
+
+/* modified Gram-Schmidt on k row vectors of length n; rows end unit length
+   (scaled by orc_inv_sqrt of the squared norm); norms[j] = row j's norm
+   before its normalisation, s * orc_inv_sqrt(s) */
 static void mgs_norms(int n, int k, int tree, double* V, double* norms) {
     for (int j = 0; j < k; ++j) {
         double* vj = V + j * n;
@@ -521,9 +534,9 @@ static void mgs_norms(int n, int k, int tree, double* V, double* norms) {
             double c = dot_ns(n, tree, vr, vj);
             for (int i = 0; i < n; ++i) vj[i] = fma(-c, vr[i], vj[i]);
         }
-        double nrm = sqrt(dot_ns(n, tree, vj, vj));
-        norms[j] = nrm;
-        double inv = 1.0 / nrm;
+        const double s2 = dot_ns(n, tree, vj, vj);
+        const double inv = orc_inv_sqrt(s2);
+        norms[j] = s2 * inv;
         for (int i = 0; i < n; ++i) vj[i] = vj[i] * inv;
     }
 }

@@ -104,12 +104,14 @@ void orc_frozen_tangent_step(int model, int n, const double* x_frozen, const dou
 
 /* orthogonalisation kernels */
 void orc_mgs(int n, int k, int tree, double* V /* k rows of n */, double* log_norms);
+/* 1/sqrt(s) of the renormalisations (seed + four Newton steps, IEEE ops only) */
+double orc_inv_sqrt(double s);
 /* log-norm accumulation of the sweep's method: norms of up to ORC_LOG_WINDOW
    consecutive renormalisations multiply into mant * 2^ex (mant in [1, 2),
    orc_split_exponent moves the binary exponent of a positive normal value
    into *ex), and log(mant * 2^ex) = orc_log_scaled(mant, ex) of each window
    is Kahan-summed */
-#define ORC_LOG_WINDOW 16
+#define ORC_LOG_WINDOW 64
 double orc_split_exponent(double m, int32_t* ex);
 double orc_log_scaled(double mant, int32_t ex);
 void orc_cgs_ref(int n, int k, double* V /* k rows of n */);

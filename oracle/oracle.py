@@ -91,6 +91,8 @@ def lib():
         L.orc_split_exponent.argtypes = [d, ctypes.POINTER(ctypes.c_int32)]
         L.orc_log_scaled.restype = d
         L.orc_log_scaled.argtypes = [d, ctypes.c_int32]
+        L.orc_inv_sqrt.restype = d
+        L.orc_inv_sqrt.argtypes = [d]
         L.orc_cgs_ref.argtypes = [i, i, P]
         L.orc_normalize_ref.argtypes = [i, i, P, d]
         L.orc_qr_ref.argtypes = [i, P, P]
@@ -178,7 +180,7 @@ def mgs(V, tree=False):
     return V, ln
 
 
-LOG_WINDOW = 16  # ORC_LOG_WINDOW
+LOG_WINDOW = 64  # ORC_LOG_WINDOW
 
 
 def split_exponent(m, ex=0):

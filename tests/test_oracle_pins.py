@@ -255,6 +255,24 @@ def test_frozen_tangent_is_dverk_on_linear_system(model, p, n, f):
 
 
 # ---------------------------------------------------------------- orthogonalisation
+def test_inverse_square_root_of_the_renormalisation():
+    """orc_inv_sqrt (seed + four Newton steps) against libm's correctly rounded
+    sqrt: within 3 * 2^-52 relative over squared norms from 1e-60 to 1e60 and
+    exact on powers of four; three steps would leave ~3e-11, a wrong seed shift
+    far more."""
+    f = O.lib().orc_inv_sqrt
+    s = np.exp(rng.uniform(-138.0, 138.0, 20000))
+    s = np.concatenate([s, [1.0, 4.0, 0.25, 2.0, 3.0, 9.0, 1e-300 * 1e10, 7.0 ** 2]])
+    y = np.array([f(float(v)) for v in s])
+    ref = 1.0 / np.sqrt(s)
+    rel = np.abs(y - ref) / ref
+    assert rel.max() <= 3 * 2.0 ** -52
+    # the norm the renormalisation records, s * y, against sqrt(s)
+    nrm = s * y
+    assert (np.abs(nrm - np.sqrt(s)) / np.sqrt(s)).max() <= 3 * 2.0 ** -52
+    assert f(4.0) == 0.5 and f(0.25) == 2.0
+
+
 def test_mgs_matches_numpy_qr():
     for n, k in [(3, 3), (8, 8), (16, 16), (32, 32), (5, 3)]:
         A = rng.normal(size=(n, k))
