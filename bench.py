@@ -214,8 +214,9 @@ def bench_surrogate(args, ctx, spec, wl_name):
     cls_final = R.classify(np.nan_to_num(res.lam[:, 0], nan=0.0), res.status)
     cls_plain = R.classify(lam_p[:, 0], st_p)
     computed = res.rerun & okp & (res.status == 0)
-    # the step's mixed-form launches of k_var_lane (the first launch and the
-    # rerun, and the optional launch beside training): total algorithmic flops
+    # the step's mixed-form launches of k_var_lane (the lattice launch, the
+    # calibration and verification launch beside it, the rerun, and the optional
+    # launch beside training): total algorithmic flops
     # over total duration = the average flops per launch over the average
     # launch duration
     per = F.per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
@@ -223,7 +224,10 @@ def bench_surrogate(args, ctx, spec, wl_name):
     fl = (per[0] * (st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"])
           + per[1] * (st["rerun_rej_steps"] + st["first_rej_steps"] + st["overlap_rej_steps"])
           + renorms * (st["rerun_points"] + st["first_points"] + st["overlap_points"]))
-    launch_ms = [st["first_kernel_ms"], st["rerun_kernel_ms"]]
+    launch_ms = [st["first_kernel_ms"]]
+    if st.get("split_first"):
+        launch_ms.append(st["split_kernel_ms"])
+    launch_ms.append(st["rerun_kernel_ms"])
     if st["overlap_points"]:
         launch_ms.append(st["overlap_kernel_ms"])
     achieved = fl / (sum(launch_ms) * 1e-3) / 1e12
@@ -252,8 +256,12 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
-    rec["roofline"] = roofline(achieved, "k_var_lane (mixed): first launch, overlap launch and rerun"
-                               if st["overlap_points"] else "k_var_lane (mixed): first launch + rerun")
+    parts = ["lattice launch", "calibration/verification launch beside it"] if st.get("split_first") \
+        else ["first launch"]
+    if st["overlap_points"]:
+        parts.append("overlap launch")
+    parts.append("rerun")
+    rec["roofline"] = roofline(achieved, "k_var_lane (mixed): " + ", ".join(parts))
     rec["roofline"]["launch_ms"] = launch_ms
     rec["roofline"]["avg_launch_ms"] = sum(launch_ms) / len(launch_ms)
     if not args.no_cpu:

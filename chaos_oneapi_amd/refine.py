@@ -21,7 +21,10 @@ the kernel takes the sorted list in order (context option item_order).  Off
 by default: on the 2048^2 Rossler plane (3.6 M rerun points, ~18 per lane)
 it measured no gain (10.40 s against 10.36 s spread over the plane).
 rerun_gather: the lane_gather option of the rerun (2^20 points of that
-plane: 2,763 ms at 8 against 2,813 ms at 0 and 2,903 ms at 16; bit-identical).
+plane: 2,537 ms at 4 against 2,564 at 0, 2,579 at 2, 2,561 at 6, 2,591-2,620
+at 8 and 2,685 at 12; bit-identical).  With the renormalisation's Newton
+inverse square root and 64-renormalisation log windows the block a wave runs
+for its gathered lanes is cheap, so little gathering pays.
 The training map and the calibration and verification runs do not depend on
 the surrogate: they are one launch; the FP64 judge runs on a side stream (with
 a context of its own) while the surrogate trains; the rerun of step 4 skips
@@ -128,8 +131,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, longest_first: bool = False,
-           rerun_gather: int = 8, overlap_points: int = 0, overlap_frac: float = 1 / 16,
-           overlap_blocks_per_cu: int = 2, first_blocks_per_cu: int = 2) -> RefineResult:
+           rerun_gather: int = 4, overlap_points: int = 0, overlap_frac: float = 1 / 16,
+           overlap_blocks_per_cu: int = 2, first_blocks_per_cu: int = 2,
+           split_first: bool = True, split_blocks_per_cu: int = 1) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -164,20 +168,42 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     s2 = (fd.n2 - 1) // (train_n - 1) if train_n > 1 else 0
     n_t = train_n * train_n
     lattice = s1 >= 1 and s2 >= 1
+    cv_ctx, side_c = None, None
     if lattice:
         ar = torch.arange(train_n, device=dev, dtype=torch.int64)
         train_idx = ((ar * s1)[:, None] * fd.n2 + (ar * s2)[None, :]).reshape(-1)
         first = torch.cat([train_idx, calib, verify]).contiguous()
-        # about one point per lane: the launch lasts as long as its slowest
-        # point, and lanes advance faster with fewer waves per SIMD (two
-        # 256-lane blocks per CU: 820 ms against 1,083 ms at three for 149,422
-        # points of the 2048^2 Rossler plane)
-        ctx.set_option("lane_max_blocks", first_blocks_per_cu * n_cus)
-        try:
-            lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
-        finally:
-            ctx.set_option("lane_max_blocks", 0)
-        lam_t, st_t = lam_f[:n_t], st_f[:n_t]
+        if split_first:
+            # training needs only the lattice: it runs on the main stream while
+            # the calibration and verification points run beside it on a side
+            # stream with a context of its own (needed only after training),
+            # each persistent grid capped at split_blocks_per_cu 256-lane blocks
+            # per CU (on the 2048^2 Rossler plane the lattice alone takes 594 ms
+            # where the three sets in one launch take 819 ms)
+            cv = first[n_t:]
+            side_c = torch.cuda.Stream(dev)
+            side_c.wait_stream(main)
+            cv_ctx = L.Context(ctx.device)
+            cv_ctx.set_option("lane_max_blocks", split_blocks_per_cu * n_cus)
+            ctx.set_option("lane_max_blocks", split_blocks_per_cu * n_cus)
+            try:
+                lam_t, st_t, na_t, nr_t = ctx.run(fd, idx_list=train_idx)
+            finally:
+                ctx.set_option("lane_max_blocks", 0)
+            with torch.cuda.stream(side_c):
+                lam_cv, st_cv, na_cv, nr_cv = cv_ctx.run(fd, idx_list=cv, stream=side_c)
+            lam_f, st_f, na_f, nr_f = None, None, na_t, nr_t
+        else:
+            # about one point per lane: the launch lasts as long as its slowest
+            # point, and lanes advance faster with fewer waves per SIMD (two
+            # 256-lane blocks per CU: 820 ms against 1,083 ms at three for
+            # 149,422 points of the 2048^2 Rossler plane)
+            ctx.set_option("lane_max_blocks", first_blocks_per_cu * n_cus)
+            try:
+                lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
+            finally:
+                ctx.set_option("lane_max_blocks", 0)
+            lam_t, st_t = lam_f[:n_t], st_f[:n_t]
         p_t = grid_params(fd, train_idx)
     else:
         tspec = dict(fine)
@@ -190,8 +216,12 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
         lam_f = torch.cat([lam_t, lam_f])
         st_f = torch.cat([st_t, st_f])
-    lam_c, st_c = lam_f[n_t:n_t + n_cal], st_f[n_t:n_t + n_cal]
-    lam_v, st_v = lam_f[n_t + n_cal:], st_f[n_t + n_cal:]
+    if side_c is None:
+        lam_c, st_c = lam_f[n_t:n_t + n_cal], st_f[n_t:n_t + n_cal]
+        lam_v, st_v = lam_f[n_t + n_cal:], st_f[n_t + n_cal:]
+    else:
+        lam_c, st_c = lam_cv[:n_cal], st_cv[:n_cal]
+        lam_v, st_v = lam_cv[n_cal:], st_cv[n_cal:]
     t = mark_stream("train_map", t, main)
     first_kernel_ms = ctx.kernel_ms()
     vmask = torch.zeros(n, dtype=torch.bool, device=dev)
@@ -271,6 +301,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     #    calibration subsample agree with the method (the verification
     #    subsample stays out of it), then the rerun set: every point the
     #    surrogate does not answer and that step 1 did not compute
+    if side_c is not None:
+        main.wait_stream(side_c)
     kappa, escapes, cal_agree, cal_answered, cal_table = calibrate_kappa(
         pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
         bad_lo, bad_hi, calib_target)
@@ -307,6 +339,11 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     torch.cuda.synchronize(dev)
     if judge_ctx is not None:
         judge_ctx.close()
+    split_kernel_ms = 0.0
+    if cv_ctx is not None:
+        split_kernel_ms = cv_ctx.kernel_ms()
+        cv_ctx.close()
+        na_f, nr_f = torch.cat([na_t, na_cv]), torch.cat([nr_t, nr_cv])
     overlap_kernel_ms = 0.0
     if ov_ctx is not None:
         overlap_kernel_ms = ov_ctx.kernel_ms()
@@ -345,6 +382,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
                  overlap_acc_steps=float(na_o.double().sum().item()) if ov_idx is not None else 0.0,
                  overlap_rej_steps=float(nr_o.double().sum().item()) if ov_idx is not None else 0.0,
                  first_kernel_ms=first_kernel_ms, first_points=int(first.numel()),
+                 split_first=cv_ctx is not None, split_kernel_ms=split_kernel_ms,
                  first_acc_steps=float(na_f.double().sum().item()),
                  first_rej_steps=float(nr_f.double().sum().item()),
                  rerun_acc_steps=float(na_r.double().sum().item()),
