@@ -81,11 +81,16 @@ __device__ __forceinline__ void mfma_k_range(const uint16_t* __restrict__ a_row,
     }
 }
 
+__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
+
+// With Hr (forward products of the hidden layers) the epilogue also stores
+// H = gelu(C) in row-major Hr[M][N] and column-major Ht[N][M] BF16.
 template <int NB>
 __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
                                                  const uint16_t* __restrict__ B, float* __restrict__ C,
                                                  const float* __restrict__ bias, int M, int N, int K,
-                                                 float beta) {
+                                                 float beta, uint16_t* __restrict__ Hr,
+                                                 uint16_t* __restrict__ Ht) {
     const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63);
     const int tiles_n = N / (16 * NB);
@@ -105,102 +110,125 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             float* c = C + (size_t)(m0 + 4 * q + i) * N + col;
-            *c = (beta != 0.f ? beta * *c : 0.f) + acc[j][i] + bv;
+            const float z = (beta != 0.f ? beta * *c : 0.f) + acc[j][i] + bv;
+            *c = z;
+            if (Hr) {
+                const int row = m0 + 4 * q + i;
+                const uint16_t h = f2bf(gelu(z));
+                Hr[(size_t)row * N + col] = h;
+                Ht[(size_t)col * M + row] = h;
+            }
         }
     }
 }
 
-// Split-K form for the weight gradients (M, N <= 256, K = the batch): split
-// s of the waves computes C_s over k in [s*kc, min(K, (s+1)*kc)) into
-// P + s*M*N; k_sum_splits adds the splits in order (deterministic).
+// Weight gradient (small M x N, K = the batch) with the split-K inside one
+// workgroup: wave s of block t computes tile t over k in [s*kc, min(K,
+// (s+1)*kc)); the partial tiles meet in LDS and are added in split order, the
+// same sums as separate split-K partial products added in order.  With bias_part
+// ([chunks][M] row-chunk sums of the layer's dZ), the blocks of the first tile
+// column also add the bias gradient of their 16 rows in chunk order.
 template <int NB>
-__global__ __launch_bounds__(256) void k_gemm_nt_splitk(const uint16_t* __restrict__ A,
-                                                        const uint16_t* __restrict__ B,
-                                                        float* __restrict__ P, int M, int N, int K,
-                                                        int kc, int splits) {
-    const int wave = (int)(blockIdx.x * 4 + (threadIdx.x >> 6));
+__global__ __launch_bounds__(1024) void k_wgrad(const uint16_t* __restrict__ A,
+                                                const uint16_t* __restrict__ B, float* __restrict__ C,
+                                                int M, int N, int K, int kc, int splits,
+                                                const float* __restrict__ bias_part, int chunks,
+                                                float* __restrict__ db) {
+    extern __shared__ float part[];  // [splits][16][16 * NB], then [kColsumChunks][16] bias chunk sums
+    constexpr int TW = 16 * NB;
+    const int split = (int)(threadIdx.x >> 6);
     const int lane = (int)(threadIdx.x & 63);
-    const int tiles_n = N / (16 * NB);
-    const int tiles = (M / 16) * tiles_n;
-    const int split = wave / tiles, t = wave % tiles;
-    if (split >= splits) return;
+    const int tiles_n = N / TW;
+    const int t = (int)blockIdx.x;
     const int m0 = (t / tiles_n) * 16;
-    const int n0 = (t % tiles_n) * (16 * NB);
-    const int k_lo = split * kc, k_hi = min(K, k_lo + kc);
+    const int n0 = (t % tiles_n) * TW;
     const int r = lane & 15, q = lane >> 4;
+    // the bias chunk sums of rows m0 .. m0+15 are fetched before the products
+    // (their latency hides behind the MFMA loop) and parked in LDS after them
+    const bool do_bias = bias_part != nullptr && n0 == 0;
+    const int nb_vals = 16 * chunks;
+    float bv[2] = {0.f, 0.f};
+    if (do_bias) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = (int)threadIdx.x + u * (int)blockDim.x;
+            if (e < nb_vals) bv[u] = bias_part[(size_t)(e >> 4) * M + m0 + (e & 15)];
+        }
+    }
     v4f acc[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* a_row = A + (size_t)(m0 + r) * K + 4 * q;
-    mfma_k_range<NB>(a_row, B + (size_t)(n0 + r) * K + 4 * q, K, k_lo, k_hi, acc);
-    float* Ps = P + (size_t)split * M * N;
+    const int k_lo = split * kc, k_hi = min(K, k_lo + kc);
+    mfma_k_range<NB>(A + (size_t)(m0 + r) * K + 4 * q, B + (size_t)(n0 + r) * K + 4 * q, K, k_lo, k_hi, acc);
+    float* P = part + split * 16 * TW;
 #pragma unroll
-    for (int j = 0; j < NB; ++j) {
-        const int col = n0 + 16 * j + r;
+    for (int j = 0; j < NB; ++j)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) Ps[(size_t)(m0 + 4 * q + i) * N + col] = acc[j][i];
+        for (int i = 0; i < 4; ++i) P[(4 * q + i) * TW + 16 * j + r] = acc[j][i];
+    float* bl = part + splits * 16 * TW;
+    if (do_bias) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int e = (int)threadIdx.x + u * (int)blockDim.x;
+            if (e < nb_vals) bl[e] = bv[u];
+        }
+        // more chunk sums than two per thread (few splits): the rest directly
+        for (int e = (int)threadIdx.x + 2 * (int)blockDim.x; e < nb_vals; e += (int)blockDim.x)
+            bl[e] = bias_part[(size_t)(e >> 4) * M + m0 + (e & 15)];
+    }
+    __syncthreads();
+    for (int e = (int)threadIdx.x; e < 16 * TW; e += (int)blockDim.x) {
+        float s = part[e];
+        for (int k = 1; k < splits; ++k) s += part[k * 16 * TW + e];
+        C[(size_t)(m0 + e / TW) * N + n0 + e % TW] = s;
+    }
+    if (do_bias && threadIdx.x < 16) {
+        float s = bl[threadIdx.x];
+        for (int k = 1; k < chunks; ++k) s += bl[k * 16 + threadIdx.x];
+        db[m0 + threadIdx.x] = s;
     }
 }
 
-__global__ void k_sum_splits(const float* __restrict__ P, int splits, int64_t mn, float* __restrict__ C) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= mn) return;
-    float s = P[i];
-    for (int k = 1; k < splits; ++k) s += P[(size_t)k * mn + i];
-    C[i] = s;
-}
 
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.f + erff(x * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad(float x) {
     return 0.5f * (1.f + erff(x * 0.70710678118654752f)) +
            x * 0.39894228040143268f * expf(-0.5f * x * x);
 }
 
-// features[b][j] = sin(2 pi (B_j . u)), features[b][F+j] = cos(...); row- and column-major bf16
-// (loss, when given, is cleared for the loss reduction of the same pass)
+// features[b][j] = sin(2 pi (B_j . u)), features[b][F+j] = cos(...); row- and column-major bf16.
+// One thread per (feature j, row b), rows fastest (the column-major stores
+// coalesce).  (loss, when given, is cleared for the loss reduction of the same pass)
 __global__ void k_fourier(const double* __restrict__ p, int n, int n_valid, const float* __restrict__ fb,
                           int F, double lo1, double inv1, double lo2, double inv2,
                           uint16_t* __restrict__ X, uint16_t* __restrict__ Xt, float* __restrict__ loss) {
-    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (loss && b == 0) *loss = 0.f;
-    if (b >= n) return;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (loss && idx == 0) *loss = 0.f;
+    if (idx >= (int64_t)n * F) return;
+    const int j = (int)(idx / n), b = (int)(idx % n);
     float u0 = 0.f, u1 = 0.f;
     if (b < n_valid) {
         u0 = (float)((p[2 * b] - lo1) * inv1);
         u1 = (float)((p[2 * b + 1] - lo2) * inv2);
     }
-    for (int j = 0; j < F; ++j) {
-        const float f = 6.283185307179586f * (fb[2 * j] * u0 + fb[2 * j + 1] * u1);
-        float s, c;
-        sincosf(f, &s, &c);
-        const uint16_t sb = f2bf(s), cb = f2bf(c);
-        X[(size_t)b * 2 * F + j] = sb;
-        X[(size_t)b * 2 * F + F + j] = cb;
-        Xt[(size_t)j * n + b] = sb;
-        Xt[(size_t)(F + j) * n + b] = cb;
-    }
-}
-
-// H = gelu(Z) in row- and column-major bf16
-__global__ void k_gelu_fwd(const float* __restrict__ Z, int n, int H, uint16_t* __restrict__ Hr,
-                           uint16_t* __restrict__ Ht) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * H) return;
-    const int b = (int)(idx / H), j = (int)(idx % H);
-    const uint16_t h = f2bf(gelu(Z[idx]));
-    Hr[idx] = h;
-    Ht[(size_t)j * n + b] = h;
+    const float f = 6.283185307179586f * (fb[2 * j] * u0 + fb[2 * j + 1] * u1);
+    float s, c;
+    sincosf(f, &s, &c);
+    const uint16_t sb = f2bf(s), cb = f2bf(c);
+    X[(size_t)b * 2 * F + j] = sb;
+    X[(size_t)b * 2 * F + F + j] = cb;
+    Xt[(size_t)j * n + b] = sb;
+    Xt[(size_t)(F + j) * n + b] = cb;
 }
 
 // dZ = dH * gelu'(Z) in row- and column-major bf16, and the partial column
 // sums of dZ (as rounded to bf16) for the bias gradient: block (x, r) covers
 // columns 64x .. 64x+63 and rows [r*rc, min(n, (r+1)*rc)); thread (c, g) takes
 // rows g, g+4, ... of the chunk in order, the four row groups are added in a
-// fixed tree and the chunk's sum goes to part[r][j] (k_sum_splits adds the
-// chunks in order, so the bias gradient is deterministic).
+// fixed tree and the chunk's sum goes to part[r][j] (k_wgrad of the layer adds
+// the chunks in order, so the bias gradient is deterministic).
 __global__ __launch_bounds__(256) void k_gelu_bwd_colsum(const float* __restrict__ dH,
                                                          const float* __restrict__ Z, int n, int H, int rc,
                                                          uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
@@ -275,7 +303,7 @@ __global__ void k_head_loss(const float* __restrict__ Z, const double* __restric
 // columns 64x .. 64x+63 and rows [r*rc, min(n, (r+1)*rc)); thread (c, g) sums
 // rows g, g+4, ... of the chunk in order (coalesced across c), the four row
 // groups are added in a fixed tree and the chunk's sum goes to part[r][j].
-// k_sum_splits then adds the chunks in order, so db is deterministic.
+// k_wgrad of the head then adds the chunks in order, so db is deterministic.
 __global__ __launch_bounds__(256) void k_colsum_part(const float* __restrict__ dZ, int n, int H, int rc,
                                                      float* __restrict__ part) {
     __shared__ float red[4][64];
@@ -400,7 +428,7 @@ struct lyap_mlp {
     float* dh = nullptr;
     float* dzf = nullptr;
     float* loss = nullptr;
-    float* split_ws = nullptr;  // split-K partial products / bias-gradient chunk sums
+    float* split_ws = nullptr;  // the head's bias-gradient chunk sums
     float* col_ws = nullptr;    // bias-gradient chunk sums of the hidden layers
     int64_t step = 0;
     std::vector<void*> allocs;
@@ -469,12 +497,9 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * kOutPad * 4);  // the head's dZ in fp32
     rc |= mlp_alloc(m, (void**)&m->loss, 4);
-    size_t max_w = 0, max_d = 0;
-    for (int l = 0; l < kLayers; ++l) {
-        max_w = std::max(max_w, (size_t)dims[l + 1] * dims[l]);
-        max_d = std::max(max_d, (size_t)dims[l + 1]);
-    }
-    rc |= mlp_alloc(m, (void**)&m->split_ws, std::max((size_t)kMaxSplits * max_w, (size_t)kColsumChunks * max_d) * 4);
+    size_t max_d = 0;
+    for (int l = 0; l < kLayers; ++l) max_d = std::max(max_d, (size_t)dims[l + 1]);
+    rc |= mlp_alloc(m, (void**)&m->split_ws, (size_t)kColsumChunks * max_d * 4);
     rc |= mlp_alloc(m, (void**)&m->col_ws, (size_t)kColsumChunks * max_d * 4);
     if (rc != LYAP_SUCCESS) {
         lyap_mlp_destroy(m);
@@ -534,41 +559,44 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     return LYAP_SUCCESS;
 }
 
+// C = A . B^T (+ bias); with Hr also H = gelu(C) (row- and column-major bf16)
 static void gemm(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, const float* bias, int M,
-                 int N, int K, float beta) {
-    const int nb = (N % 64 == 0) ? 4 : 1;
+                 int N, int K, float beta, uint16_t* Hr = nullptr, uint16_t* Ht = nullptr) {
+    // 16 x 64 tiles per wave only when they still give four waves per CU (the
+    // inference batches); the training batches' products are latency-bound and
+    // run faster on four times as many 16 x 16 tiles (same sums per element)
+    const int nb = (N % 64 == 0 && (M / 16) * (N / 64) >= 1024) ? 4 : 1;
     const int waves = (M / 16) * (N / (16 * nb));
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
-    if (nb == 4) hipLaunchKernelGGL(k_gemm_nt<4>, grid, block, 0, s, A, B, C, bias, M, N, K, beta);
-    else hipLaunchKernelGGL(k_gemm_nt<1>, grid, block, 0, s, A, B, C, bias, M, N, K, beta);
+    if (nb == 4) hipLaunchKernelGGL(k_gemm_nt<4>, grid, block, 0, s, A, B, C, bias, M, N, K, beta, Hr, Ht);
+    else hipLaunchKernelGGL(k_gemm_nt<1>, grid, block, 0, s, A, B, C, bias, M, N, K, beta, Hr, Ht);
 }
 
-// weight-gradient product (small M x N, long K): split-K when K is long
-static void gemm_wgrad(lyap_mlp* m, hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, int M,
-                       int N, int K) {
+// weight-gradient product dW^T[M][N] (small M x N, K = the batch): K split
+// into up to kMaxSplits chunks of a multiple of 16, one wave each, inside one
+// workgroup per output tile; with bias_part the same launch adds the layer's
+// bias-gradient chunk sums ([chunks][M]) into db
+static void gemm_wgrad(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+                       const float* bias_part, int chunks, float* db) {
     int splits = K / 256;
     if (splits > kMaxSplits) splits = kMaxSplits;
-    if (splits < 2) {
-        gemm(s, A, B, C, nullptr, M, N, K, 0.f);
-        return;
-    }
+    if (splits < 1) splits = 1;
     const int kc = ((K + splits - 1) / splits + 15) / 16 * 16;
     splits = (K + kc - 1) / kc;
-    const int nb = (N % 64 == 0) ? 4 : 1;
-    const int waves = splits * (M / 16) * (N / (16 * nb));
-    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    // 16 x 16 tiles unless 16 x 64 ones still give a block per CU: these
+    // products are latency-bound, so more blocks beat more reuse per wave
+    const int nb = (N % 64 == 0 && (M / 16) * (N / 64) >= 256) ? 4 : 1;
+    const int tiles = (M / 16) * (N / (16 * nb));
+    const size_t lds = ((size_t)splits * 16 * 16 * nb + (size_t)kColsumChunks * 16) * sizeof(float);
+    const dim3 grid((unsigned)tiles), block((unsigned)(64 * splits));
     if (nb == 4)
-        hipLaunchKernelGGL(k_gemm_nt_splitk<4>, grid, block, 0, s, A, B, m->split_ws, M, N, K, kc, splits);
+        hipLaunchKernelGGL(k_wgrad<4>, grid, block, lds, s, A, B, C, M, N, K, kc, splits, bias_part, chunks, db);
     else
-        hipLaunchKernelGGL(k_gemm_nt_splitk<1>, grid, block, 0, s, A, B, m->split_ws, M, N, K, kc, splits);
-    const int64_t mn = (int64_t)M * N;
-    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((mn + 255) / 256)), dim3(256), 0, s, m->split_ws, splits,
-                       mn, C);
+        hipLaunchKernelGGL(k_wgrad<1>, grid, block, lds, s, A, B, C, M, N, K, kc, splits, bias_part, chunks, db);
 }
 
-// bias gradient db[j] = sum_b dZ[b][j] (after the weight-gradient product of
-// the same layer has consumed split_ws): up to kColsumChunks row chunks summed
-// in parallel, then added in chunk order
+// row chunks of the bias-gradient sums db[j] = sum_b dZ[b][j]: up to
+// kColsumChunks chunks summed in parallel, then added in chunk order
 static void colsum_chunks(int n, int* chunks, int* rc) {
     int c = n / 64;  // 64-row chunks: 16 rows per thread
     if (c > kColsumChunks) c = kColsumChunks;
@@ -577,29 +605,18 @@ static void colsum_chunks(int n, int* chunks, int* rc) {
     *chunks = (n + *rc - 1) / *rc;
 }
 
-static void colsum(lyap_mlp* m, hipStream_t s, const float* dZ, int n, int H, float* db) {
-    int chunks, rc;
-    colsum_chunks(n, &chunks, &rc);
-    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((H + 63) / 64), (unsigned)chunks), dim3(256), 0, s, dZ, n, H,
-                       rc, m->split_ws);
-    hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((H + 255) / 256)), dim3(256), 0, s, m->split_ws, chunks,
-                       (int64_t)H, db);
-}
-
 // clear_loss: the loss accumulator is cleared by the first kernel (training)
 static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s, bool clear_loss) {
     const auto& c = m->cfg;
     const int* D = m->dims;
-    hipLaunchKernelGGL(k_fourier, dim3((n_pad + 255) / 256), dim3(256), 0, s, d_p, n_pad, n_valid, m->fourier,
+    hipLaunchKernelGGL(k_fourier, dim3((unsigned)(((int64_t)n_pad * c.n_fourier + 255) / 256)), dim3(256), 0, s,
+                       d_p, n_pad, n_valid, m->fourier,
                        c.n_fourier, c.lo1, 1.0 / (c.hi1 - c.lo1), c.lo2, 1.0 / (c.hi2 - c.lo2), m->act[0],
                        m->act_t[0], clear_loss ? m->loss : nullptr);
     for (int l = 0; l < kLayers; ++l) {
-        gemm(s, m->act[l], m->wt[l], m->z[l], m->master + m->off_b[l], n_pad, D[l + 1], D[l], 0.f);
-        if (l + 1 < kLayers) {
-            const int64_t tot = (int64_t)n_pad * D[l + 1];
-            hipLaunchKernelGGL(k_gelu_fwd, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, m->z[l], n_pad,
-                               D[l + 1], m->act[l + 1], m->act_t[l + 1]);
-        }
+        const bool hidden = l + 1 < kLayers;
+        gemm(s, m->act[l], m->wt[l], m->z[l], m->master + m->off_b[l], n_pad, D[l + 1], D[l], 0.f,
+             hidden ? m->act[l + 1] : nullptr, hidden ? m->act_t[l + 1] : nullptr);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "mlp forward: %s", hipGetErrorString(e));
@@ -614,16 +631,15 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
                        m->dzf, m->loss);
     int chunks, rc;
     colsum_chunks(n_pad, &chunks, &rc);
+    // the head's bias-gradient chunk sums, from its fp32 dZ
+    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((D[kLayers] + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
+                       m->dzf, n_pad, D[kLayers], rc, m->split_ws);
     for (int l = kLayers - 1; l >= 0; --l) {
-        // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T
-        gemm_wgrad(m, s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad);
-        // bias gradient: the head's from its fp32 dZ; a hidden layer's from the
-        // chunk sums its GELU backward pass left in col_ws
-        if (l == kLayers - 1)
-            colsum(m, s, m->dzf, n_pad, D[l + 1], m->grad + m->off_b[l]);
-        else
-            hipLaunchKernelGGL(k_sum_splits, dim3((unsigned)((D[l + 1] + 255) / 256)), dim3(256), 0, s, m->col_ws,
-                               chunks, (int64_t)D[l + 1], m->grad + m->off_b[l]);
+        // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T, and the
+        // bias gradient from the chunk sums (the head's in split_ws, a hidden
+        // layer's left in col_ws by its GELU backward pass)
+        gemm_wgrad(s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad,
+                   l == kLayers - 1 ? m->split_ws : m->col_ws, chunks, m->grad + m->off_b[l]);
         if (l == 0) break;
         // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU
         gemm(s, m->dz, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);

@@ -285,7 +285,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         perm = torch.randperm(n_t, generator=gen).to(dev)
         pp, lp, sp = p_t[perm], lam_t[perm], st_t[perm]
         for b0 in range(0, n_t - batch + 1, batch):
-            net.train_step(pp[b0:b0 + batch], lp[b0:b0 + batch], sp[b0:b0 + batch], loss)
+            # the epoch's loss is its last batch's (no copy on the other steps)
+            last = b0 + 2 * batch > n_t
+            net.train_step(pp[b0:b0 + batch], lp[b0:b0 + batch], sp[b0:b0 + batch], loss if last else None)
         losses.append(loss)
         loss = torch.zeros(1, dtype=torch.float32, device=dev)
     t = mark_stream("train", t, main)
