@@ -171,7 +171,8 @@ def bench_sweep(args, ctx, spec, wl_name):
     }
     if tuned_lanes is not None:
         rec["config"]["lane_shape"] = {"lane_max_blocks": int(tuned_lanes.best[0]),
-                                       "lane_gather": int(tuned_lanes.best[1])}
+                                       "lane_gather": int(tuned_lanes.best[1]),
+                                       "lane_quad": int(tuned_lanes.best[2])}
         rec["config"]["lane_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned_lanes.timings.items()}
     if tuned is not None:
         rec["config"]["l96_tile"] = list(tuned.best)

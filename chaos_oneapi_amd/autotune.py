@@ -270,18 +270,19 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
                cache: Cache | None = None, force: bool = False, apply: bool = True,
                timer=None, time_scale: float | None = None) -> TuneResult:
     """Pick the launch shape of a single 3-D sweep and (apply=True) set it on
-    ctx: (lane_max_blocks, lane_gather) — the persistent grid cap (0 = every
-    block that fits, else that many 256-lane blocks) and how many lanes of a
-    wave gather before renormalising together.  With time_scale the
-    candidates run over the whole grid at that fraction of the integration
-    times; without it, on a sample of the points."""
+    ctx: (lane_max_blocks, lane_gather, lane_quad) — the persistent grid cap
+    (0 = every block that fits, else that many 256-lane blocks), how many
+    lanes of a wave gather before renormalising together, and whether a point
+    takes four lanes (small grids, whose launches last as long as their slowest
+    point).  With time_scale the candidates run over the whole grid at that
+    fraction of the integration times; without it, on a sample of the points."""
     import torch
 
     cache = cache if cache is not None else Cache()
     n_total = desc.n1 * desc.n2
     name = device_name(ctx.device) if timer is None else "test"
     cus = torch.cuda.get_device_properties(ctx.device).multi_processor_count if timer is None else 256
-    key = f"lanes|{name}|{desc_signature(desc)}x{_bucket(n_total)}"
+    key = f"lanes3|{name}|{desc_signature(desc)}x{_bucket(n_total)}"
     hit = cache.get(key)
     if hit and not force:
         res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
@@ -292,21 +293,24 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
             run_desc = desc
             idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 4), dtype=torch.int64)
         timings = {}
-        for cand in ((0, 0), (0, 8), (0, 16), (2 * cus, 0), (2 * cus, 8)):
+        for cand in ((0, 0, 0), (0, 8, 0), (0, 16, 0), (2 * cus, 0, 0), (2 * cus, 8, 0), (0, 0, 1)):
             if timer:
                 timings[cand] = timer(cand, idx)
                 continue
             ctx.set_option("lane_max_blocks", cand[0])
             ctx.set_option("lane_gather", cand[1])
+            ctx.set_option("lane_quad", cand[2])
             try:
                 ctx.run(run_desc, idx_list=None if idx is None else idx.to(torch.device("cuda", ctx.device)))
                 timings[cand] = ctx.kernel_ms()
             finally:
                 ctx.set_option("lane_max_blocks", 0)
                 ctx.set_option("lane_gather", 0)
+                ctx.set_option("lane_quad", 0)
         res = TuneResult(best=pick_best(timings), timings=timings)
         cache.put(key, res.best, timings)
     if apply and timer is None:
         ctx.set_option("lane_max_blocks", int(res.best[0]))
         ctx.set_option("lane_gather", int(res.best[1]))
+        ctx.set_option("lane_quad", int(res.best[2]))
     return res

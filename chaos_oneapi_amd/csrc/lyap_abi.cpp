@@ -118,6 +118,7 @@ struct lyap_ctx {
     int l96_max_blocks = 0;
     int lane_gather = 0;
     int item_order = 0;  // 0: items spread over the grid; 1: in list order
+    int lane_quad = 0;   // 1: the sweep's method on the 3-D flows with four lanes per point
     unsigned long long* counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
@@ -297,7 +298,8 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
                      c->item_order ? 1 : spread_stride(count), c->lane_gather, 0};
     if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     if (lane_model(fd.model)) {
-        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks);
+        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks,
+                                    c->lane_quad != 0 && fd.method == LYAP_METHOD_VAR && fd.dim == 3);
     } else if (fd.model == LYAP_MODEL_L96) {
         if (fd.method != LYAP_METHOD_VAR)
             return fail(LYAP_ERR_UNSUPPORTED, "Lorenz-96 runs the coupled variational method only");
@@ -361,6 +363,11 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
     if (k == "lane_gather") {
         if (value < 0 || value > 64) return fail(LYAP_ERR_INVALID, "lane_gather must be in [0, 64]");
         c->lane_gather = (int)value;
+        return LYAP_SUCCESS;
+    }
+    if (k == "lane_quad") {
+        if (value != 0 && value != 1) return fail(LYAP_ERR_INVALID, "lane_quad must be 0 or 1");
+        c->lane_quad = (int)value;
         return LYAP_SUCCESS;
     }
     if (k == "item_order") {

@@ -406,6 +406,234 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     }
 }
 
+// lane r of a quad receives lane q of the same quad (DPP quad_perm broadcast)
+template <int Q>
+LYAP_D double quad_bcast(double v) {
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)b, ctrl, 0xF, 0xF, true);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), ctrl, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
+// Persistent loop of the sweep's method with four lanes per point (a quad),
+// for launches that are about one point per lane and so last as long as their
+// slowest point.  Every lane of the quad runs the point's base step, error
+// norm and step control (identical operands, identical results, so the quad's
+// control flow agrees without communication); lane r < K advances deviation
+// vector r alone, so a step costs one tangent vector instead of K.  At a
+// renormalisation the quad's vectors are broadcast to every lane (DPP
+// quad_perm), every lane runs the same MGS on the full matrix and keeps its
+// own row; lane r accumulates exponent r.  The arithmetic of each point is
+// that of lane_drain, so both give the same bits.
+template <class M, int INTEG, bool ADAPT, bool MIXED, class GetDesc, class GetIO>
+LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io, GetIO getio) {
+    constexpr int N = M::N, P = M::P, K = M::N, S = Tab<INTEG>::S;
+    static_assert(K <= 4, "four lanes per point");
+    using TX = typename std::conditional<MIXED, float, double>::type;
+    const int64_t n_tr = d.n_renorm_transient;
+    const int64_t m_total = d.n_renorm_transient + d.n_renorm_average;
+    const int64_t max_steps = d.max_steps, spr = d.steps_per_renorm;
+    const double rtol = d.rtol, atol = d.atol, h_min = d.h_min, h_max = d.h_max;
+    const double t_renorm = d.t_renorm, esc = d.escape_radius, h_fixed = d.h;
+    const double log_floor = ctrl_log(1e-4);
+    unsigned long long* const counter = io.counter;
+    const int64_t count = io.count;
+    const int ql = (int)(__lane_id() & 3);
+    const int lead = (int)(__lane_id() & ~3);
+
+    double x[N], v[N], p[P], k1[N];
+    double sum = 0.0, comp = 0.0, lmant = 1.0;
+    int32_t lex = 0;
+    int lcnt = 0;
+    float pf[P];
+    double t = 0.0, h = 0.0, t_next = 0.0;
+    int64_t m = 0, step_in = 0, item = -1;
+    uint32_t nacc = 0, nrej = 0;
+    Ctrl ctrl{log_floor, false};
+    bool active = false, exhausted = false, pending = false;
+
+    auto write_out = [&](uint32_t st) {
+        const auto dp = getd();
+        const auto iop = getio();
+        const double t_avg = ADAPT ? (double)dp->n_renorm_average * dp->t_renorm
+                                   : (double)dp->n_renorm_average * (double)dp->steps_per_renorm * dp->h;
+        if (ql < dp->n_exps)
+            iop->lambda[item * dp->n_exps + ql] = (st == LYAP_PT_OK) ? sum / t_avg : __builtin_nan("");
+        if (ql == 0) {
+            if (iop->status) iop->status[item] = st;
+            if (iop->n_acc) iop->n_acc[item] = nacc;
+            if (iop->n_rej) iop->n_rej[item] = nrej;
+        }
+    };
+
+    for (;;) {
+        const bool want = !active && !exhausted;
+        int64_t got = grab_items(want && ql == 0, counter);
+        got = __shfl(got, lead);
+        if (want) {
+            if (got < count) {
+                const auto dp = getd();
+                const auto iop = getio();
+                item = item_position(got, iop->stride, count);
+                const int64_t idx = iop->idx_list ? iop->idx_list[item] : iop->first + item;
+                point_params<P>(*dp, idx, iop->params, p);
+#pragma unroll
+                for (int j = 0; j < P; ++j) pf[j] = (float)p[j];
+                const double* const inits = iop->inits;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    x[i] = inits ? inits[idx * N + i] : init_component(*dp, idx, i, p);
+                    v[i] = (i == ql) ? 1.0 : 0.0;
+                }
+                sum = comp = 0.0;
+                lmant = 1.0;
+                lex = 0;
+                lcnt = 0;
+                t = 0.0;
+                h = dp->h;
+                m = 0;
+                t_next = t_renorm;
+                step_in = 0;
+                nacc = nrej = 0;
+                ctrl.log_fac_old = log_floor;
+                ctrl.rejected = false;
+                pending = false;
+                if (INTEG == LYAP_INT_DOPRI5 && ADAPT) M::rhs(x, p, k1);
+                active = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (!__any(active)) break;
+        if (__any(active && pending)) {
+            if (active && pending) {
+                double V[K][N], nrm[K];
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    V[0][i] = quad_bcast<0>(v[i]);
+                    if constexpr (K > 1) V[1][i] = quad_bcast<1>(v[i]);
+                    if constexpr (K > 2) V[2][i] = quad_bcast<2>(v[i]);
+                    if constexpr (K > 3) V[3][i] = quad_bcast<3>(v[i]);
+                }
+                mgs_rows<K, N>(V, nrm);
+                double own = nrm[0];
+#pragma unroll
+                for (int r = 1; r < K; ++r) own = (ql == r) ? nrm[r] : own;
+#pragma unroll
+                for (int i = 0; i < N; ++i) {
+                    double w = V[0][i];
+#pragma unroll
+                    for (int r = 1; r < K; ++r) w = (ql == r) ? V[r][i] : w;
+                    v[i] = w;
+                }
+                ++m;
+                t_next = (double)(m + 1) * t_renorm;
+                if (m > n_tr) {
+                    lognorm_push(lmant, lex, own);
+                    if (++lcnt == kLogWindow || m == m_total) {
+                        lognorm_flush(lmant, lex, sum, comp);
+                        lcnt = 0;
+                    }
+                }
+                pending = false;
+                if (m == m_total) {
+                    write_out(LYAP_PT_OK);
+                    active = false;
+                }
+            }
+        }
+        if (!active || pending) continue;
+
+        uint32_t st = LYAP_PT_OK;
+        bool finished = false;
+        double xn[N];
+        typename M::template J<TX> X[S];
+        if constexpr (ADAPT) {
+            if ((int64_t)nacc + (int64_t)nrej >= max_steps) {
+                st = LYAP_PT_MAXSTEPS;
+                finished = true;
+            } else {
+                double h_try = h;
+                bool clipped = false;
+                const double jn = M::jnorm(x, p);
+                if (__any(h_try * jn > 1.0)) {
+                    double jv = jn;
+                    asm volatile("" : "+v"(jv));
+                    if (h_try * jn > 1.0) h_try = 1.0 / jv;
+                }
+                if (t_next - t <= 1.0625 * h_try) {
+                    h_try = t_next - t;
+                    clipped = true;
+                }
+                if (h_try < h_min) {
+                    st = LYAP_PT_STEP_UNDERFLOW;
+                    finished = true;
+                } else {
+                    double ev[N], k1_next[N];
+#pragma unroll
+                    for (int i = 0; i < N; ++i) k1_next[i] = k1[i];
+                    base_step<M, INTEG, true, TX>(x, p, h_try, xn, ev, X, k1_next);
+                    const double e2 = err2_seq<N>(ev, x, xn, rtol, atol);
+                    if (!(e2 < kInf)) {
+                        st = LYAP_PT_NONFINITE;
+                        finished = true;
+                    } else {
+                        double h_next;
+                        const double lerr = 0.5 * ctrl_log(e2);
+                        const bool acc = ctrl_update<INTEG>(ctrl, e2 <= 1.0, e2 > 1e-8, lerr, log_floor,
+                                                            h_try, h_max, h_next);
+                        h = h_next;
+                        if (!acc) {
+                            ++nrej;
+                        } else {
+                            if (ql < K) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_try, v);
+#pragma unroll
+                            for (int i = 0; i < N; ++i) {
+                                x[i] = xn[i];
+                                k1[i] = k1_next[i];
+                            }
+                            t = clipped ? t_next : t + h_try;
+                            ++nacc;
+                            if (escaped<N>(x, esc)) {
+                                st = LYAP_PT_ESCAPED;
+                                finished = true;
+                            } else if (clipped) {
+                                pending = true;
+                            }
+                        }
+                    }
+                }
+            }
+        } else {
+            base_step<M, INTEG, false, TX>(x, p, h_fixed, xn, nullptr, X, nullptr);
+            if (ql < K) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_fixed, v);
+#pragma unroll
+            for (int i = 0; i < N; ++i) x[i] = xn[i];
+            ++nacc;
+            if (++step_in == spr) {
+                step_in = 0;
+                bool bad = false;
+#pragma unroll
+                for (int i = 0; i < N; ++i) bad = bad || !isfinite(x[i]);
+                if (bad) {
+                    st = LYAP_PT_NONFINITE;
+                    finished = true;
+                } else if (escaped<N>(x, esc)) {
+                    st = LYAP_PT_ESCAPED;
+                    finished = true;
+                } else {
+                    pending = true;
+                }
+            }
+        }
+        if (finished) {
+            write_out(st);
+            active = false;
+        }
+    }
+}
+
 // Persistent per-lane loop of the sweep's method for maps: per iteration
 // V <- J(x) V (binary32 products in the mixed form), x <- f(x); MGS and the
 // windowed log-norm accumulation every steps_per_renorm iterations; the escape test after

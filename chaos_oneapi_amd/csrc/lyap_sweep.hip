@@ -28,6 +28,12 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
     lane_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
 }
 
+// the same method with four lanes per point (latency-bound launches)
+template <class M, int INTEG, bool ADAPT, bool MIXED>
+__global__ __launch_bounds__(256) void k_var_quad(const lyap_sweep_desc d, const SweepIO io) {
+    quad_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
+}
+
 // ---------------------------------------------------------------------------
 // reference modes (fixed-step DVERK, one lane per point, grid-stride)
 // ---------------------------------------------------------------------------
@@ -500,15 +506,17 @@ __global__ void k_reset_counter(unsigned long long* c) {
 // ---------------------------------------------------------------------------
 // launch table
 // ---------------------------------------------------------------------------
+// lanes_per_point: 1 (k_var_lane and the reference kernels) or 4 (k_var_quad)
 template <class K>
 static hipError_t launch_persistent(K kern, const lyap_sweep_desc& d, const SweepIO& io,
-                                    hipStream_t stream, int num_cus, int max_blocks_hint) {
+                                    hipStream_t stream, int num_cus, int max_blocks_hint,
+                                    int lanes_per_point = 1) {
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     int64_t blocks = (int64_t)per_cu * num_cus;
-    const int64_t need = (io.count + 255) / 256;
+    const int64_t need = (io.count * lanes_per_point + 255) / 256;
     if (blocks > need) blocks = need;
     if (max_blocks_hint > 0 && blocks > max_blocks_hint) blocks = max_blocks_hint;
     if (blocks < 1) blocks = 1;
@@ -519,13 +527,14 @@ static hipError_t launch_persistent(K kern, const lyap_sweep_desc& d, const Swee
 
 template <class M>
 static hipError_t dispatch_model(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
-                                 int num_cus, int max_blocks) {
+                                 int num_cus, int max_blocks, bool quad) {
     if (d.method == LYAP_METHOD_VAR) {
         const int key = (d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0);
         switch (key) {
 #define LYAP_CASE(I, A, X)                                                                  \
     case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):                                        \
-        return launch_persistent(k_var_lane<M, I, A, X>, d, io, s, num_cus, max_blocks);
+        return quad ? launch_persistent(k_var_quad<M, I, A, X>, d, io, s, num_cus, max_blocks, 4) \
+                    : launch_persistent(k_var_lane<M, I, A, X>, d, io, s, num_cus, max_blocks);
             LYAP_CASE(LYAP_INT_DVERK, false, false)
             LYAP_CASE(LYAP_INT_DVERK, false, true)
             LYAP_CASE(LYAP_INT_DVERK, true, false)
@@ -582,18 +591,18 @@ static hipError_t dispatch_map(const lyap_sweep_desc& d, const SweepIO& io, hipS
 }
 
 hipError_t launch_lane_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
-                             int num_cus, int max_blocks) {
+                             int num_cus, int max_blocks, bool quad) {
     switch (d.model) {
         case LYAP_MODEL_HENON:
             return dispatch_map<Henon>(d, io, s, num_cus, max_blocks);
         case LYAP_MODEL_LOGISTIC:
             return dispatch_map<Logistic>(d, io, s, num_cus, max_blocks);
         case LYAP_MODEL_SHIMIZU:
-            return dispatch_model<Shimizu>(d, io, s, num_cus, max_blocks);
+            return dispatch_model<Shimizu>(d, io, s, num_cus, max_blocks, quad);
         case LYAP_MODEL_LORENZ:
-            return dispatch_model<Lorenz>(d, io, s, num_cus, max_blocks);
+            return dispatch_model<Lorenz>(d, io, s, num_cus, max_blocks, quad);
         case LYAP_MODEL_ROSSLER:
-            return dispatch_model<Rossler>(d, io, s, num_cus, max_blocks);
+            return dispatch_model<Rossler>(d, io, s, num_cus, max_blocks, quad);
         default:
             return hipErrorInvalidValue;
     }

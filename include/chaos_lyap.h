@@ -168,6 +168,11 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *       s ~ count/phi coprime to count), 1 in order (point i, i.e. the order of
  *       idx_list: a list sorted by decreasing expected cost schedules the
  *       longest points first); results are the same bits either way
+ *   "lane_quad": 1 runs the sweep's method on the 3-D flows with four lanes
+ *       per point (each lane one deviation vector; base step and control
+ *       repeated in the four lanes), for launches of about one point per lane
+ *       whose length is their slowest point; 0 (default) one lane per point;
+ *       results are the same bits either way
  */
 int lyap_ctx_set_option(lyap_ctx* ctx, const char* key, int64_t value);
 

@@ -101,9 +101,13 @@ def test_tune_l96_candidates_cover_built_layouts(tmp_path):
 
 def test_tune_lanes_grid_caps():
     desc = L.finalize(L.make_desc(W.rossler_dverk(n=64)))
-    r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=lambda c, idx: 2.0 - 0.01 * c[1] + (c[0] > 0))
-    assert r.best == (0, 16)
-    assert set(r.timings) == {(0, 0), (0, 8), (0, 16), (512, 0), (512, 8)}
+    r = A.tune_lanes(None, desc, cache=A.Cache(None),
+                     timer=lambda c, idx: 2.0 - 0.01 * c[1] + (c[0] > 0) + 0.5 * c[2])
+    assert r.best == (0, 16, 0)
+    assert set(r.timings) == {(0, 0, 0), (0, 8, 0), (0, 16, 0), (512, 0, 0), (512, 8, 0), (0, 0, 1)}
+    # a small grid whose launch is its slowest point: four lanes per point wins
+    r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=lambda c, idx: 2.0 - 1.5 * c[2])
+    assert r.best == (0, 0, 1)
 
 
 def test_sample_keeps_a_minimum_total():
@@ -159,8 +163,8 @@ def test_tune_lanes_time_scaled_runs_the_whole_grid():
 
     def timer(cand, idx):
         seen.append(idx)
-        return 1.0 if cand == (512, 8) else 2.0
+        return 1.0 if cand == (512, 8, 0) else 2.0
 
     r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=timer, time_scale=0.1)
-    assert r.best == (512, 8)
-    assert len(seen) == 5 and all(i is None for i in seen)
+    assert r.best == (512, 8, 0)
+    assert len(seen) == 6 and all(i is None for i in seen)

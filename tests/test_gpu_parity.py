@@ -68,6 +68,33 @@ def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
     assert_match(g, o, TOL_FP64, exact=True)
 
 
+@pytest.mark.parametrize("integ", [L.INT_DVERK, L.INT_DOPRI5])
+@pytest.mark.parametrize("adaptive", [0, 1])
+@pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
+@pytest.mark.parametrize("model", ["lorenz", "rossler", "shimizu"])
+def test_var_quad_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
+    """lane_quad = 1: four lanes per point (one deviation vector per lane, the
+    base step and control repeated), on a grid of one block so that quads
+    refill: the same bits as the oracle (and so as the one-lane kernel)."""
+    if model == "rossler":
+        spec = W.rossler_dverk(n=12, precision=prec, t_transient=10.0, t_average=40.0)
+    else:
+        spec = W.lorenz_fixed(n=12, precision=prec, t_transient=5.0, t_average=20.0)
+        if model == "shimizu":
+            spec.update(model=L.MODEL_SHIMIZU, n_params=2, base_params=[0.0, 0.0],
+                        init_state=[0.1, 0.0, 0.0], init_scale=1e-3)
+            spec.update(W.grid_fields(0, 0.2, 2.0, 12, 1, 0.0, 1.2, 12))
+    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9)
+    gpu_ctx.set_option("lane_quad", 1)
+    gpu_ctx.set_option("lane_max_blocks", 1)
+    try:
+        g, o = run_both(gpu_ctx, spec)
+    finally:
+        gpu_ctx.set_option("lane_quad", 0)
+        gpu_ctx.set_option("lane_max_blocks", 0)
+    assert_match(g, o, TOL_FP64, exact=True)
+
+
 @pytest.mark.parametrize("gather", [2, 16, 64])
 @pytest.mark.parametrize("integ,adaptive,prec", [(L.INT_DVERK, 1, L.PREC_MIXED),
                                                  (L.INT_DOPRI5, 1, L.PREC_FP64),
