@@ -414,24 +414,59 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     wave_lds_sync();
                     double* row = sV + u * RS;
                     double vn = 0.0;
-                    for (int j = 0; j < K; ++j) {
-                        if (u == j) {
-                            double s2 = row[0] * row[0];
-#pragma unroll 8
-                            for (int c = 1; c < N; ++c) s2 = fma(row[c], row[c], s2);
-                            const double inv = inv_sqrt_det(s2);
-                            vn = s2 * inv;  // accumulated after the loop, by every row lane at once
-#pragma unroll 8
-                            for (int c = 0; c < N; ++c) row[c] = row[c] * inv;
+                    if constexpr (G == 1 && N <= 16) {
+                        // one vector group per point (E = N <= 16): lane r keeps row r
+                        // in the registers its E vectors leave free while the
+                        // matrix is staged, and writes it to LDS once,
+                        // normalised, for the later rows
+                        double rw[N];
+                        if (u < K) {
+#pragma unroll
+                            for (int c = 0; c < N; ++c) rw[c] = row[c];
                         }
-                        wave_lds_sync();
-                        if (u > j && u < K) {
-                            const double* qj = sV + j * RS;
-                            double cdot = qj[0] * row[0];
+                        for (int j = 0; j < K; ++j) {
+                            if (u == j) {
+                                double s2 = rw[0] * rw[0];
+#pragma unroll
+                                for (int c = 1; c < N; ++c) s2 = fma(rw[c], rw[c], s2);
+                                const double inv = inv_sqrt_det(s2);
+                                vn = s2 * inv;  // accumulated after the loop, by every row lane at once
+#pragma unroll
+                                for (int c = 0; c < N; ++c) {
+                                    rw[c] = rw[c] * inv;
+                                    row[c] = rw[c];
+                                }
+                            }
+                            wave_lds_sync();
+                            if (u > j && u < K) {
+                                const double* qj = sV + j * RS;
+                                double cdot = qj[0] * rw[0];
+#pragma unroll
+                                for (int c = 1; c < N; ++c) cdot = fma(qj[c], rw[c], cdot);
+#pragma unroll
+                                for (int c = 0; c < N; ++c) rw[c] = fma(-cdot, qj[c], rw[c]);
+                            }
+                        }
+                    } else {
+                        for (int j = 0; j < K; ++j) {
+                            if (u == j) {
+                                double s2 = row[0] * row[0];
 #pragma unroll 8
-                            for (int c = 1; c < N; ++c) cdot = fma(qj[c], row[c], cdot);
+                                for (int c = 1; c < N; ++c) s2 = fma(row[c], row[c], s2);
+                                const double inv = inv_sqrt_det(s2);
+                                vn = s2 * inv;  // accumulated after the loop, by every row lane at once
 #pragma unroll 8
-                            for (int c = 0; c < N; ++c) row[c] = fma(-cdot, qj[c], row[c]);
+                                for (int c = 0; c < N; ++c) row[c] = row[c] * inv;
+                            }
+                            wave_lds_sync();
+                            if (u > j && u < K) {
+                                const double* qj = sV + j * RS;
+                                double cdot = qj[0] * row[0];
+#pragma unroll 8
+                                for (int c = 1; c < N; ++c) cdot = fma(qj[c], row[c], cdot);
+#pragma unroll 8
+                                for (int c = 0; c < N; ++c) row[c] = fma(-cdot, qj[c], row[c]);
+                            }
                         }
                     }
                     wave_lds_sync();
