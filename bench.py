@@ -419,7 +419,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a "
-              "single MI355X; multi-GPU sharding is described in DESIGN.md, not launched.",
+              "single MI355X; it launches no multi-GPU sharding (the grid would split into one "
+              "index range per GPU, each run by this same single-GPU path).",
               file=sys.stderr)
         sys.exit(2)
 
