@@ -254,6 +254,55 @@ __global__ __launch_bounds__(256) void k_gelu_bwd_colsum(const float* __restrict
     if (g == 0 && j < H) part[(size_t)r * H + j] = red[0][c] + red[1][c];
 }
 
+// Input gradient through the GELU of the layer below, in one launch:
+// dH[B][N] = dZ[B][K] . W[N][K]^T on 16x16 MFMA tiles, a block's four waves
+// covering one 64-row chunk of one 16-column group, then dZ' = bf16(dH *
+// gelu'(Z)) to row- and column-major dZr / dZt and the chunk's bias-gradient
+// column sums to part[chunk][col] -- the same operations, in the same order,
+// as the product followed by k_gelu_bwd_colsum with 64-row chunks.  dZr must
+// not alias dZ (other blocks are still reading it).
+__global__ __launch_bounds__(256) void k_gemm_dgelu(const uint16_t* __restrict__ dZ,
+                                                    const uint16_t* __restrict__ W,
+                                                    const float* __restrict__ Z, int M, int N, int K,
+                                                    uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                                                    float* __restrict__ part) {
+    __shared__ float tile[64][17];
+    __shared__ float red[4][16];
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int groups_n = N / 16;
+    const int chunk = (int)blockIdx.x / groups_n;
+    const int n0 = ((int)blockIdx.x % groups_n) * 16;
+    const int m0 = chunk * 64 + w * 16;
+    const int r = lane & 15, q = lane >> 4;
+    v4f acc[1] = {v4f{0.f, 0.f, 0.f, 0.f}};
+    mfma_k_range<1>(dZ + (size_t)(m0 + r) * K + 4 * q, W + (size_t)(n0 + r) * K + 4 * q, K, 0, K, acc);
+    const int col = n0 + r;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int row = m0 + 4 * q + i;
+        const float dh = 0.f + acc[0][i] + 0.f;  // the stored dH of the two-kernel form
+        const size_t idx = (size_t)row * N + col;
+        const uint16_t v = f2bf(dh * gelu_grad(Z[idx]));
+        dZr[idx] = v;
+        dZt[(size_t)col * M + row] = v;
+        tile[w * 16 + 4 * q + i][r] = bf2f(v);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int c = (int)(threadIdx.x & 15), g = (int)(threadIdx.x >> 4);
+        float s = 0.f;
+        for (int b = g; b < 64; b += 4) s += tile[b][c];
+        red[g][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int c = (int)(threadIdx.x & 15), g = (int)(threadIdx.x >> 4);
+        red[g][c] += red[g + 2][c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) part[(size_t)chunk * N + n0 + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
+}
+
 // loss and head gradient: Gaussian NLL on mu1 with the learned log-variance,
 // squared error on mu2, mu3 (targets normalised), logistic loss on escape.
 // Rows b >= n_valid are padding (zero gradient).
@@ -425,6 +474,7 @@ struct lyap_mlp {
     uint16_t *act[kLayers] = {}, *act_t[kLayers] = {};  // inputs of layer l (row, column major)
     float* z[kLayers] = {};                              // pre-activations
     uint16_t *dz = nullptr, *dz_t = nullptr;
+    uint16_t* dz2 = nullptr;  // the other dZ buffer (the fused input-gradient pass reads one, writes the other)
     float* dh = nullptr;
     float* dzf = nullptr;
     float* loss = nullptr;
@@ -493,6 +543,7 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
         rc |= mlp_alloc(m, (void**)&m->z[l], (size_t)B * dims[l + 1] * 4);
     }
     rc |= mlp_alloc(m, (void**)&m->dz, (size_t)B * H * 2);
+    rc |= mlp_alloc(m, (void**)&m->dz2, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dz_t, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * kOutPad * 4);  // the head's dZ in fp32
@@ -631,6 +682,8 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
                        m->dzf, m->loss);
     int chunks, rc;
     colsum_chunks(n_pad, &chunks, &rc);
+    uint16_t* dz_in = m->dz;  // dZ of the layer being differentiated (row-major)
+    uint16_t* dz_out = m->dz2;
     // the head's bias-gradient chunk sums, from its fp32 dZ
     hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((D[kLayers] + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
                        m->dzf, n_pad, D[kLayers], rc, m->split_ws);
@@ -641,10 +694,18 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
         gemm_wgrad(s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad,
                    l == kLayers - 1 ? m->split_ws : m->col_ws, chunks, m->grad + m->off_b[l]);
         if (l == 0) break;
-        // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU
-        gemm(s, m->dz, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);
-        hipLaunchKernelGGL(k_gelu_bwd_colsum, dim3((unsigned)((D[l] + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
-                           m->dh, m->z[l - 1], n_pad, D[l], rc, m->dz, m->dz_t, m->col_ws);
+        // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU:
+        // one launch when the bias chunks are 64 rows (block = chunk), else the
+        // product and k_gelu_bwd_colsum
+        if (rc == 64 && n_pad == chunks * 64 && D[l] % 16 == 0) {
+            hipLaunchKernelGGL(k_gemm_dgelu, dim3((unsigned)(chunks * (D[l] / 16))), dim3(256), 0, s, dz_in,
+                               m->wn[l], m->z[l - 1], n_pad, D[l], D[l + 1], dz_out, m->dz_t, m->col_ws);
+            std::swap(dz_in, dz_out);
+        } else {
+            gemm(s, dz_in, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);
+            hipLaunchKernelGGL(k_gelu_bwd_colsum, dim3((unsigned)((D[l] + 63) / 64), (unsigned)chunks), dim3(256), 0,
+                               s, m->dh, m->z[l - 1], n_pad, D[l], rc, dz_in, m->dz_t, m->col_ws);
+        }
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "mlp backward: %s", hipGetErrorString(e));
