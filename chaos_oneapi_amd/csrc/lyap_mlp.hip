@@ -128,18 +128,18 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
 // same sums as separate split-K partial products added in order.  With bias_part
 // ([chunks][M] row-chunk sums of the layer's dZ), the blocks of the first tile
 // column also add the bias gradient of their 16 rows in chunk order.
+// wgrad_tile is the body for tile t, on a block of 64 * splits threads with
+// `part` its LDS ([splits][16][16 * NB], then [kColsumChunks][16]).
 template <int NB>
-__global__ __launch_bounds__(1024) void k_wgrad(const uint16_t* __restrict__ A,
-                                                const uint16_t* __restrict__ B, float* __restrict__ C,
-                                                int M, int N, int K, int kc, int splits,
-                                                const float* __restrict__ bias_part, int chunks,
-                                                float* __restrict__ db) {
-    extern __shared__ float part[];  // [splits][16][16 * NB], then [kColsumChunks][16] bias chunk sums
+__device__ __forceinline__ void wgrad_tile(int t, float* part, const uint16_t* __restrict__ A,
+                                           const uint16_t* __restrict__ B, float* __restrict__ C, int M,
+                                           int N, int K, int kc, int splits,
+                                           const float* __restrict__ bias_part, int chunks,
+                                           float* __restrict__ db) {
     constexpr int TW = 16 * NB;
     const int split = (int)(threadIdx.x >> 6);
     const int lane = (int)(threadIdx.x & 63);
     const int tiles_n = N / TW;
-    const int t = (int)blockIdx.x;
     const int m0 = (t / tiles_n) * 16;
     const int n0 = (t % tiles_n) * TW;
     const int r = lane & 15, q = lane >> 4;
@@ -187,6 +187,16 @@ __global__ __launch_bounds__(1024) void k_wgrad(const uint16_t* __restrict__ A,
         for (int k = 1; k < chunks; ++k) s += bl[k * 16 + threadIdx.x];
         db[m0 + threadIdx.x] = s;
     }
+}
+
+template <int NB>
+__global__ __launch_bounds__(1024) void k_wgrad(const uint16_t* __restrict__ A,
+                                                const uint16_t* __restrict__ B, float* __restrict__ C,
+                                                int M, int N, int K, int kc, int splits,
+                                                const float* __restrict__ bias_part, int chunks,
+                                                float* __restrict__ db) {
+    extern __shared__ float part[];
+    wgrad_tile<NB>((int)blockIdx.x, part, A, B, C, M, N, K, kc, splits, bias_part, chunks, db);
 }
 
 
@@ -261,17 +271,18 @@ __global__ __launch_bounds__(256) void k_gelu_bwd_colsum(const float* __restrict
 // column sums to part[chunk][col] -- the same operations, in the same order,
 // as the product followed by k_gelu_bwd_colsum with 64-row chunks.  dZr must
 // not alias dZ (other blocks are still reading it).
-__global__ __launch_bounds__(256) void k_gemm_dgelu(const uint16_t* __restrict__ dZ,
-                                                    const uint16_t* __restrict__ W,
-                                                    const float* __restrict__ Z, int M, int N, int K,
-                                                    uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
-                                                    float* __restrict__ part) {
-    __shared__ float tile[64][17];
-    __shared__ float red[4][16];
-    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+// the body of k_gemm_dgelu for virtual block vb, run by 256 threads (tid
+// 0..255) with their own LDS tile and reduction rows; every thread of the
+// workgroup reaches its three barriers
+__device__ __forceinline__ void dgelu_chunk(int vb, int tid, float (*tile)[17], float (*red)[16],
+                                            const uint16_t* __restrict__ dZ, const uint16_t* __restrict__ W,
+                                            const float* __restrict__ Z, int M, int N, int K,
+                                            uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                                            float* __restrict__ part) {
+    const int w = tid >> 6, lane = tid & 63;
     const int groups_n = N / 16;
-    const int chunk = (int)blockIdx.x / groups_n;
-    const int n0 = ((int)blockIdx.x % groups_n) * 16;
+    const int chunk = vb / groups_n;
+    const int n0 = (vb % groups_n) * 16;
     const int m0 = chunk * 64 + w * 16;
     const int r = lane & 15, q = lane >> 4;
     v4f acc[1] = {v4f{0.f, 0.f, 0.f, 0.f}};
@@ -288,19 +299,66 @@ __global__ __launch_bounds__(256) void k_gemm_dgelu(const uint16_t* __restrict__
         tile[w * 16 + 4 * q + i][r] = bf2f(v);
     }
     __syncthreads();
-    if (threadIdx.x < 64) {
-        const int c = (int)(threadIdx.x & 15), g = (int)(threadIdx.x >> 4);
+    if (tid < 64) {
+        const int c = tid & 15, g = tid >> 4;
         float s = 0.f;
         for (int b = g; b < 64; b += 4) s += tile[b][c];
         red[g][c] = s;
     }
     __syncthreads();
-    if (threadIdx.x < 32) {
-        const int c = (int)(threadIdx.x & 15), g = (int)(threadIdx.x >> 4);
+    if (tid < 32) {
+        const int c = tid & 15, g = tid >> 4;
         red[g][c] += red[g + 2][c];
     }
     __syncthreads();
-    if (threadIdx.x < 16) part[(size_t)chunk * N + n0 + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
+    if (tid < 16) part[(size_t)chunk * N + n0 + tid] = red[0][tid] + red[1][tid];
+}
+
+__global__ __launch_bounds__(256) void k_gemm_dgelu(const uint16_t* __restrict__ dZ,
+                                                    const uint16_t* __restrict__ W,
+                                                    const float* __restrict__ Z, int M, int N, int K,
+                                                    uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                                                    float* __restrict__ part) {
+    __shared__ float tile[64][17];
+    __shared__ float red[4][16];
+    dgelu_chunk((int)blockIdx.x, (int)threadIdx.x, tile, red, dZ, W, Z, M, N, K, dZr, dZt, part);
+}
+
+// One launch for a layer's weight gradient (k_wgrad, 16x16 tiles, eight
+// splits: 512-thread blocks) and the input gradient of the layer below
+// (k_gemm_dgelu, two 64-row chunks a block): the two products are independent
+// and each is latency-bound, so side by side they take about as long as the
+// longer one.  Blocks [0, wgrad_blocks) are weight-gradient tiles.
+struct WgradArgs {
+    const uint16_t* A;
+    const uint16_t* B;
+    float* C;
+    int M, N, K, kc, splits;
+    const float* bias_part;
+    int chunks;
+    float* db;
+};
+struct DgeluArgs {
+    const uint16_t* dZ;
+    const uint16_t* W;
+    const float* Z;
+    int M, N, K;
+    uint16_t* dZr;
+    uint16_t* dZt;
+    float* part;
+};
+__global__ __launch_bounds__(512) void k_wgrad_dgelu(WgradArgs wa, DgeluArgs da, int wgrad_blocks) {
+    extern __shared__ float lds[];
+    if ((int)blockIdx.x < wgrad_blocks) {
+        wgrad_tile<1>((int)blockIdx.x, lds, wa.A, wa.B, wa.C, wa.M, wa.N, wa.K, wa.kc, wa.splits, wa.bias_part,
+                      wa.chunks, wa.db);
+        return;
+    }
+    const int half = (int)(threadIdx.x >> 8);
+    float(*tile)[17] = reinterpret_cast<float(*)[17]>(lds + half * (64 * 17 + 64));
+    float(*red)[16] = reinterpret_cast<float(*)[16]>(lds + half * (64 * 17 + 64) + 64 * 17);
+    dgelu_chunk(((int)blockIdx.x - wgrad_blocks) * 2 + half, (int)(threadIdx.x & 255), tile, red, da.dZ, da.W,
+                da.Z, da.M, da.N, da.K, da.dZr, da.dZt, da.part);
 }
 
 // loss and head gradient: Gaussian NLL on mu1 with the learned log-variance,
@@ -474,12 +532,12 @@ struct lyap_mlp {
     uint16_t *act[kLayers] = {}, *act_t[kLayers] = {};  // inputs of layer l (row, column major)
     float* z[kLayers] = {};                              // pre-activations
     uint16_t *dz = nullptr, *dz_t = nullptr;
-    uint16_t* dz2 = nullptr;  // the other dZ buffer (the fused input-gradient pass reads one, writes the other)
+    uint16_t *dz2 = nullptr, *dz_t2 = nullptr;  // the other dZ buffers (the input-gradient pass reads one pair, writes the other)
     float* dh = nullptr;
     float* dzf = nullptr;
     float* loss = nullptr;
     float* split_ws = nullptr;  // the head's bias-gradient chunk sums
-    float* col_ws = nullptr;    // bias-gradient chunk sums of the hidden layers
+    float *col_ws = nullptr, *col_ws2 = nullptr;  // bias-gradient chunk sums of the hidden layers (a pair)
     int64_t step = 0;
     std::vector<void*> allocs;
 };
@@ -544,6 +602,7 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     }
     rc |= mlp_alloc(m, (void**)&m->dz, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dz2, (size_t)B * H * 2);
+    rc |= mlp_alloc(m, (void**)&m->dz_t2, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dz_t, (size_t)B * H * 2);
     rc |= mlp_alloc(m, (void**)&m->dh, (size_t)B * H * 4);
     rc |= mlp_alloc(m, (void**)&m->dzf, (size_t)B * kOutPad * 4);  // the head's dZ in fp32
@@ -552,6 +611,7 @@ int lyap_mlp_create(int device, const lyap_mlp_config* cfg, lyap_mlp** out) {
     for (int l = 0; l < kLayers; ++l) max_d = std::max(max_d, (size_t)dims[l + 1]);
     rc |= mlp_alloc(m, (void**)&m->split_ws, (size_t)kColsumChunks * max_d * 4);
     rc |= mlp_alloc(m, (void**)&m->col_ws, (size_t)kColsumChunks * max_d * 4);
+    rc |= mlp_alloc(m, (void**)&m->col_ws2, (size_t)kColsumChunks * max_d * 4);
     if (rc != LYAP_SUCCESS) {
         lyap_mlp_destroy(m);
         return LYAP_ERR_HIP;
@@ -627,16 +687,22 @@ static void gemm(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, 
 // into up to kMaxSplits chunks of a multiple of 16, one wave each, inside one
 // workgroup per output tile; with bias_part the same launch adds the layer's
 // bias-gradient chunk sums ([chunks][M]) into db
-static void gemm_wgrad(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
-                       const float* bias_part, int chunks, float* db) {
+static void wgrad_shape(int M, int N, int K, int* splits_out, int* kc_out, int* nb_out) {
     int splits = K / 256;
     if (splits > kMaxSplits) splits = kMaxSplits;
     if (splits < 1) splits = 1;
     const int kc = ((K + splits - 1) / splits + 15) / 16 * 16;
-    splits = (K + kc - 1) / kc;
+    *splits_out = (K + kc - 1) / kc;
+    *kc_out = kc;
     // 16 x 16 tiles unless 16 x 64 ones still give a block per CU: these
     // products are latency-bound, so more blocks beat more reuse per wave
-    const int nb = (N % 64 == 0 && (M / 16) * (N / 64) >= 256) ? 4 : 1;
+    *nb_out = (N % 64 == 0 && (M / 16) * (N / 64) >= 256) ? 4 : 1;
+}
+
+static void gemm_wgrad(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, int M, int N, int K,
+                       const float* bias_part, int chunks, float* db) {
+    int splits, kc, nb;
+    wgrad_shape(M, N, K, &splits, &kc, &nb);
     const int tiles = (M / 16) * (N / (16 * nb));
     const size_t lds = ((size_t)splits * 16 * 16 * nb + (size_t)kColsumChunks * 16) * sizeof(float);
     const dim3 grid((unsigned)tiles), block((unsigned)(64 * splits));
@@ -682,30 +748,49 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
                        m->dzf, m->loss);
     int chunks, rc;
     colsum_chunks(n_pad, &chunks, &rc);
-    uint16_t* dz_in = m->dz;  // dZ of the layer being differentiated (row-major)
-    uint16_t* dz_out = m->dz2;
-    // the head's bias-gradient chunk sums, from its fp32 dZ
+    // dZ of the layer being differentiated (row- and column-major) and its
+    // bias-gradient chunk sums; the input-gradient pass writes the layer
+    // below's into the other buffer of each pair (a paired launch still reads
+    // the current ones)
+    uint16_t *dz_in = m->dz, *dz_out = m->dz2;
+    uint16_t *dzt_in = m->dz_t, *dzt_out = m->dz_t2;
+    const float* bias_in = m->split_ws;  // the head's, from its fp32 dZ
+    float* cw_out = m->col_ws;
     hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((D[kLayers] + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
                        m->dzf, n_pad, D[kLayers], rc, m->split_ws);
+    const bool dgelu_ok = rc == 64 && n_pad == chunks * 64;
     for (int l = kLayers - 1; l >= 0; --l) {
-        // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T, and the
-        // bias gradient from the chunk sums (the head's in split_ws, a hidden
-        // layer's left in col_ws by its GELU backward pass)
-        gemm_wgrad(s, m->dz_t, m->act_t[l], m->grad + m->off_w[l], D[l + 1], D[l], n_pad,
-                   l == kLayers - 1 ? m->split_ws : m->col_ws, chunks, m->grad + m->off_b[l]);
-        if (l == 0) break;
-        // input gradient dH[B][in] = dZ[B][out] . W[in][out]^T, then through GELU:
-        // one launch when the bias chunks are 64 rows (block = chunk), else the
-        // product and k_gelu_bwd_colsum
-        if (rc == 64 && n_pad == chunks * 64 && D[l] % 16 == 0) {
-            hipLaunchKernelGGL(k_gemm_dgelu, dim3((unsigned)(chunks * (D[l] / 16))), dim3(256), 0, s, dz_in,
-                               m->wn[l], m->z[l - 1], n_pad, D[l], D[l + 1], dz_out, m->dz_t, m->col_ws);
-            std::swap(dz_in, dz_out);
+        // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T with the
+        // bias gradient from the chunk sums; input gradient dH[B][in] = dZ[B][out]
+        // . W[in][out]^T through the GELU below (l >= 1)
+        float* const dW = m->grad + m->off_w[l];
+        float* const db = m->grad + m->off_b[l];
+        int splits, kc, nb;
+        wgrad_shape(D[l + 1], D[l], n_pad, &splits, &kc, &nb);
+        if (l >= 1 && dgelu_ok && D[l] % 16 == 0 && nb == 1 && splits == 8 && chunks % 2 == 0) {
+            const int wb = (D[l + 1] / 16) * (D[l] / 16);
+            const WgradArgs wa{dzt_in, m->act_t[l], dW, D[l + 1], D[l], n_pad, kc, splits, bias_in, chunks, db};
+            const DgeluArgs da{dz_in, m->wn[l], m->z[l - 1], n_pad, D[l], D[l + 1], dz_out, dzt_out, cw_out};
+            const size_t lds = std::max((size_t)(splits * 16 * 16 + kColsumChunks * 16), (size_t)(2 * (64 * 17 + 64))) *
+                               sizeof(float);
+            hipLaunchKernelGGL(k_wgrad_dgelu, dim3((unsigned)(wb + chunks * (D[l] / 16) / 2)), dim3(512), lds, s, wa,
+                               da, wb);
         } else {
-            gemm(s, dz_in, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);
-            hipLaunchKernelGGL(k_gelu_bwd_colsum, dim3((unsigned)((D[l] + 63) / 64), (unsigned)chunks), dim3(256), 0,
-                               s, m->dh, m->z[l - 1], n_pad, D[l], rc, dz_in, m->dz_t, m->col_ws);
+            gemm_wgrad(s, dzt_in, m->act_t[l], dW, D[l + 1], D[l], n_pad, bias_in, chunks, db);
+            if (l == 0) break;
+            if (dgelu_ok && D[l] % 16 == 0) {
+                hipLaunchKernelGGL(k_gemm_dgelu, dim3((unsigned)(chunks * (D[l] / 16))), dim3(256), 0, s, dz_in,
+                                   m->wn[l], m->z[l - 1], n_pad, D[l], D[l + 1], dz_out, dzt_out, cw_out);
+            } else {
+                gemm(s, dz_in, m->wn[l], m->dh, nullptr, n_pad, D[l], D[l + 1], 0.f);
+                hipLaunchKernelGGL(k_gelu_bwd_colsum, dim3((unsigned)((D[l] + 63) / 64), (unsigned)chunks), dim3(256),
+                                   0, s, m->dh, m->z[l - 1], n_pad, D[l], rc, dz_out, dzt_out, cw_out);
+            }
         }
+        std::swap(dz_in, dz_out);
+        std::swap(dzt_in, dzt_out);
+        bias_in = cw_out;
+        cw_out = cw_out == m->col_ws ? m->col_ws2 : m->col_ws;
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return mfail(LYAP_ERR_HIP, "mlp backward: %s", hipGetErrorString(e));
