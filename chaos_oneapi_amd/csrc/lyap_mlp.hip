@@ -364,14 +364,11 @@ __global__ __launch_bounds__(512) void k_wgrad_dgelu(WgradArgs wa, DgeluArgs da,
 // loss and head gradient: Gaussian NLL on mu1 with the learned log-variance,
 // squared error on mu2, mu3 (targets normalised), logistic loss on escape.
 // Rows b >= n_valid are padding (zero gradient).
-__global__ void k_head_loss(const float* __restrict__ Z, const double* __restrict__ lam,
-                            const uint32_t* __restrict__ status, int n, int n_valid,
-                            const double* __restrict__ ym, const double* __restrict__ ys, float inv_n,
-                            uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
-                            float* __restrict__ dZf, float* __restrict__ loss) {
-    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (b >= n) return;
-    float g[kOutPad];
+// the loss of row b and its gradient g[kOutPad] (zero for padding rows)
+__device__ __forceinline__ float head_row(int b, const float* __restrict__ Z, const double* __restrict__ lam,
+                                          const uint32_t* __restrict__ status, int n_valid,
+                                          const double* __restrict__ ym, const double* __restrict__ ys,
+                                          float inv_n, float (&g)[kOutPad]) {
 #pragma unroll
     for (int j = 0; j < kOutPad; ++j) g[j] = 0.f;
     float l = 0.f;
@@ -396,6 +393,18 @@ __global__ void k_head_loss(const float* __restrict__ Z, const double* __restric
             g[3] = 0.5f * (1.f - d1 * d1 * e) * inv_n;
         }
     }
+    return l;
+}
+
+__global__ void k_head_loss(const float* __restrict__ Z, const double* __restrict__ lam,
+                            const uint32_t* __restrict__ status, int n, int n_valid,
+                            const double* __restrict__ ym, const double* __restrict__ ys, float inv_n,
+                            uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                            float* __restrict__ dZf, float* __restrict__ loss) {
+    const int b = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (b >= n) return;
+    float g[kOutPad];
+    const float l = head_row(b, Z, lam, status, n_valid, ym, ys, inv_n, g);
 #pragma unroll
     for (int j = 0; j < kOutPad; ++j) {
         const uint16_t gb = f2bf(g[j]);
@@ -404,6 +413,61 @@ __global__ void k_head_loss(const float* __restrict__ Z, const double* __restric
         dZf[(size_t)b * kOutPad + j] = g[j];
     }
     if (loss && b < n_valid) atomicAdd(loss, l * inv_n);
+}
+
+// The head in one launch (training): Z = H . W^T + bias on 16x16 MFMA tiles,
+// a block's four waves covering one 64-row chunk, then each row's loss and
+// gradient (as k_head_loss) and the chunk's bias-gradient column sums (as
+// k_colsum_part with 64-row chunks) -- the same operations in the same order.
+__global__ __launch_bounds__(256) void k_gemm_head_loss(const uint16_t* __restrict__ A,
+                                                        const uint16_t* __restrict__ W,
+                                                        const float* __restrict__ bias, int n, int K,
+                                                        float* __restrict__ Z, const double* __restrict__ lam,
+                                                        const uint32_t* __restrict__ status, int n_valid,
+                                                        const double* __restrict__ ym,
+                                                        const double* __restrict__ ys, float inv_n,
+                                                        uint16_t* __restrict__ dZr, uint16_t* __restrict__ dZt,
+                                                        float* __restrict__ dZf, float* __restrict__ loss,
+                                                        float* __restrict__ part) {
+    __shared__ float gt[64][kOutPad + 1];
+    __shared__ float red[4][kOutPad];
+    const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+    const int m0 = (int)blockIdx.x * 64 + w * 16;
+    const int r = lane & 15, q = lane >> 4;
+    v4f acc[1] = {v4f{0.f, 0.f, 0.f, 0.f}};
+    mfma_k_range<1>(A + (size_t)(m0 + r) * K + 4 * q, W + (size_t)r * K + 4 * q, K, 0, K, acc);
+    const float bv = bias[r];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) Z[(size_t)(m0 + 4 * q + i) * kOutPad + r] = 0.f + acc[0][i] + bv;
+    __syncthreads();  // the block's Z rows are read back below (same workgroup)
+    if (threadIdx.x < 64) {
+        const int b = (int)blockIdx.x * 64 + (int)threadIdx.x;
+        float g[kOutPad];
+        const float l = head_row(b, Z, lam, status, n_valid, ym, ys, inv_n, g);
+#pragma unroll
+        for (int j = 0; j < kOutPad; ++j) {
+            const uint16_t gb = f2bf(g[j]);
+            dZr[(size_t)b * kOutPad + j] = gb;
+            dZt[(size_t)j * n + b] = gb;
+            dZf[(size_t)b * kOutPad + j] = g[j];
+            gt[threadIdx.x][j] = g[j];
+        }
+        if (loss && b < n_valid) atomicAdd(loss, l * inv_n);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const int c = (int)(threadIdx.x & 15), g = (int)(threadIdx.x >> 4);
+        float s = 0.f;
+        for (int b = g; b < 64; b += 4) s += gt[b][c];
+        red[g][c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        const int c = (int)(threadIdx.x & 15), g = (int)(threadIdx.x >> 4);
+        red[g][c] += red[g + 2][c];
+    }
+    __syncthreads();
+    if (threadIdx.x < kOutPad) part[(size_t)blockIdx.x * kOutPad + threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x];
 }
 
 // Partial column sums of dZ [n][H] over row chunks: block (x, r) covers
@@ -723,14 +787,16 @@ static void colsum_chunks(int n, int* chunks, int* rc) {
 }
 
 // clear_loss: the loss accumulator is cleared by the first kernel (training)
-static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s, bool clear_loss) {
+// head = false: the hidden layers only (training runs the head with its loss)
+static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipStream_t s, bool clear_loss,
+                   bool head = true) {
     const auto& c = m->cfg;
     const int* D = m->dims;
     hipLaunchKernelGGL(k_fourier, dim3((unsigned)(((int64_t)n_pad * c.n_fourier + 255) / 256)), dim3(256), 0, s,
                        d_p, n_pad, n_valid, m->fourier,
                        c.n_fourier, c.lo1, 1.0 / (c.hi1 - c.lo1), c.lo2, 1.0 / (c.hi2 - c.lo2), m->act[0],
                        m->act_t[0], clear_loss ? m->loss : nullptr);
-    for (int l = 0; l < kLayers; ++l) {
+    for (int l = 0; l < (head ? kLayers : kLayers - 1); ++l) {
         const bool hidden = l + 1 < kLayers;
         gemm(s, m->act[l], m->wt[l], m->z[l], m->master + m->off_b[l], n_pad, D[l + 1], D[l], 0.f,
              hidden ? m->act[l + 1] : nullptr, hidden ? m->act_t[l + 1] : nullptr);
@@ -740,14 +806,29 @@ static int forward(lyap_mlp* m, const double* d_p, int n_pad, int n_valid, hipSt
     return LYAP_SUCCESS;
 }
 
+// the head runs in one launch with its loss when the bias chunks are 64 rows
+static bool head_fused(int n_pad) {
+    int chunks, rc;
+    colsum_chunks(n_pad, &chunks, &rc);
+    return rc == 64 && n_pad == chunks * 64;
+}
+
 static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, int n_pad, int n_valid,
                     hipStream_t s) {
     const int* D = m->dims;
-    hipLaunchKernelGGL(k_head_loss, dim3((n_pad + 255) / 256), dim3(256), 0, s, m->z[kLayers - 1], d_lam,
-                       d_status, n_pad, n_valid, m->norm, m->norm + 3, 1.0f / (float)n_valid, m->dz, m->dz_t,
-                       m->dzf, m->loss);
     int chunks, rc;
     colsum_chunks(n_pad, &chunks, &rc);
+    const bool fused_head = head_fused(n_pad);
+    if (fused_head) {
+        const int l = kLayers - 1;
+        hipLaunchKernelGGL(k_gemm_head_loss, dim3((unsigned)chunks), dim3(256), 0, s, m->act[l], m->wt[l],
+                           m->master + m->off_b[l], n_pad, D[l], m->z[l], d_lam, d_status, n_valid, m->norm,
+                           m->norm + 3, 1.0f / (float)n_valid, m->dz, m->dz_t, m->dzf, m->loss, m->split_ws);
+    } else {
+        hipLaunchKernelGGL(k_head_loss, dim3((n_pad + 255) / 256), dim3(256), 0, s, m->z[kLayers - 1], d_lam,
+                           d_status, n_pad, n_valid, m->norm, m->norm + 3, 1.0f / (float)n_valid, m->dz, m->dz_t,
+                           m->dzf, m->loss);
+    }
     // dZ of the layer being differentiated (row- and column-major) and its
     // bias-gradient chunk sums; the input-gradient pass writes the layer
     // below's into the other buffer of each pair (a paired launch still reads
@@ -756,8 +837,9 @@ static int backward(lyap_mlp* m, const double* d_lam, const uint32_t* d_status, 
     uint16_t *dzt_in = m->dz_t, *dzt_out = m->dz_t2;
     const float* bias_in = m->split_ws;  // the head's, from its fp32 dZ
     float* cw_out = m->col_ws;
-    hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((D[kLayers] + 63) / 64), (unsigned)chunks), dim3(256), 0, s,
-                       m->dzf, n_pad, D[kLayers], rc, m->split_ws);
+    if (!fused_head)
+        hipLaunchKernelGGL(k_colsum_part, dim3((unsigned)((D[kLayers] + 63) / 64), (unsigned)chunks), dim3(256), 0,
+                           s, m->dzf, n_pad, D[kLayers], rc, m->split_ws);
     const bool dgelu_ok = rc == 64 && n_pad == chunks * 64;
     for (int l = kLayers - 1; l >= 0; --l) {
         // weight gradient dW^T[out][in] = dZ^T[out][B] . act^T[in][B]^T with the
@@ -812,7 +894,7 @@ int lyap_mlp_backward(lyap_mlp* m, const double* d_p, const double* d_lambda, co
     hipStream_t s = (hipStream_t)stream;
     MLP_HIP(hipSetDevice(m->device));
     const int n_pad = (int)((n + 63) / 64 * 64);
-    if ((rc = forward(m, d_p, n_pad, (int)n, s, true))) return rc;
+    if ((rc = forward(m, d_p, n_pad, (int)n, s, true, !head_fused(n_pad)))) return rc;
     if ((rc = backward(m, d_lambda, d_status, n_pad, (int)n, s))) return rc;
     if (d_loss) MLP_HIP(hipMemcpyAsync(d_loss, m->loss, 4, hipMemcpyDeviceToDevice, s));
     return LYAP_SUCCESS;
