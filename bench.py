@@ -202,11 +202,12 @@ def bench_surrogate(args, ctx, spec, wl_name):
         R.refine(ctx, spec, **kw)
     torch.cuda.synchronize()
     times, res = [], None
-    for _ in range(args.steps):
+    for k in range(args.steps):
         t0 = time.perf_counter()
         res = R.refine(ctx, spec, **kw)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
+        print(f"step {k}: {times[-1]:.3f} s, phases {res.timings}", file=sys.stderr, flush=True)
     ms_per_step = 1e3 * sum(times) / len(times)
     st = res.stats
     lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
@@ -434,7 +435,8 @@ def main():
 
     ctx = L.Context(0)
     if args.workload == "rossler-2048-surrogate":
-        args.steps = 2 if args.steps is None else args.steps
+        # three steps: single steps vary by up to ~0.6 s (host stalls, clocks)
+        args.steps = 3 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         rec = bench_surrogate(args, ctx, W.rossler_dverk(2048, precision=L.PREC_MIXED),
                               "rossler-2048x2048-dverk65-adaptive-surrogate")
