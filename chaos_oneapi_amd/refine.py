@@ -81,6 +81,24 @@ class RefineResult:
 # coarse step: on the 2048^2 Rossler plane kappa = 1 overshoots the budget by
 # 2-5x, and one network answered 6,584 calibration points at 1.25 (23 wrong,
 # budget 20) but only 4,447 at 1.5 (13 wrong)
+# Side streams of refine(), created once per device: a fresh torch stream per
+# call comes from torch's pool in rotation, and with four hardware queues per
+# process (GPU_MAX_HW_QUEUES) consecutive pool streams land on different
+# queues from call to call, sometimes the main stream's, which serialises a
+# side launch behind the main one (measured: 0.6-0.8 s stalls in every other
+# step).  The same three streams every call keep one placement.
+_SIDE_STREAMS = {}
+
+
+def side_streams(dev):
+    import torch
+
+    key = dev.index if dev.index is not None else torch.cuda.current_device()
+    if key not in _SIDE_STREAMS:
+        _SIDE_STREAMS[key] = tuple(torch.cuda.Stream(dev) for _ in range(3))
+    return _SIDE_STREAMS[key]
+
+
 KAPPAS = (1.0, 1.05, 1.1, 1.15, 1.2, 1.25, 1.3, 1.35, 1.4, 1.45, 1.5, 1.6, 1.75, 2.0, 2.5, 3.0,
           5.0, 8.0, 16.0)
 
@@ -182,7 +200,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
             # per CU (on the 2048^2 Rossler plane the lattice alone takes 594 ms
             # where the three sets in one launch take 819 ms)
             cv = first[n_t:]
-            side_c = torch.cuda.Stream(dev)
+            side_c = side_streams(dev)[0]
             side_c.wait_stream(main)
             cv_ctx = L.Context(ctx.device)
             cv_ctx.set_option("lane_max_blocks", split_blocks_per_cu * n_cus)
@@ -239,7 +257,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     # judge slowed it by ~40 %, beside the rerun it cost the rerun ~0.3 s)
     judge_ctx = None
     if judge:
-        side_j = torch.cuda.Stream(dev)
+        side_j = side_streams(dev)[1]
         side_j.wait_stream(main)
         judge_ctx = L.Context(ctx.device)
         # judge_quad: four lanes per point shorten the judge alone (2048^2
@@ -270,7 +288,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         ov_ctx = L.Context(ctx.device)
         ov_ctx.set_option("lane_max_blocks", overlap_blocks_per_cu * n_cus)
         ov_ctx.set_option("lane_gather", rerun_gather)
-        side_o = torch.cuda.Stream(dev)
+        side_o = side_streams(dev)[2]
         side_o.wait_stream(main)
         with torch.cuda.stream(side_o):
             lam_o, st_o, na_o, nr_o = ov_ctx.run(fd, idx_list=ov_idx, stream=side_o)
