@@ -422,8 +422,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         stats["max_dlambda_chosen_vs_fp64"] = float(np.max(np.abs(lc[both] - lj[both]))) if both.any() else 0.0
         # the final map's class at the verification points, answering with the
         # surrogate wherever the point was not flagged
-        fl = flagged.cpu().numpy()[v]
-        pr = pred.cpu().numpy()[v]
+        fl = flagged[verify].cpu().numpy()
+        pr = pred[verify].cpu().numpy()
         sur_status = np.where(pr[:, 4] >= 0.5, 2, 0)
         cls_final = np.where(fl, classify(lc[:, 0], sc), classify(pr[:, 0], sur_status))
         cls_judge = classify(lj[:, 0], sj)
