@@ -26,11 +26,15 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
 The plain FP64 sweep of the same grid is timed once for the speed-up.  Prints
 one JSON line.
 
-Roofline: the sweep is bound by the vector ALU (FP64, and FP32 without packed
-instructions, both 78.6 TFLOP/s on MI355X); `achieved` is the algorithmic flop
-count of the dominant kernel launch (per-point accepted/rejected step counters
-x per-step flops, chaos_oneapi_amd/flops.py) over that launch's duration from
-HIP events on its stream.
+Roofline: the sweep is bound by the vector ALU; `achieved` is the algorithmic
+flop count of the dominant kernel launch (per-point accepted/rejected step
+counters x per-step flops, chaos_oneapi_amd/flops.py) over that launch's
+duration from HIP events on its stream; `peak` is the published MI355X vector
+peak weighted by the launch's binary64/binary32 flop mix (78.6 / 157.3
+TFLOP/s); the issue rate the valu_peak probe measures on the box (binary64
+and unpacked binary32 FMAs) is reported beside it.  `traffic` comes from the
+rocprofv3 byte-counter summary of the same workload in profiles/ when one is
+there (FETCH_SIZE and WRITE_SIZE scaled by their readings for a plain copy).
 
 cpu_baseline: the CPU FP64 oracle (oracle/lyap_oracle.c, OpenMP) timed on a
 bounded seeded subsample of the same grid; its values double as a parity check.
@@ -49,8 +53,68 @@ sys.path.insert(0, ROOT)
 
 METRIC = ("param points/sec on 1 GPU at |Δλ1|≤1e-3 vs FP64 oracle; "
           "chosen form vs plain FP64")
-PEAK_FP64_VALU_TFLOPS = 78.6
 CHAOS_THRESHOLD = 0.01
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def valu_probe():
+    """The valu_peak probe's measured FMA issue rates (None if not built)."""
+    import subprocess
+
+    exe = os.path.join(ROOT, "chaos_oneapi_amd", "valu_peak")
+    if not os.path.exists(exe):
+        return None
+    r = subprocess.run([exe, "20000", "3"], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        raise RuntimeError(f"valu_peak failed: {r.stderr}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the workload's dominant kernel from the
+    committed byte-counter summary, or None."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f).get(workload)
+    except (OSError, ValueError):
+        return None
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def error_map_stats(lam_a, lam_b, ok, n1, n2, blocks=4):
+    """Per-exponent |lam_a - lam_b| over the points where ok: max, 99.9th
+    percentile, median, points over 1e-3, and a blocks x blocks map of the
+    block maxima over the (p1, p2) grid (p1 outer)."""
+    import numpy as np
+
+    d = np.where(ok[:, None], np.abs(lam_a - lam_b), np.nan)
+    k = d.shape[1]
+    out = {"max": [], "p999": [], "median": [], "over_1e-3": [], "block_max": []}
+    grid = d.reshape(n1, n2, k)
+    e1 = np.linspace(0, n1, blocks + 1).astype(int)
+    e2 = np.linspace(0, n2, blocks + 1).astype(int)
+    for j in range(k):
+        v = d[ok, j]
+        out["max"].append(float(v.max()) if v.size else 0.0)
+        out["p999"].append(float(np.quantile(v, 0.999)) if v.size else 0.0)
+        out["median"].append(float(np.median(v)) if v.size else 0.0)
+        out["over_1e-3"].append(int(np.sum(v > 1e-3)))
+        bm = [[float(np.nanmax(grid[e1[a]:e1[a + 1], e2[b]:e2[b + 1], j]))
+               if np.isfinite(grid[e1[a]:e1[a + 1], e2[b]:e2[b + 1], j]).any() else None
+               for b in range(blocks)] for a in range(blocks)]
+        out["block_max"].append(bm)
+    return out
 
 
 def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int, idx_pool=None):
@@ -77,9 +141,20 @@ def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int, idx_
         chunk = min(4 * chunk, 4096)
     idx = np.concatenate(idx_all)
     lam_o = np.concatenate(lam_all)
+    # one core on a smaller seeded sample (about a quarter of the budget)
+    t1, n1c = 0.0, 0
+    one = rng.permutation(order)[:4096]
+    while t1 < budget_s / 4 and n1c < len(one):
+        t0 = time.perf_counter()
+        O.sweep(od, idx_list=one[n1c:n1c + 4], n_threads=1)
+        t1 += time.perf_counter() - t0
+        n1c += 4
     rec = {"value": done / t_cpu, "unit": "points/s", "cores": threads, "kind": "port",
            "sample": f"{done} seeded random points of the same grid, FP64 oracle, "
-                     f"OpenMP {threads} threads, {t_cpu:.1f} s"}
+                     f"OpenMP {threads} threads, {t_cpu:.1f} s",
+           "single_core_value": n1c / t1 if t1 > 0 else None,
+           "single_core_sample": f"{n1c} seeded random points, 1 thread, {t1:.1f} s",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
     parity = {"oracle_sample": int(done)}
     if lam_fp64 is not None:
         ok = np.isfinite(lam_o[:, 0]) & np.isfinite(lam_fp64[idx, 0])
@@ -149,12 +224,11 @@ def bench_sweep(args, ctx, spec, wl_name):
     ms_per_step = wall * 1e3 / args.steps
     lam_c, st_c = bufs[0].cpu().numpy().copy(), bufs[1].cpu().numpy().copy()
     na_c, nr_c = bufs[2].cpu().numpy().copy(), bufs[3].cpu().numpy().copy()
-    flops_c = F.sweep_flops(desc, na_c, nr_c)
+    f64_c, f32_c = F.sweep_flops_split(desc, na_c, nr_c)
     lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
     ok = (st_c == 0) & (st_p == 0)
     d1 = float(np.max(np.abs(lam_c[ok, 0] - lam_p[ok, 0]))) if ok.any() else 0.0
     cls = float(np.mean((lam_c[ok, 0] > CHAOS_THRESHOLD) == (lam_p[ok, 0] > CHAOS_THRESHOLD)))
-    achieved = flops_c / (kern_avg * 1e-3) / 1e12
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
     rec["config"] = {
         "workload": wl_name, "points": n,
@@ -168,7 +242,12 @@ def bench_sweep(args, ctx, spec, wl_name):
         "classification_agreement": cls, "points_status_ok": int(ok.sum()),
         "mean_steps_per_point": float(na_c.mean()), "mean_rejects_per_point": float(nr_c.mean()),
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
+        # per-exponent error maps of the chosen form against the plain FP64 sweep
+        "dlambda_chosen_vs_plain_fp64_per_exponent": error_map_stats(lam_c, lam_p, ok, desc.n1, desc.n2),
     }
+    if args.error_maps:
+        np.savez_compressed(args.error_maps, lam_chosen=lam_c, lam_fp64=lam_p, status_chosen=st_c,
+                            status_fp64=st_p, n1=desc.n1, n2=desc.n2)
     if tuned_lanes is not None:
         rec["config"]["lane_shape"] = {"lane_max_blocks": int(tuned_lanes.best[0]),
                                        "lane_gather": int(tuned_lanes.best[1]),
@@ -179,7 +258,8 @@ def bench_sweep(args, ctx, spec, wl_name):
         rec["config"]["l96_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}
     kname = "k_var_lane" if desc.model != L.MODEL_L96 else (
         "k_l96_wave" if tuned is not None and tuned.best[0] == "ppw" else "k_l96")
-    rec["roofline"] = roofline(achieved, kname + " (mixed) over the grid")
+    rec["roofline"] = roofline(f64_c, f32_c, kern_avg, kname + " (mixed) over the grid", args.probe,
+                               measured_traffic(wl_name))
     if not args.no_cpu:
         cpu, parity = cpu_baseline(desc, lam_c, lam_p, args.cpu_budget, args.cpu_threads)
         rec["cpu_baseline"] = cpu
@@ -211,59 +291,105 @@ def bench_surrogate(args, ctx, spec, wl_name):
     ms_per_step = 1e3 * sum(times) / len(times)
     st = res.stats
     lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
+    print(f"plain FP64 sweep {plain_ms:.0f} ms", file=sys.stderr, flush=True)
+    # the mixed form alone over the whole plane (no surrogate), with the
+    # rerun's renormalisation gather: what the surrogate adds to the mixed form
+    stream = torch.cuda.current_stream()
+    ctx.set_option("lane_gather", 4)
+    try:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        lam_m, st_m, _, _ = ctx.run(desc)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        mixed_ms = e0.elapsed_time(e1)
+    finally:
+        ctx.set_option("lane_gather", 0)
+    lam_m, st_m = lam_m.cpu().numpy(), st_m.cpu().numpy()
+    print(f"mixed-only sweep {mixed_ms:.0f} ms", file=sys.stderr, flush=True)
     # the final map against the plain FP64 map over the whole plane
     okp = st_p == 0
     cls_final = R.classify(np.nan_to_num(res.lam[:, 0], nan=0.0), res.status)
     cls_plain = R.classify(lam_p[:, 0], st_p)
     computed = res.rerun & okp & (res.status == 0)
+    answered = ~res.rerun
+    ans_val = answered & okp & (res.status == 0)
+    dl_s = np.abs(res.lam[ans_val, 0] - lam_p[ans_val, 0])
     # the step's mixed-form launches of k_var_lane (the lattice launch, the
     # calibration and verification launch beside it, the rerun, and the optional
-    # launch beside training): total algorithmic flops
-    # over total duration = the average flops per launch over the average
-    # launch duration
+    # launch beside training): total algorithmic flops over total duration = the
+    # average flops per launch over the average launch duration
     per = F.per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
+    f32_acc = F.per_step_split(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive), True)
     renorms = F.mgs(desc.dim, desc.dim) * (desc.n_renorm_transient + desc.n_renorm_average)
-    fl = (per[0] * (st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"])
-          + per[1] * (st["rerun_rej_steps"] + st["first_rej_steps"] + st["overlap_rej_steps"])
-          + renorms * (st["rerun_points"] + st["first_points"] + st["overlap_points"]))
+    acc = st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"]
+    rej = st["rerun_rej_steps"] + st["first_rej_steps"] + st["overlap_rej_steps"]
+    fl = per[0] * acc + per[1] * rej + renorms * (st["rerun_points"] + st["first_points"] + st["overlap_points"])
+    f32 = f32_acc * acc
     launch_ms = [st["first_kernel_ms"]]
     if st.get("split_first"):
         launch_ms.append(st["split_kernel_ms"])
     launch_ms.append(st["rerun_kernel_ms"])
     if st["overlap_points"]:
         launch_ms.append(st["overlap_kernel_ms"])
-    achieved = fl / (sum(launch_ms) * 1e-3) / 1e12
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
     dl_c = np.abs(res.lam[computed, 0] - lam_p[computed, 0])
     rec["config"] = {
         "workload": wl_name, "points": n, "chosen_form": "mixed + surrogate refinement",
         "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
-        "rerun_fraction": st["rerun_fraction"], "train_points": st["train_points"],
+        "mixed_only_points_per_s": n / (mixed_ms * 1e-3),
+        "speedup_mixed_only_vs_plain_fp64": plain_ms / mixed_ms,
+        "surrogate_gain_vs_mixed_only": mixed_ms / ms_per_step,
+        "rerun_fraction": st["rerun_fraction"], "surrogate_answered_fraction": 1.0 - st["rerun_fraction"],
+        "train_points": st["train_points"],
         "verify_points": st["verify"], "calibration_points": st["calib"],
         "std_multiplier_kappa": st["kappa"] if math.isfinite(st["kappa"]) else None,
         "escape_answers": st["escape_answers"], "calibration_agreement": st["calib_agreement"],
-        "calibration_table": [[k if math.isfinite(k) else None, e, a, w]
-                              for k, e, a, w in st["calib_table"]],
+        "calibration_table": [[k if math.isfinite(k) else None, e, a, w, d]
+                              for k, e, a, w, d in st["calib_table"]],
+        # the gates on the held-out verification subsample: the chosen form's
+        # answer at each point (the surrogate's where the rule answers it, the
+        # method's elsewhere) against the FP64 judge
+        "gate_met_verify": st.get("gate_met"),
         "max_dlambda1_chosen_vs_fp64_verify": st.get("max_dlambda1_chosen_vs_fp64"),
+        "dlambda1_chosen_vs_fp64_verify_p999": st.get("dlambda1_chosen_vs_fp64_p999"),
+        "chosen_points_over_1e-3_verify": st.get("chosen_points_over_gate"),
         "class_agreement_verify": st.get("class_agreement"),
         "surrogate_answered_verify": st.get("surrogate_answered_verify"),
+        "surrogate_valued_answers_verify": st.get("surrogate_valued_answers_verify"),
+        "max_dlambda1_surrogate_answers_verify": st.get("max_dlambda1_surrogate_answers"),
         "surrogate_class_agreement_verify": st.get("surrogate_class_agreement"),
+        "max_dlambda1_computed_vs_fp64_verify": st.get("max_dlambda1_computed_vs_fp64"),
+        "max_dlambda_method_vs_fp64_verify_per_exponent": st.get("max_dlambda_method_vs_fp64_per_exponent"),
+        # over the whole plane, against the plain FP64 sweep
+        "surrogate_valued_answers_full_plane": int(ans_val.sum()),
+        "max_dlambda1_surrogate_answers_full_plane": float(dl_s.max()) if dl_s.size else 0.0,
+        "surrogate_answers_over_1e-3_full_plane": int(np.sum(dl_s > 1e-3)),
         "max_dlambda1_computed_vs_plain_fp64": float(np.max(dl_c)) if dl_c.size else 0.0,
         "computed_points_over_1e-3_vs_plain_fp64": int(np.sum(dl_c > 1e-3)),
         "dlambda1_computed_vs_plain_fp64_p999": float(np.quantile(dl_c, 0.999)) if dl_c.size else 0.0,
+        "class_agreement_full_plane": float(np.mean(cls_final == cls_plain)),
+        "class_agreement_answered_full_plane": float(np.mean(cls_final[answered] == cls_plain[answered])) if answered.any() else 1.0,
+        # per-exponent error maps of the mixed form against FP64 over the plane
+        "dlambda_mixed_vs_plain_fp64_per_exponent": error_map_stats(lam_m, lam_p, (st_m == 0) & okp, desc.n1, desc.n2),
         "rerun_lane_gather": st.get("rerun_gather"),
         "overlap_points": st.get("overlap_points"),
-        "class_agreement_full_plane": float(np.mean(cls_final == cls_plain)),
         "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
+    if args.error_maps:
+        np.savez_compressed(args.error_maps, lam_chosen=res.lam, lam_mixed=lam_m, lam_fp64=lam_p,
+                            status_chosen=res.status, status_fp64=st_p, answered=answered,
+                            n1=desc.n1, n2=desc.n2)
     parts = ["lattice launch", "calibration/verification launch beside it"] if st.get("split_first") \
         else ["first launch"]
     if st["overlap_points"]:
         parts.append("overlap launch")
     parts.append("rerun")
-    rec["roofline"] = roofline(achieved, "k_var_lane (mixed): " + ", ".join(parts))
+    # the launches' flops over their summed durations (each charged its own)
+    rec["roofline"] = roofline(fl - f32, f32, sum(launch_ms), "k_var_lane (mixed): " + ", ".join(parts),
+                               args.probe, measured_traffic(wl_name))
     rec["roofline"]["launch_ms"] = launch_ms
     rec["roofline"]["avg_launch_ms"] = sum(launch_ms) / len(launch_ms)
     if not args.no_cpu:
@@ -317,7 +443,8 @@ def bench_batch(args, ctx, n_points, wl_name):
     ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
     outs = [tuple(t.cpu().numpy() for t in o) for o in b.outputs]
     b.close()
-    fl = sum(F.sweep_flops(c["desc"], o[2], o[3]) for c, o in zip(classes, outs))
+    split = [F.sweep_flops_split(c["desc"], o[2], o[3]) for c, o in zip(classes, outs)]
+    f64_b, f32_b = sum(x[0] for x in split), sum(x[1] for x in split)
     # the same classes as separate single-class launches of the same kernels
     # (the batch layout's Lorenz-96 points per wave)
     single_ms = 0.0
@@ -348,10 +475,14 @@ def bench_batch(args, ctx, n_points, wl_name):
     for c, o, o64 in zip(mix, outs, outs64):
         ok = (o[1] == 0) & (o64[1] == 0)
         d1 = float(np.max(np.abs(o[0][ok, 0] - o64[0][ok, 0]))) if ok.any() else 0.0
+        dk = np.abs(o[0][ok] - o64[0][ok])
         worst = max(worst, d1)
         per_class[c["name"]] = {"points": c["count"], "status_ok": int(ok.sum()),
                                 "mean_steps": float(o[2].mean()), "max_steps": int(o[2].max()),
-                                "max_dlambda1_vs_fp64": d1}
+                                "max_dlambda1_vs_fp64": d1,
+                                "points_over_1e-3_lambda1": int(np.sum(dk[:, 0] > 1e-3)) if dk.size else 0,
+                                "max_dlambda_vs_fp64_per_exponent": [float(x) for x in dk.max(axis=0)] if dk.size else [],
+                                "dlambda_vs_fp64_p999_per_exponent": [float(x) for x in np.quantile(dk, 0.999, axis=0)] if dk.size else []}
     kern_avg = sum(kern) / len(kern)
     rec = base_record(args, n_points / (ms_per_step * 1e-3), ms_per_step, classes[0]["desc"])
     rec["config"] = {
@@ -363,8 +494,54 @@ def bench_batch(args, ctx, n_points, wl_name):
         "single_class_launches_ms": single_ms, "batched_kernel_ms": kern_avg,
         "max_dlambda1_chosen_vs_plain_fp64": worst, "classes": per_class,
     }
-    rec["roofline"] = roofline(fl / (kern_avg * 1e-3) / 1e12, "k_batch (mixed) persistent launch")
+    rec["roofline"] = roofline(f64_b, f32_b, kern_avg, "k_batch (mixed) persistent launch", args.probe,
+                               measured_traffic(wl_name))
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline_batch(mix, outs64, args.cpu_budget, args.cpu_threads)
+        rec["cpu_baseline"] = cpu
+        rec["config"].update(parity)
     return rec
+
+
+def cpu_baseline_batch(mix, outs64, budget_s, threads):
+    """The CPU FP64 oracle on a seeded sample of every class of the batch:
+    each class gets a share of the budget in proportion to its share of the
+    points; the mix's rate is 1 / sum_c share_c * (seconds per point of c).
+    The sample's values are compared with the FP64 batch's."""
+    import numpy as np
+
+    from chaos_oneapi_amd import lyap as L
+    from oracle import oracle as O
+
+    rng = np.random.default_rng(12346)
+    total = sum(c["count"] for c in mix)
+    sec_per_point, sampled, worst = 0.0, {}, 0.0
+    t_all = 0.0
+    for c, o64 in zip(mix, outs64):
+        share = c["count"] / total
+        fd = L.finalize(L.make_desc(dict(c["spec"], precision=L.PREC_FP64)))
+        od = O.make_desc(fd.to_dict())
+        order = rng.permutation(c["count"])
+        done, t_c, chunk = 0, 0.0, max(threads, 16)
+        while t_c < budget_s * max(share, 0.05) and done < len(order):
+            idx = np.sort(order[done:done + chunk])
+            p = np.ascontiguousarray(c["params"][idx])
+            t0 = time.perf_counter()
+            lam_o, st_o, _, _ = O.sweep(od, first=0, count=len(idx), params_in=p, n_threads=threads)
+            t_c += time.perf_counter() - t0
+            ok = (st_o == 0) & (o64[1][idx] == 0)
+            if ok.any():
+                worst = max(worst, float(np.max(np.abs(lam_o[ok] - o64[0][idx][ok]))))
+            done += len(idx)
+            chunk = min(2 * chunk, 1024)
+        sec_per_point += share * t_c / done
+        t_all += t_c
+        sampled[c["name"]] = done
+    rec = {"value": 1.0 / sec_per_point, "unit": "points/s", "cores": threads, "kind": "port",
+           "sample": "seeded samples per class " + ", ".join(f"{k} {v}" for k, v in sampled.items())
+                     + f", FP64 oracle, OpenMP {threads} threads, {t_all:.1f} s; rate of the 40/30/20/8/2 mix",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+    return rec, {"oracle_sample": int(sum(sampled.values())), "max_dlambda_fp64_vs_oracle": worst}
 
 
 def bench_hbm_copy(args):
@@ -399,9 +576,26 @@ def base_record(args, value, ms_per_step, desc):
     }
 
 
-def roofline(achieved, kernel):
-    return {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_VALU_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / PEAK_FP64_VALU_TFLOPS, "traffic": None, "kernel": kernel}
+def roofline(f64, f32, ms, kernel, probe=None, traffic=None):
+    """Roofline record of a launch (or launches) doing f64 + f32 algorithmic
+    flops in ms milliseconds: against the precision-weighted published peak,
+    and against the binary64 / unpacked binary32 FMA issue rate measured by
+    the valu_peak probe (the rate this build's instructions can reach)."""
+    from chaos_oneapi_amd import flops as F
+
+    achieved = (f64 + f32) / (ms * 1e-3) / 1e12
+    peak = F.weighted_peak(f64, f32)
+    rec = {"bound": "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+           "frac": achieved / peak, "traffic": traffic, "kernel": kernel,
+           "flops_fp64": f64, "flops_fp32": f32, "launch_ms": ms,
+           "peak_basis": "published vector peaks 78.6 (fp64) / 157.3 (fp32) TFLOP/s weighted by the flop mix"}
+    if probe is not None:
+        tot = f64 + f32
+        meas = tot / (f64 / probe["fp64_fma_tflops"] + f32 / probe["fp32_unpacked_fma_tflops"]) if tot else 0.0
+        rec["measured_issue_peak"] = meas
+        rec["frac_of_measured_issue_peak"] = achieved / meas if meas else None
+        rec["valu_probe"] = probe
+    return rec
 
 
 def main():
@@ -417,6 +611,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--fourier-sigma", type=float, default=6.0)
     ap.add_argument("--batch-points", type=int, default=1 << 20)
+    ap.add_argument("--error-maps", default=None,
+                    help="write the full per-point lambda maps (.npz) to this path")
+    ap.add_argument("--no-probe", action="store_true", help="skip the valu_peak issue-rate probe")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a "
@@ -433,6 +630,9 @@ def main():
     from chaos_oneapi_amd import lyap as L
     from chaos_oneapi_amd import workloads as W
 
+    # the issue-rate probe runs before this process touches the GPU (a child
+    # process of its own)
+    args.probe = None if args.no_probe else valu_probe()
     ctx = L.Context(0)
     if args.workload == "rossler-2048-surrogate":
         # three steps: single steps vary by up to ~0.6 s (host stalls, clocks)

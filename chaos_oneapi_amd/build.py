@@ -5,7 +5,9 @@
 Every HIP/C++ source under csrc/ is compiled by hipcc for gfx950 with
 -ffp-contract=off (the FP64 path is bit-reproducible against the CPU oracle)
 and with packed FP32 instructions disabled.  Objects go to _build/, the
-library to chaos_oneapi_amd/liblyap.so, the driver to chaos_oneapi_amd/chaos_lyap.
+library to chaos_oneapi_amd/liblyap.so, the driver to chaos_oneapi_amd/chaos_lyap,
+and the vector-ALU peak probe (csrc/probe/valu_peak.hip, read by bench.py's
+roofline) to chaos_oneapi_amd/valu_peak.
 """
 from __future__ import annotations
 
@@ -22,6 +24,8 @@ HOST = os.path.join(CSRC, "host")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "liblyap.so")
 CLI = os.path.join(PKG, "chaos_lyap")
+PROBE_SRC = os.path.join(CSRC, "probe", "valu_peak.hip")
+PROBE = os.path.join(PKG, "valu_peak")
 
 ARCH = os.environ.get("CHAOS_LYAP_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
@@ -84,6 +88,12 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"driver link failed\n{r.stdout}\n{r.stderr}")
         os.replace(CLI + ".tmp", CLI)
+    if os.path.exists(PROBE_SRC):
+        cmd = [HIPCC, "-O3", "-std=c++17", "-x", "hip"] + DEVICE + [PROBE_SRC, "-o", PROBE + ".tmp"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"probe build failed\n{r.stdout}\n{r.stderr}")
+        os.replace(PROBE + ".tmp", PROBE)
     return LIB
 
 

@@ -13,6 +13,18 @@ counted.  Counts follow the operations the kernels perform:
   MGS            per vector j: j*(4N-1) + (2N-1) + N + 2
 
 Per point: n_acc * (base + K * tangent) + n_rej * base + renorms * MGS.
+
+Precision split of the mixed form: the base step, error norm, MGS and log
+accumulation are binary64; a tangent vector's stage arguments, Jacobian
+products and stage sums are binary32 except the last add of its increment to
+the binary64 deviation vector (N flops per vector and step).
+
+Peaks (MI355X, published): binary64 vector 78.6 TFLOP/s, binary32 vector
+157.3 TFLOP/s (the latter counts packed two-wide FMAs, which this build does
+not emit; unpacked binary32 FMAs issue at the binary64 rate, measured by the
+valu_peak probe).  A launch's precision-weighted peak is the rate at which
+its flop mix would run if every flop ran at its precision's peak:
+total / (f64 / P64 + f32 / P32).
 """
 from __future__ import annotations
 
@@ -59,6 +71,38 @@ def per_step(model: int, n: int, k: int, integ: int, adaptive: bool):
     tan_st = _stages(integ, adaptive, for_tangent=True)
     tangent = len(tan_st) * jv_cost + n * sum(2 * z + 1 for z in tan_st if z > 0) + n * (2 * nnz_b + 1)
     return base + k * tangent, base
+
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+
+
+def per_step_split(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool):
+    """Binary32 flops of an accepted step (the rest of per_step's accepted
+    count is binary64; a rejected step is all binary64)."""
+    if not mixed:
+        return 0
+    acc, base = per_step(model, n, k, integ, adaptive)
+    return (acc - base) - k * n
+
+
+def weighted_peak(f64: float, f32: float) -> float:
+    """Precision-weighted published peak (TFLOP/s) of a flop mix."""
+    tot = f64 + f32
+    if tot <= 0:
+        return PEAK_FP64_TFLOPS
+    return tot / (f64 / PEAK_FP64_TFLOPS + f32 / PEAK_FP32_TFLOPS)
+
+
+def sweep_flops_split(desc, n_acc, n_rej):
+    """(binary64 flops, binary32 flops) of a sweep from its step counters."""
+    import numpy as np
+
+    total = sweep_flops(desc, n_acc, n_rej)
+    f32_step = per_step_split(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive),
+                              desc.precision == L.PREC_MIXED)
+    f32 = float(f32_step * np.asarray(n_acc, dtype=np.float64).sum())
+    return total - f32, f32
 
 
 def mgs(n: int, k: int) -> int:

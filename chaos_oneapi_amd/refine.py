@@ -12,8 +12,9 @@ Pipeline, all on the device through liblyap.so:
      calibration and verification subsamples (calib_frac, verify_frac of the
      grid, disjoint); the answer rule (std multiplier kappa >= 1, escape
      answers on or off) is the one that answers the most calibration points
-     while the map's class agreement there stays >= calib_target (the
-     predicted std alone is not calibrated);
+     while the map's class agreement there stays >= calib_target and every
+     lambda1 it answers there is within dlam_cal (half the 1e-3 gate) of the
+     method's value (the predicted std alone is not calibrated);
   5. the plain FP64 method judges the verification subsample.
 With longest_first the rerun takes its points longest first: a point's
 expected cost is the step count of its nearest training-lattice point, and
@@ -34,10 +35,11 @@ its persistent grid capped so the training kernels keep room.  Off by
 default: on the 2048^2 Rossler plane it saved at most 0.16 s of 9.3 s and
 made training time erratic (0.99 s, once 1.98 s, against 0.75 s alone).
 The returned map takes the method's values where it ran and the surrogate's
-elsewhere.  Gates reported on the verification subsample: max |dlambda1|
-between the chosen form and FP64 on computed points, and agreement of the
-chaotic/regular/escaped class between the final map (surrogate answers
-included) and FP64.
+elsewhere.  Gates reported on the held-out verification subsample
+(verify_gates): max |dlambda1| between the chosen form's answer (the
+surrogate's where the rule answers the point, the method's elsewhere) and
+FP64, and agreement of the chaotic/regular/escaped class between the same
+answers and FP64.
 """
 from __future__ import annotations
 
@@ -76,11 +78,6 @@ class RefineResult:
     stats: dict = field(default_factory=dict)
 
 
-# std multipliers the calibration tries; a fine grid (steps of 0.05 up to 1.5)
-# lets the rule spend the wrong-answer budget instead of stopping at the next
-# coarse step: on the 2048^2 Rossler plane kappa = 1 overshoots the budget by
-# 2-5x, and one network answered 6,584 calibration points at 1.25 (23 wrong,
-# budget 20) but only 4,447 at 1.5 (13 wrong)
 # Side streams of refine(), created once per device: a fresh torch stream per
 # call comes from torch's pool in rotation, and with four hardware queues per
 # process (GPU_MAX_HW_QUEUES) consecutive pool streams land on different
@@ -99,28 +96,54 @@ def side_streams(dev):
     return _SIDE_STREAMS[key]
 
 
+# std multipliers the calibration tries.  The answer rule kappa * std <= std_thr
+# (std_thr = 0.01) admits predicted stds up to std_thr / kappa; an answered
+# lambda1 must also be within the accuracy gate (1e-3) of the method's value,
+# which needs stds of a few 1e-4, hence the multipliers up to 128 (std <= 7.8e-5).
+# inf: the surrogate answers no lambda (only confident escapes, when allowed).
 KAPPAS = (1.0, 1.05, 1.1, 1.15, 1.2, 1.25, 1.3, 1.35, 1.4, 1.45, 1.5, 1.6, 1.75, 2.0, 2.5, 3.0,
-          5.0, 8.0, 16.0)
+          4.0, 5.0, 6.0, 8.0, 12.0, 16.0, 24.0, 32.0, 48.0, 64.0, 96.0, 128.0, float("inf"))
+
+# accuracy gate of the chosen form: max |dlambda1| against the FP64 method
+DLAMBDA1_GATE = 1e-3
 
 
 def answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes=True):
     """Points the surrogate answers at std multiplier kappa: confidently
     escaping (when escapes is set), or decided (not escaping), away from
-    lambda1 = 0 and with kappa * std below the threshold.  Works on numpy
-    arrays and torch tensors."""
-    settled = (kappa * std1 <= std_thr) & (abs(mu1) >= lam_thr) & (pbad <= bad_lo)
+    lambda1 = 0 and with kappa * std below the threshold (kappa inf: no
+    lambda answers).  Works on numpy arrays and torch tensors."""
+    if kappa == float("inf"):
+        settled = pbad < -1.0  # no lambda answers (probabilities are >= 0)
+    else:
+        settled = (kappa * std1 <= std_thr) & (abs(mu1) >= lam_thr) & (pbad <= bad_lo)
     return ((pbad >= bad_hi) | settled) if escapes else settled
 
 
-def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target):
-    """The answer rule that answers the most calibration points while keeping
-    the final map's class agreement on the calibration subsample at or above
+def answer_errors(pred, lam, status, answered):
+    """|mu1 - lambda1| at the answered points where the surrogate answers a
+    value (escape probability < 0.5) and the method's point is OK; the other
+    answered points are judged by their class only."""
+    import numpy as np
+
+    both = answered & (pred[:, 4] < 0.5) & (status == 0) & np.isfinite(lam[:, 0])
+    return np.abs(pred[both, 0].astype(np.float64) - lam[both, 0])
+
+
+def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target,
+                    dlam_cal=0.5 * DLAMBDA1_GATE):
+    """The answer rule that answers the most calibration points while (1) the
+    final map's class agreement on the calibration subsample stays at or above
     `target` (the method's own points agree by construction, so this bounds
-    the wrong answers: wrong <= (1 - target) * n).  Rules: kappa in KAPPAS,
-    with or without the escape answers (which kappa does not scale).
-    Returns (kappa, escapes, agreement, answered, table); kappa inf: the
-    surrogate answers nothing.  pred [n,5] numpy, lam/status the method's
-    values at the same points; table lists (kappa, escapes, answered, wrong)."""
+    the wrong answers: wrong <= (1 - target) * n) and (2) every lambda1 it
+    answers there is within dlam_cal of the method's value (half the 1e-3
+    accuracy gate, so the held-out verification subsample, which sees other
+    points of the same error distribution, keeps a margin).  Rules: kappa in
+    KAPPAS, with or without the escape answers (which kappa does not scale).
+    Returns (kappa, escapes, agreement, answered, table); kappa inf and no
+    escapes: the surrogate answers nothing.  pred [n,5] numpy, lam/status the
+    method's values at the same points; table lists (kappa, escapes, answered,
+    wrong, max |dlambda1| of the answered values)."""
     import numpy as np
 
     n = len(pred)
@@ -137,10 +160,59 @@ def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target)
                            bad_hi, escapes) & finite
             n_a = int(a.sum())
             wrong = int(np.sum(sur[a] != computed[a]))
-            table.append((kappa, escapes, n_a, wrong))
-            if n_a > 0 and wrong <= budget and n_a > best[3]:
+            err = answer_errors(pred, lam, status, a)
+            dmax = float(err.max()) if err.size else 0.0
+            table.append((kappa, escapes, n_a, wrong, dmax))
+            if n_a > 0 and wrong <= budget and dmax <= dlam_cal and n_a > best[3]:
                 best = (kappa, escapes, 1.0 - wrong / n, n_a)
     return best + (table,)
+
+
+def verify_gates(flagged, pred, lam_m, st_m, lam_j, st_j):
+    """The gates of the chosen form on the held-out verification points.
+
+    flagged [n] bool: the answer rule does not answer the point (the method
+    computes it); pred [n,5] the surrogate's outputs; lam_m/st_m the method's
+    values in the chosen precision (every verification point is computed, so
+    these are what the rerun would give); lam_j/st_j the FP64 judge.  The
+    chosen form's answer at a point is the surrogate's where the rule answers
+    it and the method's elsewhere, exactly as for a point outside the
+    subsample.  |dlambda1| is taken where the chosen answer is a value and the
+    judge's point is OK; the class (chaotic / regular / escaped) everywhere."""
+    import numpy as np
+
+    ans = ~flagged
+    sur_status = np.where(pred[:, 4] >= 0.5, 2, 0)
+    chosen_l1 = np.where(ans, pred[:, 0].astype(np.float64), lam_m[:, 0])
+    chosen_ok = np.where(ans, sur_status == 0, st_m == 0)
+    both = chosen_ok & (st_j == 0)
+    d1 = np.abs(chosen_l1[both] - lam_j[both, 0])
+    d1_sur = np.abs(chosen_l1[both & ans] - lam_j[both & ans, 0])
+    comp = flagged & (st_m == 0) & (st_j == 0)
+    d1_comp = np.abs(lam_m[comp, 0] - lam_j[comp, 0])
+    cm = (st_m == 0) & (st_j == 0)
+    dk = np.abs(lam_m[cm] - lam_j[cm])
+    cls_final = np.where(ans, classify(pred[:, 0], sur_status), classify(lam_m[:, 0], st_m))
+    cls_judge = classify(lam_j[:, 0], st_j)
+    agree = cls_final == cls_judge
+    out = dict(
+        verify_points_valued=int(both.sum()),
+        max_dlambda1_chosen_vs_fp64=float(d1.max()) if d1.size else 0.0,
+        dlambda1_chosen_vs_fp64_p999=float(np.quantile(d1, 0.999)) if d1.size else 0.0,
+        chosen_points_over_gate=int(np.sum(d1 > DLAMBDA1_GATE)),
+        max_dlambda1_surrogate_answers=float(d1_sur.max()) if d1_sur.size else 0.0,
+        surrogate_valued_answers_verify=int(d1_sur.size),
+        max_dlambda1_computed_vs_fp64=float(d1_comp.max()) if d1_comp.size else 0.0,
+        class_agreement=float(np.mean(agree)) if agree.size else 1.0,
+        surrogate_answered_verify=int(ans.sum()),
+        surrogate_class_agreement=float(np.mean(agree[ans])) if ans.any() else 1.0,
+        # the method in the chosen precision against FP64, every exponent
+        # (all verification points, answered or not)
+        max_dlambda_method_vs_fp64_per_exponent=[float(x) for x in dk.max(axis=0)] if dk.size else [],
+        dlambda_method_vs_fp64_p999_per_exponent=[float(x) for x in np.quantile(dk, 0.999, axis=0)] if dk.size else [],
+    )
+    out["gate_met"] = bool(out["max_dlambda1_chosen_vs_fp64"] <= DLAMBDA1_GATE and out["class_agreement"] >= 0.999)
+    return out
 
 
 def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2048,
@@ -148,7 +220,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
            lam_thr: float = 0.02, bad_lo: float = 0.02, bad_hi: float = 0.98,
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
-           calib_target: float = 0.9995, longest_first: bool = False,
+           calib_target: float = 0.9995, dlam_cal: float = 0.5 * DLAMBDA1_GATE,
+           longest_first: bool = False,
            rerun_gather: int = 4, overlap_points: int = 0, overlap_frac: float = 1 / 16,
            overlap_blocks_per_cu: int = 2, first_blocks_per_cu: int = 2,
            split_first: bool = True, split_blocks_per_cu: int = 1,
@@ -331,10 +404,10 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         main.wait_stream(side_c)
     kappa, escapes, cal_agree, cal_answered, cal_table = calibrate_kappa(
         pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
-        bad_lo, bad_hi, calib_target)
+        bad_lo, bad_hi, calib_target, dlam_cal)
     tm["calibrate"] = time.perf_counter() - t
     mu1, std1, pbad = pred[:, 0], pred[:, 3], pred[:, 4]
-    if kappa == float("inf"):
+    if cal_answered == 0:
         flagged = torch.ones(n, dtype=torch.bool, device=dev)
     else:
         flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes)
@@ -417,21 +490,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         v = verify.cpu().numpy()
         lj, sj = lam_j.cpu().numpy(), st_j.cpu().numpy()
         lc, sc = lam_v.cpu().numpy(), st_v.cpu().numpy()
-        both = (sc == 0) & (sj == 0)
-        stats["max_dlambda1_chosen_vs_fp64"] = float(np.max(np.abs(lc[both, 0] - lj[both, 0]))) if both.any() else 0.0
-        stats["max_dlambda_chosen_vs_fp64"] = float(np.max(np.abs(lc[both] - lj[both]))) if both.any() else 0.0
-        # the final map's class at the verification points, answering with the
-        # surrogate wherever the point was not flagged
-        fl = flagged[verify].cpu().numpy()
-        pr = pred[verify].cpu().numpy()
-        sur_status = np.where(pr[:, 4] >= 0.5, 2, 0)
-        cls_final = np.where(fl, classify(lc[:, 0], sc), classify(pr[:, 0], sur_status))
-        cls_judge = classify(lj[:, 0], sj)
-        stats["class_agreement"] = float(np.mean(cls_final == cls_judge))
-        stats["surrogate_answered_verify"] = int((~fl).sum())
-        stats["surrogate_class_agreement"] = float(np.mean(cls_final[~fl] == cls_judge[~fl])) if (~fl).any() else 1.0
-        sur_ok = (~fl) & (sj == 0) & (sur_status == 0)
-        stats["surrogate_max_dlambda1"] = float(np.max(np.abs(pr[sur_ok, 0] - lj[sur_ok, 0]))) if sur_ok.any() else 0.0
+        stats.update(verify_gates(flagged[verify].cpu().numpy(), pred[verify].cpu().numpy(),
+                                  lam_v.cpu().numpy(), st_v.cpu().numpy(), lam_j.cpu().numpy(),
+                                  st_j.cpu().numpy()))
     net.close()
     return RefineResult(lam=lam.cpu().numpy(), status=status.cpu().numpy(),
                         rerun=rerun_mask.cpu().numpy(), timings=tm, stats=stats)

@@ -778,6 +778,21 @@ static int ctrl_update(ctrl_t* c, int integ, int ok, int above_floor, double ler
     return 0;
 }
 
+/* One decision of the step controller on an error norm `err` (not its
+   log), exported so that tests can hold it against the reference's pow-form
+   rule (cuda/tools/dopri.dp.cpp:9-12,27-29,109-139): the controller state is
+   facOld as its log and the rejected-last-step flag, both updated in place.
+   Returns 1 on accept; *h_next is the next attempt. */
+int orc_ctrl_update(int integ, double err, double* log_fac_old, int* rejected, double h_try,
+                    double h_max, double* h_next) {
+    ctrl_t c = {*log_fac_old, *rejected};
+    const int acc = ctrl_update(&c, integ, err <= 1.0, err > 1e-4, orc_ctrl_log(err),
+                                orc_ctrl_log(1e-4), h_try, h_max, h_next);
+    *log_fac_old = c.log_fac_old;
+    *rejected = c.rejected;
+    return acc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* points                                                                    */
 /* ------------------------------------------------------------------------ */

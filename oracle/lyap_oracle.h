@@ -19,8 +19,8 @@
  * plus the method the sweep adds on top of the reference (coupled variational
  * RK with the Jacobian at every stage state, modified Gram-Schmidt on exact
  * time boundaries, adaptive DVERK/Dopri5, FP32 tangent stages with FP64
- * renormalisation, norm products over windows of 16 renormalisations and
- * Kahan-compensated sums of the windows' logs).
+ * renormalisation, norm products over windows of 64 renormalisations
+ * (ORC_LOG_WINDOW) and Kahan-compensated sums of the windows' logs).
  *
  * The descriptor below has the same memory layout as lyap_sweep_desc in
  * include/chaos_lyap.h so one Python dictionary can fill both.
@@ -44,6 +44,11 @@ int orc_is_map(int model);
    to [-1.6, 2.5], the range that survives the step-factor limits) */
 double orc_ctrl_log(double x);
 double orc_ctrl_exp(double x);
+/* one step-controller decision on error norm err (dopriStep's PI rule,
+   cuda/tools/dopri.dp.cpp:109-139, in the one-exponential form); the state
+   (log facOld, rejected flag) is updated in place; returns 1 on accept */
+int orc_ctrl_update(int integ, double err, double* log_fac_old, int* rejected, double h_try,
+                    double h_max, double* h_next);
 enum { ORC_METHOD_VAR = 0, ORC_METHOD_VAR_REF = 1, ORC_METHOD_QR_REF = 2, ORC_METHOD_FD_REF = 3 };
 enum { ORC_INT_DVERK = 0, ORC_INT_DOPRI5 = 1 };
 enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };

@@ -77,6 +77,11 @@ def lib():
         L.orc_pow.restype = d; L.orc_pow.argtypes = [d, d]
         L.orc_ctrl_log.restype = d; L.orc_ctrl_log.argtypes = [d]
         L.orc_ctrl_exp.restype = d; L.orc_ctrl_exp.argtypes = [d]
+        L.orc_ctrl_update.restype = i
+        L.orc_ctrl_update.argtypes = [i, d, ctypes.POINTER(d), ctypes.POINTER(ctypes.c_int), d, d,
+                                      ctypes.POINTER(d)]
+        L.orc_jac_norm.restype = d
+        L.orc_jac_norm.argtypes = [i, i, P, P]
         L.orc_uniform.restype = d; L.orc_uniform.argtypes = [u64, i64, i64]
         L.orc_loop_count.restype = i64; L.orc_loop_count.argtypes = [d, d, d]
         L.orc_rhs.argtypes = [i, i, P, P, P]
@@ -144,6 +149,22 @@ def jvp(model, x, p, v):
     out = np.empty_like(x)
     lib().orc_jvp(model, len(x), _ptr(x), _ptr(p), _ptr(v), _ptr(out))
     return out
+
+
+def ctrl_update(integ, err, log_fac_old, rejected, h_try, h_max):
+    """One step-controller decision: (accepted, h_next, log_fac_old, rejected)."""
+    lf = ctypes.c_double(float(log_fac_old))
+    rj = ctypes.c_int(int(rejected))
+    hn = ctypes.c_double()
+    acc = lib().orc_ctrl_update(int(integ), float(err), ctypes.byref(lf), ctypes.byref(rj),
+                                float(h_try), float(h_max), ctypes.byref(hn))
+    return bool(acc), hn.value, lf.value, bool(rj.value)
+
+
+def jac_norm(model, x, p):
+    """max_i sum_j |J_ij(x)|: the adaptive step cap's norm."""
+    x, p = f64(x), f64(np.pad(p, (0, MAX_PARAMS - len(p))))
+    return lib().orc_jac_norm(model, len(x), _ptr(x), _ptr(p))
 
 
 def rk_step(model, integ, x, p, h, want_err=True):

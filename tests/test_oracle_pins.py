@@ -395,6 +395,125 @@ def test_step_cap_bounds_steps_by_jacobian_norm():
     assert st[0] == O.OK and na[0] >= 60 * jn * 0.9
 
 
+# ------------------------------------------------------- step controller
+def reference_controller(integ, err, fac_old, is_reject, step, max_step, beta=0.04, safe=0.9,
+                         fac1=0.2, fac2=10.0, q=None):
+    """dopriStep's step-size rule as the reference writes it, with pow
+    (cuda/tools/dopri.dp.cpp:9-12,27-29,109-139): expo1 = 1/q - 0.75*beta,
+    fac = max(1/fac2, min(1/fac1, err^expo1 / facOld^beta / safe)), newStep =
+    step / fac; accepted (err <= 1): facOld = max(err, 1e-4), |newStep| capped
+    at maxStep, and no growth right after a rejection; rejected: newStep =
+    step / min(1/fac1, err^expo1 / safe).  q = 5 for Dopri5 as in the
+    reference; 6 for DVERK's 6(5) pair.  Returns (accepted, new step, facOld,
+    rejected flag)."""
+    if q is None:
+        q = 6.0 if integ == O.INT_DVERK else 5.0
+    expo1 = 1.0 / q - beta * 0.75
+    facc1, facc2 = 1.0 / fac1, 1.0 / fac2
+    fac11 = err ** expo1
+    fac = fac11 / fac_old ** beta
+    fac = max(facc2, min(facc1, fac / safe))
+    new = step / fac
+    if err <= 1.0:
+        fac_old = max(err, 1e-4)
+        if abs(new) > max_step:
+            new = max_step
+        if is_reject:
+            new = min(abs(new), abs(step))
+        return True, new, fac_old, False
+    return False, step / min(facc1, fac11 / safe), fac_old, True
+
+
+def controller_cases():
+    """Single decisions (random and boundary states) and sequences that carry
+    the state (facOld, rejected flag) through accepts and rejects."""
+    rng = np.random.default_rng(11)
+    single = []
+    for _ in range(3000):
+        single.append((10.0 ** rng.uniform(-14, 4), 10.0 ** rng.uniform(-4, 0), bool(rng.integers(2)),
+                       10.0 ** rng.uniform(-5, 0), 0.5))
+    # err = 1 (accept edge), just above 1, the 1e-4 floor and either side of it,
+    # growth limit (tiny err), shrink limit on rejection (huge err), the h_max
+    # cap, reject after reject
+    for e in (1.0, np.nextafter(1.0, 2.0), 1e-4, 0.99e-4, 1.01e-4, 1e-30, 1e-12, 1e4, 1e8):
+        for fo in (1e-4, 0.3, 1.0):
+            for rej in (False, True):
+                single.append((e, fo, rej, 0.01, 0.5))
+                single.append((e, fo, rej, 0.45, 0.5))
+    seqs = [10.0 ** rng.uniform(-10, 3, 200) for _ in range(20)]
+    seqs.append(np.array([5.0, 7.0, 2.0, 0.5, 3.0, 3.0, 1e-9, 1e-9, 0.9, 1e5]))
+    return single, seqs
+
+
+def controller_mismatch(integ, rule):
+    """Largest relative difference between the oracle's controller and `rule`
+    over controller_cases() (next step, and facOld on accepts)."""
+    single, seqs = controller_cases()
+    worst = 0.0
+    for err, fo, rej, h, hmax in single:
+        acc, hn, lfo, rj = O.ctrl_update(integ, err, O.lib().orc_ctrl_log(fo), rej, h, hmax)
+        racc, rhn, rfo, rrj = rule(integ, err, fo, rej, h, hmax)
+        if acc != racc or rj != rrj:
+            return math.inf
+        worst = max(worst, abs(hn - rhn) / abs(rhn), abs(lfo - math.log(rfo)) / max(1.0, abs(math.log(rfo))))
+    for seq in seqs:
+        lfo, rej, fo, rrej, h, rh = O.lib().orc_ctrl_log(1e-4), False, 1e-4, False, 0.01, 0.01
+        for err in seq:
+            acc, h, lfo, rej = O.ctrl_update(integ, err, lfo, rej, rh, 0.5)
+            racc, rh, fo, rrej = rule(integ, err, fo, rrej, rh, 0.5)
+            if acc != racc or rej != rrej:
+                return math.inf
+            worst = max(worst, abs(h - rh) / abs(rh))
+            rh = h  # both continue from the oracle's step (no drift)
+    return worst
+
+
+@pytest.mark.parametrize("integ", [O.INT_DVERK, O.INT_DOPRI5])
+def test_controller_matches_reference_pow_rule(integ):
+    """The oracle's one-exponential controller (the form the kernels share)
+    against the reference's pow form: same accept/reject decisions, next steps
+    within 2e-9 relative (the controller's own log/exp series are 2e-10 /
+    3e-10 accurate, pinned in test_step_control_log_exp, and enter through
+    expo * log(err) with |log err| up to ~70)."""
+    assert controller_mismatch(integ, reference_controller) <= 2e-9
+
+
+@pytest.mark.parametrize("integ", [O.INT_DVERK, O.INT_DOPRI5])
+def test_controller_comparison_catches_planted_mistakes(integ):
+    """The comparison above is sharp: the reference rule with beta's sign
+    flipped, with the other integrator's exponent, with the factor limits
+    swapped, with no floor on facOld, or without the no-growth-after-reject
+    rule differs from the oracle by far more than its 2e-9 bound."""
+    mutants = [
+        lambda i, e, f, r, h, m: reference_controller(i, e, f, r, h, m, beta=-0.04),
+        lambda i, e, f, r, h, m: reference_controller(i, e, f, r, h, m,
+                                                      q=5.0 if i == O.INT_DVERK else 6.0),
+        lambda i, e, f, r, h, m: reference_controller(i, e, f, r, h, m, fac1=0.1, fac2=5.0),
+        lambda i, e, f, r, h, m: (lambda a: (a[0], a[1], max(e, 1e-12) if a[0] else a[2], a[3]))(
+            reference_controller(i, e, f, r, h, m)),
+        lambda i, e, f, r, h, m: reference_controller(i, e, f, False, h, m)[:3] + (not (e <= 1.0),),
+    ]
+    for mut in mutants:
+        assert controller_mismatch(integ, mut) > 1e-3
+
+
+@pytest.mark.parametrize("model,n", [(O.MODEL_SHIMIZU, 3), (O.MODEL_LORENZ, 3), (O.MODEL_ROSSLER, 3),
+                                     (O.MODEL_L96, 8), (O.MODEL_L96, 16), (O.MODEL_L96, 32)])
+def test_step_cap_norm_is_the_jacobian_inf_norm(model, n):
+    """The adaptive step cap h * ||J(x)||_inf <= 1 uses orc_jac_norm: the
+    largest row sum of |J|, with J assembled column by column from the
+    Jacobian-vector product (itself pinned against central differences of the
+    right-hand side) and normed by numpy.linalg.norm(J, inf)."""
+    rng = np.random.default_rng(100 + model * 7 + n)
+    npar = {O.MODEL_SHIMIZU: 2, O.MODEL_LORENZ: 3, O.MODEL_ROSSLER: 3, O.MODEL_L96: 2}[model]
+    for _ in range(50):
+        x = rng.normal(0, 5, n)
+        p = rng.uniform(-3, 30, npar)
+        J = np.stack([O.jvp(model, x, p, np.eye(n)[j]) for j in range(n)], axis=1)
+        ref = np.linalg.norm(J, np.inf)
+        assert abs(O.jac_norm(model, x, p) - ref) <= 4e-16 * ref
+
+
 def test_status_words():
     # escaping orbit (Rossler with a large a), step underflow, step budget
     d = lorenz_desc(model=O.MODEL_ROSSLER, base_params=[0.45, 4.0, 0.2], sweep_idx1=0, sweep_idx2=1,

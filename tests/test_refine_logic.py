@@ -69,16 +69,89 @@ def test_calibration_drops_wrong_escape_answers():
 
 
 def test_calibration_spends_the_wrong_answer_budget():
-    """A few wrong answers are acceptable when the map's agreement stays at the
-    target: 4 wrong of 10000 calibration points is within 0.05 %."""
+    """A few wrong classes are acceptable when the map's agreement stays at the
+    target: 4 wrong of 10000 calibration points is within 0.05 %.  The wrong
+    ones here are points the method found escaping while the surrogate
+    answered a value (judged by class only: an escaped point has no lambda)."""
     n = 10000
     lam = np.tile([0.1, 0.0, -1.0], (n, 1))
     st = np.zeros(n, dtype=np.uint32)
-    mu1 = np.full(n, 0.1)
-    mu1[:4] = -0.1  # confidently wrong
-    p = pred_rows(mu1, np.full(n, 0.001), np.zeros(n))
+    st[:4] = 4
+    lam[:4] = np.nan
+    p = pred_rows(np.full(n, 0.1), np.full(n, 0.001), np.zeros(n))
     kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
     assert kappa == 1.0 and answered == n and abs(agree - 0.9996) < 1e-12
+
+
+def test_calibration_keeps_answered_values_within_the_accuracy_gate():
+    """Every lambda1 the rule answers on the calibration subsample must be
+    within dlam_cal (5e-4) of the method's value.  With errors of 0.3 * std the
+    rule must stop at std <= 5e-4 / 0.3 = 1.67e-3, i.e. kappa >= 0.01 / 1.67e-3
+    = 6: the smallest multiplier of the grid that keeps the gate is 6, even
+    though every answer's class is right at kappa 1."""
+    rng = np.random.default_rng(2)
+    n = 20000
+    std = rng.uniform(0.0, 0.01, n)
+    lam1 = rng.uniform(0.05, 0.2, n)
+    mu1 = lam1 + 0.3 * std * rng.choice([-1.0, 1.0], n)
+    lam = np.stack([lam1, np.zeros(n), np.full(n, -1.0)], 1)
+    st = np.zeros(n, dtype=np.uint32)
+    kappa, escapes, agree, answered, table = R.calibrate_kappa(
+        pred_rows(mu1, std, np.zeros(n)), lam, st, target=0.9995, **THR)
+    assert kappa == 6.0 and agree == 1.0
+    assert answered == int(np.sum(6.0 * std <= 0.01))
+    row = [r for r in table if r[0] == 6.0 and r[1]][0]
+    assert row[4] <= 5e-4
+    # at kappa 5 (std <= 2e-3) the largest error exceeds 5e-4: rejected
+    assert [r for r in table if r[0] == 5.0 and r[1]][0][4] > 5e-4
+    # a looser calibration bound admits smaller multipliers
+    k2 = R.calibrate_kappa(pred_rows(mu1, std, np.zeros(n)), lam, st, target=0.9995,
+                           dlam_cal=1e-3, **THR)[0]
+    assert k2 == 3.0
+
+
+def test_calibration_falls_back_to_escape_answers_only():
+    """When no lambda answer meets the accuracy bound, the rule that answers
+    only the confident escapes (kappa inf) is taken."""
+    n = 10000
+    lam = np.tile([0.1, 0.0, -1.0], (n, 1))
+    st = np.zeros(n, dtype=np.uint32)
+    st[:500] = 4
+    lam[:500] = np.nan
+    pbad = np.zeros(n)
+    pbad[:500] = 0.995
+    mu1 = np.full(n, 0.1 + 2e-3)  # every value 2e-3 off: over the gate
+    p = pred_rows(mu1, np.full(n, 1e-5), pbad)
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    assert kappa == float("inf") and escapes and answered == 500 and agree == 1.0
+
+
+def test_verify_gates_use_the_surrogate_where_it_answers():
+    """The chosen form's answer at a verification point is the surrogate's
+    where the rule answers it, the method's elsewhere; |dlambda1| is taken
+    where that answer is a value and the FP64 judge's point is OK."""
+    flagged = np.array([True, False, False, False, True])
+    pred = pred_rows(np.array([0.5, 0.1008, 0.3, -0.2, 0.0]), np.zeros(5),
+                     np.array([0.0, 0.0, 0.99, 0.0, 0.0]))
+    lam_m = np.array([[0.2, 0, -1], [0.1, 0, -1], [0.3, 0, -1], [-0.2, 0, -1], [0.0, 0, -1]], float)
+    lam_m[0, 0] = 0.2 + 1e-6
+    st_m = np.zeros(5, dtype=np.uint32)
+    lam_j = lam_m.copy()
+    lam_j[0, 0] = 0.2
+    st_j = np.array([0, 0, 4, 0, 0], dtype=np.uint32)
+    lam_j[2] = np.nan
+    g = R.verify_gates(flagged, pred, lam_m, st_m, lam_j, st_j)
+    # point 1: surrogate 0.1008 vs judge 0.1 -> 8e-4; point 0: method 1e-6 off;
+    # point 2: surrogate says escaped, judge escaped -> class only
+    assert abs(g["max_dlambda1_chosen_vs_fp64"] - 8e-4) < 1e-8  # pred is binary32
+    assert abs(g["max_dlambda1_surrogate_answers"] - 8e-4) < 1e-8
+    assert abs(g["max_dlambda1_computed_vs_fp64"] - 1e-6) < 1e-12
+    assert g["surrogate_answered_verify"] == 3 and g["surrogate_valued_answers_verify"] == 2
+    assert g["class_agreement"] == 1.0 and g["gate_met"]
+    # a surrogate value 2e-3 off breaks the gate
+    pred[3, 0] = -0.198
+    g = R.verify_gates(flagged, pred, lam_m, st_m, lam_j, st_j)
+    assert g["chosen_points_over_gate"] == 1 and not g["gate_met"]
 
 
 def test_classify():
