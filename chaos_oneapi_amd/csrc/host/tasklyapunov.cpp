@@ -32,7 +32,18 @@ int TaskLyapunov::init(std::shared_ptr<Parameters> paramsBase, const std::string
         return -1;
     }
     ModelFactoryRegistry::Storage& storage = Singleton<ModelFactoryRegistry::Storage>::Instance();
-    mModelPtr = &storage._data.at(modelName);
+    auto it = storage._data.find(modelName);
+    if (it == storage._data.end()) {
+        printf("error: model %s is not registered\n", modelName.c_str());
+        return -1;
+    }
+    if (it->second._model_id < 0) {
+        printf("error: model %s is registered but has no compiled kernel in liblyap.so: its right-hand "
+               "side and Jacobian must be added as a device model (the reference's device function "
+               "pointers are not used)\n", modelName.c_str());
+        return -1;
+    }
+    mModelPtr = &it->second;
     const auto& P = *mParameters;
     mKNodsNum = (size_t)(P.first_sweep_param[3] * P.second_sweep_param[3]);
     const int gpus = P.num_GPUs > 0 ? P.num_GPUs : 1;

@@ -401,10 +401,23 @@ int lyap_cuda_lyapunov(lyap_ctx* c, int block_num, int block_size, size_t task_n
                        double add_skip_steps, void* stream) {
     (void)block_num;
     (void)block_size;
-    if (model_type != LYAP_TYPE_FLOW)
-        return fail(LYAP_ERR_UNSUPPORTED, "map-type models have no kernel in this build");
+    if (!c) return fail(LYAP_ERR_INVALID, "lyap_cuda_lyapunov: NULL context");
+    if (model_type != LYAP_TYPE_FLOW && model_type != LYAP_TYPE_MAP)
+        return fail(LYAP_ERR_INVALID, "lyap_cuda_lyapunov: unknown model type %d", model_type);
+    if (task_num > (size_t)INT32_MAX)
+        return fail(LYAP_ERR_INVALID, "lyap_cuda_lyapunov: task_num %zu above %d", task_num, INT32_MAX);
     if (!d_inits || !d_params)
         return fail(LYAP_ERR_INVALID, "lyap_cuda_lyapunov: inits and params are required");
+    // the model type given must be the model's (cuda/lyapunov.dp.cpp:274-317
+    // branches on it); a map advances one iteration per unit of time, so the
+    // descriptor's finalisation sets h = 1 and rejects the QR mode for maps as
+    // the reference does (cuda/lyapunov.dp.cpp:302-305)
+    bool type_ok = false;
+    for (int i = 0; i < kNumModels; ++i)
+        if (kModels[i].model == model && kModels[i].dim == dimension) type_ok = kModels[i].type == model_type;
+    if (!type_ok)
+        return fail(LYAP_ERR_INVALID, "lyap_cuda_lyapunov: model %d (dimension %d) is not of type %d", model,
+                    dimension, model_type);
     lyap_sweep_desc d;
     std::memset(&d, 0, sizeof(d));
     d.model = model;
@@ -418,7 +431,7 @@ int lyap_cuda_lyapunov(lyap_ctx* c, int block_num, int block_size, size_t task_n
     d.t_transient = time_skip;
     d.t_slave = time_skip_clp;
     d.t_average = calc_time;
-    d.t_renorm = add_skip_steps * integration_step;
+    d.t_renorm = add_skip_steps * (model_type == LYAP_TYPE_MAP ? 1.0 : integration_step);
     d.n1 = 1;
     d.n2 = (int32_t)task_num;
     d.sweep_idx1 = 0;

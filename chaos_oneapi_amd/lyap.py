@@ -249,9 +249,22 @@ class Context:
             nr = torch.empty(count, dtype=torch.int32, device=dev)
         else:
             lam, st, na, nr = out
-        for t in (idx_list, params, inits):
-            if t is not None:
-                assert t.is_cuda and t.is_contiguous()
+        for t, name, dt in ((idx_list, "idx_list", torch.int64), (params, "params", torch.float64),
+                            (inits, "inits", torch.float64)):
+            if t is not None and not (t.is_cuda and t.is_contiguous() and t.dtype == dt):
+                raise ValueError(f"{name} must be a contiguous {dt} CUDA tensor")
+        if idx_list is not None and count > idx_list.numel():
+            raise ValueError(f"count {count} exceeds the {idx_list.numel()} entries of idx_list")
+        n_rows = first + count
+        if params is not None and params.numel() < n_rows * desc.n_params and idx_list is None:
+            raise ValueError(f"params holds {params.numel()} values, {n_rows * desc.n_params} needed")
+        if inits is not None and inits.numel() < n_rows * desc.dim and idx_list is None:
+            raise ValueError(f"inits holds {inits.numel()} values, {n_rows * desc.dim} needed")
+        if out is not None:
+            want = ((count, desc.n_exps), (count,), (count,), (count,))
+            for t, w in zip(out, want):
+                if t is not None and (tuple(t.shape[:len(w)]) != w or not t.is_contiguous()):
+                    raise ValueError(f"output tensor of shape {tuple(t.shape)}, {w} expected")
         s = stream if stream is not None else torch.cuda.current_stream(dev)
         check(load().lyap_sweep_run(self._h, ctypes.byref(desc), int(first), int(count),
                                     _ptr(idx_list), _ptr(params), _ptr(inits), _ptr(lam),
@@ -268,13 +281,20 @@ class Context:
 
     def cuda_lyapunov(self, task_num, is_qr, is_var, inits, dimension, model, params, params_dim,
                       eps, h, lyap_exps, exps_num, time_skip, time_skip_clp, calc_time,
-                      add_skip_steps, stream=None):
-        """The reference-shaped launch (cudaLyapunov's arguments)."""
+                      add_skip_steps, model_type=TYPE_FLOW, stream=None):
+        """The reference-shaped launch (cudaLyapunov's arguments); inits,
+        params and lyap_exps are float64 CUDA tensors, AoS per point."""
         import torch
 
+        for t, name, need in ((inits, "inits", task_num * dimension), (params, "params", task_num * params_dim),
+                              (lyap_exps, "lyap_exps", task_num * exps_num)):
+            if not (t.is_cuda and t.is_contiguous() and t.dtype == torch.float64):
+                raise ValueError(f"{name} must be a contiguous float64 CUDA tensor")
+            if t.numel() < need:
+                raise ValueError(f"{name} holds {t.numel()} values, {need} needed")
         s = stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", self.device))
         check(load().lyap_cuda_lyapunov(
-            self._h, 32, 32, int(task_num), int(is_qr), int(is_var), TYPE_FLOW, _ptr(inits),
+            self._h, 32, 32, int(task_num), int(is_qr), int(is_var), int(model_type), _ptr(inits),
             int(dimension), int(model), _ptr(params), int(params_dim), float(eps), float(h),
             _ptr(lyap_exps), int(exps_num), float(time_skip), float(time_skip_clp),
             float(calc_time), float(add_skip_steps), ctypes.c_void_p(s.cuda_stream)))

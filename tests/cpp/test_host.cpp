@@ -156,13 +156,50 @@ static void test_task_init_and_output() {
     CHECK(v == 0.123456789);
 }
 
+// A registrar in the reference's shape (cuda/models/flows/shimizu.dp.cpp:70-211):
+// five device-function-pointer fetchers and the model strings.  "shimizu" binds
+// to the compiled model of that name; "lorenz4d" (named in the reference's
+// config.json, with no code there or here) registers without a kernel.
+static int g_fetches = 0;
+static struct ReferenceStyleRegistry : public ModelFactoryRegistry {
+    ReferenceStyleRegistry() {
+        auto func = [](diffSysFunc* p) { ++g_fetches; *p = nullptr; };
+        auto func_var = [](diffSysFuncVar* p) { ++g_fetches; *p = nullptr; };
+        ModelType type = ModelType::flow;
+        RegisterModel("shimizu", 3, 2, type, func, func_var, func_var, func_var, func_var, "alpha, lamda",
+                      "x, y, z", "x' = y;y' = - alpha * y - x * z + x;z' = - lamda * z + x ** 2.0");
+        RegisterModel("lorenz4d", 4, 8, type, func, func_var, func_var, func_var, func_var, "a, b", "x, y, z, w",
+                      "(not compiled)");
+    }
+} g_reference_style;
+
+static void test_reference_style_registration() {
+    auto& st = Singleton<ModelFactoryRegistry::Storage>::Instance();
+    const auto& sh = st._data.at("shimizu");
+    CHECK(sh._model_id == LYAP_MODEL_SHIMIZU && sh._dimension == 3 && sh._param_number == 2);
+    CHECK(sh._model_param == "alpha, lamda" && sh._type == ModelType::flow);
+    CHECK(sh._model_func && sh._func_Q);  // kept, never called
+    CHECK(st._data.at("lorenz4d")._model_id == -1);
+    CHECK(g_fetches == 0);
+    // a config of the reference's shimizu entry runs through the bound model;
+    // the uncompiled one is refused by init
+    json cfg = json::parse(kConfig);
+    auto params = ConfigFactory::paramsFromJson(ToolID::LYAPUNOV, "shimizu", cfg);
+    TaskLyapunov ok;
+    CHECK(ok.init(params, "shimizu") == 0 && ok.descriptor().model == LYAP_MODEL_SHIMIZU);
+    TaskLyapunov bad;
+    CHECK(bad.init(params, "lorenz4d") == -1);
+    CHECK(bad.init(params, "no_such_model") == -1);
+}
+
 int main() {
     test_json();
     test_params();
     test_task_init_and_output();
+    test_reference_style_registration();
     // models registered at start-up, sorted as the reference's std::map
     auto& st = Singleton<ModelFactoryRegistry::Storage>::Instance();
-    CHECK(st._data.size() == 8 && st._data.begin()->first == "henon");
+    CHECK(st._data.size() == 9 && st._data.begin()->first == "henon");
     CHECK(st._data.at("shimizu")._param_number == 2);
     std::printf(g_fail ? "FAILED %d\n" : "OK\n", g_fail);
     return g_fail ? 1 : 0;
