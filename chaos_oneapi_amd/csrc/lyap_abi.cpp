@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -493,6 +494,107 @@ int longest_first_rank(int kind) {
     }
 }
 
+// work counters of a batch: one per class, 8 words apart (separate cache lines)
+constexpr int kCounterStride = 8;
+
+// A batch's class records, validated and in queue order, built on the host
+// without allocating (fixed arrays), shared by lyap_batch_create and
+// lyap_batch_run_noalloc.  io.counter is left for the caller to point.
+struct BatchPlan {
+    lyap::BatchClass recs[lyap::kMaxBatchClasses];
+    int n = 0;  // non-empty classes
+    int integ = -1, adapt = -1, prec = -1;
+    int64_t wave_tasks = 0;
+    lyap_batch_tuning tuning{};
+};
+
+int plan_batch(const lyap_batch_class* classes, int n_classes, const lyap_batch_tuning* tuning,
+               BatchPlan* plan) {
+    if (n_classes < 1 || n_classes > lyap::kMaxBatchClasses)
+        return fail(LYAP_ERR_INVALID, "a batch holds 1 to %d classes, got %d", lyap::kMaxBatchClasses,
+                    n_classes);
+    lyap_batch_tuning tu{};
+    if (tuning) tu = *tuning;
+    if (tu.layout < 0 || tu.layout >= lyap::kBatchLayouts)
+        return fail(LYAP_ERR_INVALID, "batch layout %d is not built", tu.layout);
+    if (tu.waves_per_cu < 0 || tu.order < 0 || tu.order > 2 || tu.gather < 0 || tu.gather > 64)
+        return fail(LYAP_ERR_INVALID, "batch tuning: waves_per_cu >= 0, order in 0..2, gather in 0..64");
+    plan->tuning = tu;
+    lyap::BatchClass recs[lyap::kMaxBatchClasses];
+    int rank[lyap::kMaxBatchClasses];
+    int n = 0;
+    for (int j = 0; j < n_classes; ++j) {
+        const lyap_batch_class& k = classes[j];
+        lyap_sweep_desc fd = k.desc;
+        int rc = lyap_desc_finalize(&fd);
+        if (rc != LYAP_SUCCESS) return fail(rc, "batch class %d: %s", j, g_last_error.c_str());
+        if (fd.method != LYAP_METHOD_VAR)
+            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: a batch runs the variational method only", j);
+        const int kind = batch_kind(fd);
+        if (kind < 0)
+            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: no batch kernel for model %d dimension %d",
+                        j, fd.model, fd.dim);
+        if (j == 0) {
+            plan->integ = fd.integrator;
+            plan->adapt = fd.adaptive ? 1 : 0;
+            plan->prec = fd.precision;
+        } else if (fd.integrator != plan->integ || (fd.adaptive ? 1 : 0) != plan->adapt ||
+                   fd.precision != plan->prec) {
+            return fail(LYAP_ERR_INVALID,
+                        "batch class %d: integrator, adaptive flag and precision must match class 0", j);
+        }
+        if (k.count < 0) return fail(LYAP_ERR_INVALID, "batch class %d: negative count", j);
+        if (k.count == 0) continue;
+        if (!k.d_lambda) return fail(LYAP_ERR_INVALID, "batch class %d: d_lambda is NULL", j);
+        if (!k.d_params && k.count > (int64_t)fd.n1 * (int64_t)fd.n2)
+            return fail(LYAP_ERR_INVALID, "batch class %d: %lld points beyond the %d x %d grid", j,
+                        (long long)k.count, fd.n1, fd.n2);
+        lyap::BatchClass& r = recs[n];
+        std::memset(&r, 0, sizeof(r));
+        r.d = fd;
+        r.io = lyap::SweepIO{0,          k.count,    nullptr,   k.d_params, k.d_inits,
+                             k.d_lambda, k.d_status, k.d_n_acc, k.d_n_rej,  nullptr,
+                             spread_stride(k.count), tu.gather, 0};
+        r.kind = kind;
+        rank[n] = tu.order == 0 ? longest_first_rank(kind)
+                                : (tu.order == 2 ? (kind >= lyap::BATCH_L96_8 ? 0 : 1) : 0);
+        const int ppw = lyap::batch_points_per_wave(tu.layout, fd.dim);
+        plan->wave_tasks += (k.count + ppw - 1) / ppw;
+        ++n;
+    }
+    if (!lyap::batch_variant_built(plan->integ, plan->adapt != 0, plan->prec == LYAP_PREC_MIXED, tu.layout,
+                                   tu.occupancy3 != 0))
+        return fail(LYAP_ERR_UNSUPPORTED, "batch kernel layout %d%s is built for adaptive Dopri5 only",
+                    tu.layout, tu.occupancy3 ? " (3 waves per SIMD)" : "");
+    // stable order by rank (insertion sort: at most 16 records)
+    int ord[lyap::kMaxBatchClasses];
+    for (int j = 0; j < n; ++j) {
+        int q = j;
+        while (q > 0 && rank[ord[q - 1]] > rank[j]) {
+            ord[q] = ord[q - 1];
+            --q;
+        }
+        ord[q] = j;
+    }
+    for (int j = 0; j < n; ++j) plan->recs[j] = recs[ord[j]];
+    plan->n = n;
+    return LYAP_SUCCESS;
+}
+
+// persistent grid of a planned batch on a device with num_cus CUs
+hipError_t plan_waves(const BatchPlan& plan, int num_cus, int64_t* waves) {
+    int per_cu = 0;
+    hipError_t e = lyap::batch_max_waves_per_cu(plan.integ, plan.adapt != 0, plan.prec == LYAP_PREC_MIXED,
+                                                plan.tuning.layout, plan.tuning.occupancy3 != 0, &per_cu);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+    if (plan.tuning.waves_per_cu > 0 && plan.tuning.waves_per_cu < per_cu) per_cu = plan.tuning.waves_per_cu;
+    int64_t w = (int64_t)per_cu * num_cus;
+    if (w > plan.wave_tasks) w = plan.wave_tasks;
+    *waves = w < 1 ? 1 : w;
+    return hipSuccess;
+}
+
 void batch_free(lyap_batch* b) {
     if (!b) return;
     hipSetDevice(b->device);
@@ -514,100 +616,34 @@ int lyap_batch_create(lyap_ctx* c, const lyap_batch_class* classes, int n_classe
                       const lyap_batch_tuning* tuning, lyap_batch** out) {
     if (!c || !classes || !out) return fail(LYAP_ERR_INVALID, "lyap_batch_create: NULL argument");
     *out = nullptr;
-    if (n_classes < 1 || n_classes > lyap::kMaxBatchClasses)
-        return fail(LYAP_ERR_INVALID, "a batch holds 1 to %d classes, got %d", lyap::kMaxBatchClasses,
-                    n_classes);
-    lyap_batch_tuning tu{};
-    if (tuning) tu = *tuning;
-    if (tu.layout < 0 || tu.layout >= lyap::kBatchLayouts)
-        return fail(LYAP_ERR_INVALID, "batch layout %d is not built", tu.layout);
-    if (tu.waves_per_cu < 0 || tu.order < 0 || tu.order > 2 || tu.gather < 0 || tu.gather > 64)
-        return fail(LYAP_ERR_INVALID, "batch tuning: waves_per_cu >= 0, order in 0..2, gather in 0..64");
-
-    std::vector<lyap::BatchClass> recs;
-    std::vector<int> rank;
-    int integ = -1, adapt = -1, prec = -1;
-    int64_t wave_tasks = 0;
-    for (int j = 0; j < n_classes; ++j) {
-        const lyap_batch_class& k = classes[j];
-        lyap_sweep_desc fd = k.desc;
-        int rc = lyap_desc_finalize(&fd);
-        if (rc != LYAP_SUCCESS) return fail(rc, "batch class %d: %s", j, g_last_error.c_str());
-        if (fd.method != LYAP_METHOD_VAR)
-            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: a batch runs the variational method only", j);
-        const int kind = batch_kind(fd);
-        if (kind < 0)
-            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: no batch kernel for model %d dimension %d",
-                        j, fd.model, fd.dim);
-        if (j == 0) {
-            integ = fd.integrator;
-            adapt = fd.adaptive ? 1 : 0;
-            prec = fd.precision;
-        } else if (fd.integrator != integ || (fd.adaptive ? 1 : 0) != adapt || fd.precision != prec) {
-            return fail(LYAP_ERR_INVALID,
-                        "batch class %d: integrator, adaptive flag and precision must match class 0", j);
-        }
-        if (k.count < 0) return fail(LYAP_ERR_INVALID, "batch class %d: negative count", j);
-        if (k.count == 0) continue;
-        if (!k.d_lambda) return fail(LYAP_ERR_INVALID, "batch class %d: d_lambda is NULL", j);
-        if (!k.d_params && k.count > (int64_t)fd.n1 * (int64_t)fd.n2)
-            return fail(LYAP_ERR_INVALID, "batch class %d: %lld points beyond the %d x %d grid", j,
-                        (long long)k.count, fd.n1, fd.n2);
-        lyap::BatchClass r;
-        std::memset(&r, 0, sizeof(r));
-        r.d = fd;
-        r.io = lyap::SweepIO{0,         k.count,   nullptr,    k.d_params, k.d_inits,
-                             k.d_lambda, k.d_status, k.d_n_acc, k.d_n_rej, nullptr,
-                             spread_stride(k.count), tu.gather, 0};
-        r.kind = kind;
-        recs.push_back(r);
-        rank.push_back(tu.order == 0 ? longest_first_rank(kind)
-                                     : (tu.order == 2 ? (kind >= lyap::BATCH_L96_8 ? 0 : 1) : 0));
-        const int ppw = lyap::batch_points_per_wave(tu.layout, fd.dim);
-        wave_tasks += (k.count + ppw - 1) / ppw;
-    }
-    if (!lyap::batch_variant_built(integ, adapt != 0, prec == LYAP_PREC_MIXED, tu.layout,
-                                   tu.occupancy3 != 0))
-        return fail(LYAP_ERR_UNSUPPORTED,
-                    "batch kernel layout %d%s is built for adaptive Dopri5 only", tu.layout,
-                    tu.occupancy3 ? " (3 waves per SIMD)" : "");
-    // stable order by rank
-    std::vector<int> ord(recs.size());
-    for (size_t j = 0; j < ord.size(); ++j) ord[j] = (int)j;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return rank[a] < rank[b]; });
+    BatchPlan plan;
+    int rc = plan_batch(classes, n_classes, tuning, &plan);
+    if (rc != LYAP_SUCCESS) return rc;
+    const lyap_batch_tuning& tu = plan.tuning;
 
     hipError_t e = hipSetDevice(c->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     lyap_batch* b = new lyap_batch();
     b->device = c->device;
     b->tuning = tu;
-    b->n = (int)recs.size();
+    b->n = plan.n;
     if (b->n == 0) {
         *out = b;
         return LYAP_SUCCESS;
     }
-    int per_cu = 0;
-    if ((e = lyap::batch_max_waves_per_cu(integ, adapt != 0, prec == LYAP_PREC_MIXED, tu.layout,
-                                          tu.occupancy3 != 0, &per_cu)) != hipSuccess) {
+    if ((e = plan_waves(plan, c->num_cus, &b->waves)) != hipSuccess) {
         batch_free(b);
         return hip_fail(e, "batch occupancy");
     }
-    if (per_cu < 1) per_cu = 1;
-    if (tu.waves_per_cu > 0 && tu.waves_per_cu < per_cu) per_cu = tu.waves_per_cu;
-    b->waves = (int64_t)per_cu * c->num_cus;
-    if (b->waves > wave_tasks) b->waves = wave_tasks;
-    if (b->waves < 1) b->waves = 1;
-
-    std::vector<lyap::BatchClass> sorted;
-    for (int j : ord) sorted.push_back(recs[j]);
-    if ((e = hipMalloc(&b->d_counters, 64 * sizeof(unsigned long long))) != hipSuccess ||
-        (e = hipMalloc(&b->d_cls, sorted.size() * sizeof(lyap::BatchClass))) != hipSuccess) {
+    if ((e = hipMalloc(&b->d_counters, lyap::kMaxBatchClasses * kCounterStride * sizeof(unsigned long long))) !=
+            hipSuccess ||
+        (e = hipMalloc(&b->d_cls, plan.n * sizeof(lyap::BatchClass))) != hipSuccess) {
         batch_free(b);
         return hip_fail(e, "hipMalloc(batch)");
     }
-    for (size_t j = 0; j < sorted.size(); ++j) sorted[j].io.counter = b->d_counters + 8 * j;
-    if ((e = hipMemcpy(b->d_cls, sorted.data(), sorted.size() * sizeof(lyap::BatchClass),
-                       hipMemcpyHostToDevice)) != hipSuccess) {
+    for (int j = 0; j < plan.n; ++j) plan.recs[j].io.counter = b->d_counters + kCounterStride * j;
+    if ((e = hipMemcpy(b->d_cls, plan.recs, plan.n * sizeof(lyap::BatchClass), hipMemcpyHostToDevice)) !=
+        hipSuccess) {
         batch_free(b);
         return hip_fail(e, "hipMemcpy(batch classes)");
     }
@@ -621,7 +657,7 @@ int lyap_batch_create(lyap_ctx* c, const lyap_batch_class* classes, int n_classe
         batch_free(b);
         return hip_fail(e, "hipStreamBeginCapture");
     }
-    hipError_t el = lyap::launch_batch(b->d_cls, b->n, integ, adapt != 0, prec == LYAP_PREC_MIXED,
+    hipError_t el = lyap::launch_batch(b->d_cls, b->n, plan.integ, plan.adapt != 0, plan.prec == LYAP_PREC_MIXED,
                                        tu.layout, tu.occupancy3 != 0, b->waves, b->capture);
     e = hipStreamEndCapture(b->capture, &b->graph);
     if (el != hipSuccess || e != hipSuccess) {
@@ -680,3 +716,92 @@ int lyap_batch_layout_ppw(int layout, int dim, int* ppw) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched launch that allocates nothing
+// ---------------------------------------------------------------------------
+// The class records and work counters of lyap_batch_run_noalloc live in a
+// file-scope device array: records contiguous from offset 0, counters from
+// kNoallocCounterOffset.  Each device holds its own copy; host code reaches it
+// only through the address hipGetSymbolAddress returns on that device.
+namespace {
+
+constexpr size_t kNoallocCounterOffset =
+    ((lyap::kMaxBatchClasses * sizeof(lyap::BatchClass) + 255) / 256) * 256;
+constexpr size_t kNoallocScratchBytes =
+    kNoallocCounterOffset + lyap::kMaxBatchClasses * kCounterStride * sizeof(unsigned long long);
+constexpr int kNoallocMaxDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+// one class record, passed by value (kernel arguments: no host-to-device copy
+// and no staging buffer), written into the scratch array
+__global__ void k_stage_batch_class(lyap::BatchClass rec, lyap::BatchClass* dst) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *dst = rec;
+}
+
+namespace {
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kNoallocMaxDevices] = {};
+int g_scratch_cus[kNoallocMaxDevices] = {};
+
+// the device's scratch address and CU count, looked up once per device with
+// that device current (the caller has made it current)
+int noalloc_scratch(int device, unsigned char** ptr, int* cus) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        int n = 0;
+        if ((e = hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+            return hip_fail(e, "hipDeviceGetAttribute");
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+        g_scratch_cus[device] = n;
+    }
+    *ptr = g_scratch_ptr[device];
+    *cus = g_scratch_cus[device];
+    return LYAP_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" int lyap_batch_run_noalloc(int device, const lyap_batch_class* classes, int n_classes,
+                                      const lyap_batch_tuning* tuning, void* stream) {
+    if (!classes) return fail(LYAP_ERR_INVALID, "lyap_batch_run_noalloc: NULL classes");
+    if (n_classes > lyap::kMaxBatchClasses)
+        return fail(LYAP_ERR_INVALID,
+                    "lyap_batch_run_noalloc: %d classes exceed the scratch array's %d records (%zu bytes)",
+                    n_classes, lyap::kMaxBatchClasses, kNoallocScratchBytes);
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device < 0 || device >= count || device >= kNoallocMaxDevices)
+        return fail(LYAP_ERR_INVALID, "lyap_batch_run_noalloc: device %d of %d", device, count);
+    BatchPlan plan;
+    int rc = plan_batch(classes, n_classes, tuning, &plan);
+    if (rc != LYAP_SUCCESS) return rc;
+    if (plan.n == 0) return LYAP_SUCCESS;
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, "hipSetDevice");
+    unsigned char* base = nullptr;
+    int cus = 0;
+    if ((rc = noalloc_scratch(device, &base, &cus)) != LYAP_SUCCESS) return rc;
+    int64_t waves = 0;
+    if ((e = plan_waves(plan, cus, &waves)) != hipSuccess) return hip_fail(e, "batch occupancy");
+    lyap::BatchClass* d_cls = reinterpret_cast<lyap::BatchClass*>(base);
+    unsigned long long* d_counters = reinterpret_cast<unsigned long long*>(base + kNoallocCounterOffset);
+    hipStream_t s = (hipStream_t)stream;
+    for (int j = 0; j < plan.n; ++j) {
+        plan.recs[j].io.counter = d_counters + kCounterStride * j;
+        hipLaunchKernelGGL(k_stage_batch_class, dim3(1), dim3(64), 0, s, plan.recs[j], d_cls + j);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "batch class staging");
+    // the counters are zeroed by the launch's own reset kernel
+    e = lyap::launch_batch(d_cls, plan.n, plan.integ, plan.adapt != 0, plan.prec == LYAP_PREC_MIXED,
+                           plan.tuning.layout, plan.tuning.occupancy3 != 0, waves, s);
+    if (e != hipSuccess) return hip_fail(e, "batch launch");
+    return LYAP_SUCCESS;
+}

@@ -35,7 +35,7 @@ EXPORTS = (
     "lyap_mlp_forward", "lyap_mlp_param_count", "lyap_mlp_get", "lyap_mlp_set_params",
     "lyap_batch_create", "lyap_batch_launch", "lyap_batch_last_ms", "lyap_batch_info",
     "lyap_batch_layout_ppw",
-    "lyap_batch_destroy",
+    "lyap_batch_destroy", "lyap_batch_run_noalloc",
 )
 
 
@@ -157,6 +157,7 @@ def load(path: str = LIB_PATH):
     L.lyap_batch_info.argtypes = [vp, P(i64), P(ctypes.c_int)]
     L.lyap_batch_layout_ppw.argtypes = [ctypes.c_int, ctypes.c_int, P(ctypes.c_int)]
     L.lyap_batch_destroy.argtypes = [vp]
+    L.lyap_batch_run_noalloc.argtypes = [ctypes.c_int, P(BatchClassRec), ctypes.c_int, P(BatchTuning), vp]
     L.lyap_cuda_lyapunov.argtypes = [
         vp, ctypes.c_int, ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         vp, i32, i32, vp, i32, d, d, vp, i32, d, d, d, d, vp]
@@ -373,6 +374,40 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_run_noalloc(device: int, classes, layout: int = 0, occupancy3: bool = False,
+                      waves_per_cu: int = 0, order: int = 0, gather: int = 0, stream=None):
+    """lyap_batch_run_noalloc: the batched launch on the caller's stream with
+    the library's fixed scratch array (no allocation in the library).
+    classes: dicts with desc, count, optional params / inits, and outputs
+    (lam, status, n_acc, n_rej) as device tensors the caller allocated."""
+    import torch
+
+    recs = (BatchClassRec * len(classes))()
+    for j, c in enumerate(classes):
+        lam, st, na, nr = c["outputs"]
+        n = int(c["count"])
+        if lam.shape != (n, c["desc"].n_exps) or lam.dtype != torch.float64 or not lam.is_contiguous():
+            raise ValueError("lambda output must be a contiguous float64 [count, n_exps] tensor")
+        for t in (st, na, nr):
+            if t is not None and (t.numel() < n or t.dtype != torch.int32):
+                raise ValueError("status/step outputs must be int32 tensors of count entries")
+        for key in ("params", "inits"):
+            t = c.get(key)
+            if t is not None and not (t.is_cuda and t.is_contiguous() and t.dtype == torch.float64):
+                raise ValueError(f"{key} must be a contiguous float64 CUDA tensor")
+        r = recs[j]
+        r.desc = c["desc"]
+        r.count = n
+        r.d_params = c["params"].data_ptr() if c.get("params") is not None else None
+        r.d_inits = c["inits"].data_ptr() if c.get("inits") is not None else None
+        r.d_lambda, r.d_status = lam.data_ptr(), _ptr(st)
+        r.d_n_acc, r.d_n_rej = _ptr(na), _ptr(nr)
+    tu = BatchTuning(int(layout), int(bool(occupancy3)), int(waves_per_cu), int(order), int(gather))
+    s = stream if stream is not None else torch.cuda.current_stream(torch.device("cuda", device))
+    check(load().lyap_batch_run_noalloc(int(device), recs, len(classes), ctypes.byref(tu),
+                                        ctypes.c_void_p(s.cuda_stream)))
 
 
 class Mlp:

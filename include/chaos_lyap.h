@@ -249,6 +249,25 @@ int lyap_batch_destroy(lyap_batch* b);
 int lyap_batch_layout_ppw(int layout, int dim, int* ppw);
 
 /*
+ * The batched launch without any allocation (no counterpart in the
+ * reference): for callers that cannot allocate device memory or create
+ * runtime objects while they run — code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  It runs the same kernels
+ * as lyap_batch_create + lyap_batch_launch, with the same results, but keeps
+ * the class records and work counters in a fixed device array of the library
+ * (at most 16 classes; 12,032 bytes per device, held until the process ends),
+ * staged by kernels on `stream`, and launches directly on `stream` (no graph
+ * of its own; the caller may capture the call).  The caller still owns every
+ * input and output buffer.  Every call on a device shares that device's
+ * array: calls that overlap on one device (other streams, other threads)
+ * race; order them on one stream.  More than 16 classes is an error
+ * (LYAP_ERR_INVALID).  The first call on a device looks the array up
+ * (hipGetSymbolAddress) and reads the CU count; no call allocates.
+ */
+int lyap_batch_run_noalloc(int device, const lyap_batch_class* classes, int n_classes,
+                           const lyap_batch_tuning* tuning, void* stream);
+
+/*
  * Surrogate network (no counterpart in the reference): (p1, p2) -> 2*n_fourier
  * Fourier features -> three hidden layers of `hidden` GELU units ->
  * [mu1, mu2, mu3, log-variance of lambda1, escape logit].  BF16 weights with
