@@ -76,6 +76,12 @@ class RefineResult:
     rerun: np.ndarray
     timings: dict = field(default_factory=dict)
     stats: dict = field(default_factory=dict)
+    # the held-out verification subsample: grid indices, the chosen precision's
+    # values there, and the FP64 judge's (None without the judge)
+    verify_idx: np.ndarray | None = None
+    verify_lam: np.ndarray | None = None
+    verify_fp64: np.ndarray | None = None
+    verify_fp64_status: np.ndarray | None = None
 
 
 # Side streams of refine(), created once per device: a fresh torch stream per
@@ -494,5 +500,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
                                   lam_v.cpu().numpy(), st_v.cpu().numpy(), lam_j.cpu().numpy(),
                                   st_j.cpu().numpy()))
     net.close()
-    return RefineResult(lam=lam.cpu().numpy(), status=status.cpu().numpy(),
-                        rerun=rerun_mask.cpu().numpy(), timings=tm, stats=stats)
+    out = RefineResult(lam=lam.cpu().numpy(), status=status.cpu().numpy(),
+                       rerun=rerun_mask.cpu().numpy(), timings=tm, stats=stats,
+                       verify_idx=verify.cpu().numpy(), verify_lam=lam_v.cpu().numpy())
+    if judge:
+        out.verify_fp64, out.verify_fp64_status = lam_j.cpu().numpy(), st_j.cpu().numpy()
+    return out

@@ -1,0 +1,155 @@
+"""Every benchmark configuration at its own settings (full transient and
+averaging times, the stated tolerances, step budgets and renormalisation
+intervals), on seeded subsamples of its grid, bit for bit against the CPU
+oracle in FP64 and in the oracle's mixed form:
+
+  C1  Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01, T 100 + 1000, MGS every 1
+  C2  Rossler (b=0.2) 512x512, adaptive DVERK 6(5) rtol=atol=1e-10, T 200 + 2000,
+      QR every 0.5
+  C3  Lorenz-96 N=16 256x256, adaptive Dopri5 rtol=1e-9, T 100 + 500, QR every 0.2
+  C4  the batched 40/30/20/8/2 % mix, adaptive Dopri5 rtol=1e-9, one persistent
+      launch replayed from its hipGraph
+  C5  the surrogate-refined Rossler sweep at full times (512x512 plane, 64x64
+      lattice), its FP64 judge and computed points against the oracle, its gates
+
+Each test prints the per-exponent max |dlambda_i| of the mixed form against
+FP64 on its sample (the error map's summary); the bar for those is the
+mixed form being the oracle's mixed form bit for bit."""
+import numpy as np
+import pytest
+import torch
+
+from chaos_oneapi_amd import lyap as L
+from chaos_oneapi_amd import refine as R
+from chaos_oneapi_amd import workloads as W
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+THREADS = 16
+
+
+def host(t):
+    lam, st, na, nr = t
+    return (lam.cpu().numpy(), st.cpu().numpy().astype(np.uint32), na.cpu().numpy().astype(np.uint32),
+            nr.cpu().numpy().astype(np.uint32))
+
+
+def exact(g, o, what):
+    for a, b, name in zip(g, o, ("lambda", "status", "accepted", "rejected")):
+        np.testing.assert_array_equal(a, b, err_msg=f"{what}: {name}")
+
+
+def sample(desc, k, seed):
+    n = desc.n1 * desc.n2
+    return np.sort(np.random.default_rng(seed).choice(n, size=k, replace=False)).astype(np.int64)
+
+
+def run_config(ctx, spec, k, seed, options=()):
+    """FP64 and mixed on k seeded points of the grid: GPU == oracle bit for bit;
+    returns the per-exponent max |mixed - FP64|."""
+    out = {}
+    for prec in (L.PREC_FP64, L.PREC_MIXED):
+        desc = L.finalize(L.make_desc(dict(spec, precision=prec)))
+        idx = sample(desc, k, seed)
+        print(f"case model={desc.model} dim={desc.dim} prec={prec} points={k} options={dict(options)}",
+              flush=True)
+        for key, v in options:
+            ctx.set_option(key, v)
+        try:
+            g = host(ctx.run(desc, idx_list=torch.as_tensor(idx, device="cuda")))
+            torch.cuda.synchronize()
+        finally:
+            for key, _ in options:
+                ctx.set_option(key, 0)
+        o = O.sweep(O.make_desc(desc.to_dict()), idx_list=idx, n_threads=THREADS)
+        exact(g, o, f"prec {prec}")
+        print(f"  status ok {(g[1] == 0).sum()}/{k}, steps mean {g[2].mean():.0f} max {g[2].max()}",
+              flush=True)
+        out[prec] = g
+    ok = (out[0][1] == 0) & (out[1][1] == 0)
+    d = np.abs(out[1][0][ok] - out[0][0][ok]).max(axis=0) if ok.any() else np.zeros(0)
+    print("  mixed vs FP64 per exponent: " + " ".join(f"{x:.2e}" for x in d), flush=True)
+    return d
+
+
+@pytest.mark.parametrize("quad", [0, 1])
+def test_c1_lorenz_64_dopri5_fixed(gpu_ctx, quad):
+    run_config(gpu_ctx, W.lorenz_fixed(64), 64, 101, (("lane_quad", quad),))
+
+
+@pytest.mark.parametrize("gather", [0, 4])
+def test_c2_rossler_512_dverk_1e10(gpu_ctx, gather):
+    spec = W.rossler_dverk(512)
+    desc = L.finalize(L.make_desc(spec))
+    assert desc.rtol == 1e-10 and desc.atol == 1e-10 and desc.max_steps == 2_000_000
+    assert desc.n_renorm_transient == 400 and desc.n_renorm_average == 4000
+    run_config(gpu_ctx, spec, 64, 102, (("lane_gather", gather),))
+
+
+@pytest.mark.parametrize("ppw", [0, 4])
+def test_c3_lorenz96_16_dopri5(gpu_ctx, ppw):
+    spec = W.lorenz96_dopri(256, 16)
+    desc = L.finalize(L.make_desc(spec))
+    assert desc.rtol == 1e-9 and desc.n_renorm_average == 2500
+    run_config(gpu_ctx, spec, 16, 103, (("l96_ppw_16", ppw),))
+
+
+@pytest.mark.parametrize("tuning", [dict(), dict(layout=1, occupancy3=True, gather=4)])
+def test_c4_batch_mix_full_settings(gpu_ctx, tuning):
+    dev = torch.device("cuda", 0)
+    mix = W.batch_mix(400, seed=104)
+    counts = {c["name"]: c["count"] for c in mix}
+    print(f"case C4 batch {counts} tuning {tuning}", flush=True)
+    assert counts["lorenz96_32"] >= 4
+    classes = [dict(desc=L.finalize(L.make_desc(c["spec"])), count=c["count"],
+                    params=torch.from_numpy(c["params"]).to(dev).contiguous()) for c in mix]
+    b = L.Batch(gpu_ctx, classes, **tuning)
+    outs = [host(o) for o in b.launch()]
+    torch.cuda.synchronize()
+    b.close()
+    for c, cl, g in zip(mix, classes, outs):
+        o = O.sweep(O.make_desc(cl["desc"].to_dict()), count=c["count"], params_in=c["params"],
+                    n_threads=THREADS)
+        exact(g, o, c["name"])
+        print(f"  {c['name']}: {c['count']} points exact, ok {(g[1] == 0).sum()}, "
+              f"steps max/min {g[2].max()}/{max(1, g[2][g[1] == 0].min() if (g[1] == 0).any() else 1)}",
+              flush=True)
+    # per-exponent error of the mixed form against FP64 for the Lorenz-96 N=32 class
+    c32 = [c for c in mix if c["name"] == "lorenz96_32"][0]
+    d64 = L.finalize(L.make_desc(dict(c32["spec"], precision=L.PREC_FP64)))
+    o64 = O.sweep(O.make_desc(d64.to_dict()), count=c32["count"], params_in=c32["params"], n_threads=THREADS)
+    gm = outs[[c["name"] for c in mix].index("lorenz96_32")]
+    ok = (gm[1] == 0) & (o64[1] == 0)
+    print("  N=32 mixed vs FP64 per exponent: "
+          + " ".join(f"{x:.1e}" for x in np.abs(gm[0][ok] - o64[0][ok]).max(axis=0)), flush=True)
+
+
+def test_c5_surrogate_refinement_full_times(gpu_ctx):
+    spec = W.rossler_dverk(512, precision=L.PREC_MIXED)
+    print("case C5 refine rossler 512x512 full times, 64x64 lattice", flush=True)
+    res = R.refine(gpu_ctx, spec, train_n=64, epochs=120, batch=1024)
+    st = res.stats
+    print({k: v for k, v in st.items() if k not in ("losses", "calib_table")}, flush=True)
+    assert st["gate_met"] and st["max_dlambda1_chosen_vs_fp64"] <= R.DLAMBDA1_GATE
+    assert st["class_agreement"] >= 0.999
+    # the FP64 judge and the mixed values at a seeded part of the verification
+    # subsample against the oracle
+    fd = L.finalize(L.make_desc(spec))
+    pick = np.random.default_rng(105).choice(len(res.verify_idx), size=96, replace=False)
+    idx = res.verify_idx[pick]
+    for prec, lam_g, st_g in ((L.PREC_FP64, res.verify_fp64[pick], res.verify_fp64_status[pick]),
+                              (L.PREC_MIXED, res.verify_lam[pick], None)):
+        od = O.make_desc(fd.to_dict())
+        od.precision = prec
+        lam_o, st_o, _, _ = O.sweep(od, idx_list=idx, n_threads=THREADS)
+        np.testing.assert_array_equal(lam_g, lam_o)
+        if st_g is not None:
+            np.testing.assert_array_equal(st_g.astype(np.uint32), st_o)
+    # computed points of the final map (the rerun among them) are the oracle's mixed form
+    comp = np.flatnonzero(res.rerun)
+    idx = np.sort(np.random.default_rng(106).choice(comp, size=96, replace=False))
+    od = O.make_desc(fd.to_dict())
+    lam_o, st_o, _, _ = O.sweep(od, idx_list=idx, n_threads=THREADS)
+    np.testing.assert_array_equal(res.lam[idx], lam_o)
+    np.testing.assert_array_equal(res.status[idx].astype(np.uint32), st_o)
