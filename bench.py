@@ -13,6 +13,8 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
       is uncertain or |mu1| < 0.02, and the FP64 judge on a seeded 1 %
       verification subsample.
   rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
+  rossler-2048  the 2048x2048 plane in the mixed form alone (no surrogate): one
+      mixed-form sweep of every point, the same kernel as the surrogate's rerun.
   lorenz-64     Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01; one mixed-form sweep.
   lorenz96-16   Lorenz-96 N=16 on a 256x256 (F, d) grid, adaptive Dopri5 rtol=1e-9; the
       tile shape (waves per point or points per wave) picked by the autotuner first.
@@ -197,7 +199,12 @@ def bench_sweep(args, ctx, spec, wl_name):
     from chaos_oneapi_amd import autotune as A
 
     tuned = tuned_lanes = None
-    if desc.model == L.MODEL_L96:
+    if args.no_tune:
+        # fixed shapes (profiling runs): the surrogate rerun's gather for the
+        # 3-D flows, the default Lorenz-96 kernel
+        if desc.dim == 3:
+            ctx.set_option("lane_gather", 4)
+    elif desc.model == L.MODEL_L96:
         tuned = A.tune_l96(ctx, desc, cache=A.Cache(None), time_scale=0.1)  # sets the tile shape on ctx
     elif desc.dim == 3:
         # persistent grid cap and renormalisation gather, set on ctx for the
@@ -225,6 +232,16 @@ def bench_sweep(args, ctx, spec, wl_name):
     lam_c, st_c = bufs[0].cpu().numpy().copy(), bufs[1].cpu().numpy().copy()
     na_c, nr_c = bufs[2].cpu().numpy().copy(), bufs[3].cpu().numpy().copy()
     f64_c, f32_c = F.sweep_flops_split(desc, na_c, nr_c)
+    kname = "k_var_lane" if desc.model != L.MODEL_L96 else (
+        "k_l96_wave" if tuned is not None and tuned.best[0] == "ppw" else "k_l96")
+    if args.no_plain:
+        # profiling runs: the chosen form's launches only
+        rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
+        rec["config"] = {"workload": wl_name, "points": n, "profiling_run": True,
+                         "mean_steps_per_point": float(na_c.mean())}
+        rec["roofline"] = roofline(f64_c, f32_c, kern_avg, kname + " (mixed) over the grid", args.probe,
+                                   measured_traffic(wl_name))
+        return rec
     lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
     ok = (st_c == 0) & (st_p == 0)
     d1 = float(np.max(np.abs(lam_c[ok, 0] - lam_p[ok, 0]))) if ok.any() else 0.0
@@ -256,8 +273,6 @@ def bench_sweep(args, ctx, spec, wl_name):
     if tuned is not None:
         rec["config"]["l96_tile"] = list(tuned.best)
         rec["config"]["l96_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}
-    kname = "k_var_lane" if desc.model != L.MODEL_L96 else (
-        "k_l96_wave" if tuned is not None and tuned.best[0] == "ppw" else "k_l96")
     rec["roofline"] = roofline(f64_c, f32_c, kern_avg, kname + " (mixed) over the grid", args.probe,
                                measured_traffic(wl_name))
     if not args.no_cpu:
@@ -614,6 +629,10 @@ def main():
     ap.add_argument("--error-maps", default=None,
                     help="write the full per-point lambda maps (.npz) to this path")
     ap.add_argument("--no-probe", action="store_true", help="skip the valu_peak issue-rate probe")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="single-grid workloads: fixed launch shapes instead of the autotuner (profiling)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="single-grid workloads: skip the plain FP64 sweep and comparisons (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a "
@@ -646,6 +665,8 @@ def main():
             args.warmup = 1 if args.warmup is None else args.warmup
         if args.workload == "rossler-512":
             spec, name = W.rossler_dverk(512, precision=L.PREC_MIXED), "rossler-512x512-dverk65-adaptive"
+        elif args.workload == "rossler-2048":
+            spec, name = W.rossler_dverk(2048, precision=L.PREC_MIXED), "rossler-2048x2048-dverk65-adaptive"
         elif args.workload == "lorenz-64":
             spec, name = W.lorenz_fixed(64, precision=L.PREC_MIXED), "lorenz-64x64-dopri5-fixed"
         elif args.workload == "lorenz96-16":
