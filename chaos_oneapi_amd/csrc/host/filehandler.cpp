@@ -40,55 +40,73 @@ int32_t saveNpy(const std::string& path, const std::vector<std::vector<double>>&
     return f.good() ? 0 : -1;
 }
 
-// filehandler.cpp:35-89 of the reference
+namespace {
+
+// One text row of the _Data block: every value as std::to_string prints it
+// (fixed notation, six decimals), then the row's optional label, separated by
+// single spaces and closed by a newline.
+void append_row(std::string& text, const std::vector<double>& values, const std::string* label) {
+    bool first = true;
+    for (double v : values) {
+        if (!first) text += ' ';
+        text += std::to_string(v);
+        first = false;
+    }
+    if (label) {
+        if (!first) text += ' ';
+        text += *label;
+    }
+    text += '\n';
+}
+
+json model_block(const ModelFactoryRegistry::ModelData& m) {
+    json block = json::object();
+    block["Name"] = m._model_name;
+    block["Dimension"] = m._dimension;
+    block["Number of parameters"] = m._param_number;
+    block["Name of parameters"] = m._model_param;
+    block["Name of phase"] = m._model_phase;
+    block["Equations"] = m._model_equat;
+    block["systemType"] = (int)m._type;
+    return block;
+}
+
+}  // namespace
+
+// The output of the reference's saveCalculatedData (filehandler.cpp:35-89):
+// one JSON object {Parameters, ID_Tool, _Data, Model} pretty-printed with a
+// four-space indent into "Data/<tool> <model>.json".  Here the folder is
+// created when missing and the rows also go, losslessly, to a .npy sidecar.
 int32_t saveCalculatedData(ToolID tool_id, const std::string& modelName, const Parameters* params,
                            const Task* task) {
-    if (!task->mResultsStr.empty()) assert(task->mResultsStr.size() == task->mResults.size());
     assert(params != nullptr);
-    json outputData = json::object();
-    outputData["Parameters"] = params->paramsToJson();
-    const std::string tool_name = GetToolName(tool_id);
-    outputData["ID_Tool"] = tool_name;
+    const auto& rows = task->mResults;
+    const auto& labels = task->mResultsStr;
+    assert(labels.empty() || labels.size() == rows.size());
+    const std::string tool = GetToolName(tool_id);
 
-    std::string data_str;
-    if (!task->mResults.empty()) data_str.reserve(task->mResults.size() * task->mResults.at(0).size() * 9);
-    size_t row_cnt = 0;
-    for (const auto& row : task->mResults) {
-        for (double val : row) {
-            data_str.append(std::to_string(val));
-            data_str.append(" ");
-        }
-        if (!task->mResultsStr.empty()) {
-            data_str.append(task->mResultsStr.at(row_cnt));
-            data_str.append(" ");
-        }
-        data_str.at(data_str.size() - 1) = '\n';
-        row_cnt += 1;
-    }
-    outputData["_Data"] = data_str;
+    std::string table;
+    table.reserve(rows.empty() ? 0 : rows.size() * rows.front().size() * 12);
+    for (size_t r = 0; r < rows.size(); ++r) append_row(table, rows[r], labels.empty() ? nullptr : &labels[r]);
 
-    ModelFactoryRegistry::Storage& storage = Singleton<ModelFactoryRegistry::Storage>::Instance();
-    const auto& m = storage._data.at(modelName);
-    json modelData = json::object();
-    modelData["Name"] = m._model_name;
-    modelData["Dimension"] = m._dimension;
-    modelData["Number of parameters"] = m._param_number;
-    modelData["Name of parameters"] = m._model_param;
-    modelData["Name of phase"] = m._model_phase;
-    modelData["Equations"] = m._model_equat;
-    modelData["systemType"] = (int)m._type;
-    outputData["Model"] = modelData;
+    const auto& models = Singleton<ModelFactoryRegistry::Storage>::Instance()._data;
+    const auto& model = models.at(modelName);
+    json doc = json::object();
+    doc["Parameters"] = params->paramsToJson();
+    doc["ID_Tool"] = tool;
+    doc["_Data"] = table;
+    doc["Model"] = model_block(model);
 
-    mkdir("Data", 0755);  // the reference required Data/ to exist already
-    const std::string stem = "Data/" + tool_name + " " + m._model_name;
-    std::ofstream jsonOStream(stem + ".json");
-    if (!jsonOStream.good()) {
-        std::cout << "Unable to write the file: " << stem << ".json" << std::endl;
+    mkdir("Data", 0755);
+    const std::string base = "Data/" + tool + " " + model._model_name;
+    std::ofstream out(base + ".json");
+    if (!out.good()) {
+        std::cout << "Unable to write the file: " << base << ".json" << std::endl;
         return -1;
     }
-    jsonOStream << outputData.dump(4) << std::endl;
-    if (saveNpy(stem + ".npy", task->mResults) != 0) {
-        std::cout << "Unable to write the file: " << stem << ".npy" << std::endl;
+    out << doc.dump(4) << std::endl;
+    if (saveNpy(base + ".npy", rows) != 0) {
+        std::cout << "Unable to write the file: " << base << ".npy" << std::endl;
         return -1;
     }
     return 0;

@@ -3,9 +3,9 @@
 //
 // Weights are kept as an FP32 master copy plus two BF16 copies (W^T for the
 // forward products, W for the input-gradient products); every matrix product
-// runs on v_mfma_f32_16x16x16bf16_1k with FP32 accumulation in one NT GEMM
-// kernel, C[M,N] = A[M,K] * B[N,K]^T (+ bias), both operands row-major so each
-// lane fetches four contiguous bf16 per operand.  Activations are stored in
+// runs on gfx950's v_mfma_f32_16x16x32_bf16 with FP32 accumulation in one NT
+// GEMM kernel, C[M,N] = A[M,K] * B[N,K]^T (+ bias), both operands row-major so
+// each lane fetches eight contiguous bf16 (16 bytes) per operand and k-step.  Activations are stored in
 // both row- and column-major BF16 so weight gradients are NT products too.
 // Adam runs on the FP32 master weights and re-issues the BF16 copies.
 #include <hip/hip_runtime.h>
@@ -23,6 +23,7 @@
 #include "lyap_launch.hpp"
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 namespace lyap {
@@ -43,39 +44,53 @@ LYAP_HD uint16_t f2bf(float f) {
 LYAP_HD float bf2f(uint16_t b) { return __builtin_bit_cast(float, (uint32_t)b << 16); }
 
 // ---------------------------------------------------------------------------
-// NT GEMM on MFMA.  One wave computes a 16 x (16*NB) tile of C; lane l holds
-// A[m0 + l%16][k0 + 4(l/16) .. +3] and B[n0 + 16j + l%16][k0 + 4(l/16) .. +3];
-// the accumulator element i of lane l is C[m0 + 4(l/16) + i][n0 + 16j + l%16].
-// M, N multiples of 16 (of 16*NB for N), K a multiple of 16.
+// NT GEMM on MFMA.  One wave computes a 16 x (16*NB) tile of C with gfx950's
+// double-K BF16 product, v_mfma_f32_16x16x32_bf16: per 32-wide k-step, lane l
+// holds A[m0 + l%16][k0 + 8(l/16) .. +7] and B[n0 + 16j + l%16][k0 + 8(l/16) .. +7]
+// (one 16-byte load each); a k range whose length is an odd multiple of 16
+// ends with one v_mfma_f32_16x16x16bf16_1k step (four bf16 per lane).  The
+// accumulator element i of lane l is C[m0 + 4(l/16) + i][n0 + 16j + l%16].
+// M, N multiples of 16 (of 16*NB for N), K and the k range multiples of 16.
 // ---------------------------------------------------------------------------
-// acc[j] += A-tile . B-tile_j over k in [k_lo, k_hi): four k-steps' loads are
-// issued before their MFMAs (one step's loads would otherwise wait out the L2
-// latency alone), then a step at a time for the rest; each accumulator still
-// takes the k-steps in increasing order
+// acc[j] += A-tile . B-tile_j over k in [k_lo, k_hi).  a_row / b_row are the
+// lane's rows (A row m0 + l%16, B row n0 + l%16) without the k offset; q = l/16.
+// Four k-steps' loads are issued before their MFMAs (one step's loads would
+// otherwise wait out the L2 latency alone), then a step at a time for the
+// rest; each accumulator takes the k-steps in increasing order.
 template <int NB>
 __device__ __forceinline__ void mfma_k_range(const uint16_t* __restrict__ a_row,
-                                             const uint16_t* __restrict__ b_base, int K, int k_lo,
+                                             const uint16_t* __restrict__ b_row, int q, int K, int k_lo,
                                              int k_hi, v4f (&acc)[NB]) {
+    const uint16_t* a8 = a_row + 8 * q;
+    const uint16_t* b8 = b_row + 8 * q;
     int k0 = k_lo;
-    for (; k0 + 64 <= k_hi; k0 += 64) {
-        v4s a[4], b[4][NB];
+    for (; k0 + 128 <= k_hi; k0 += 128) {
+        v8bf a[4], b[4][NB];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            a[u] = *reinterpret_cast<const v4s*>(a_row + k0 + 16 * u);
+            a[u] = *reinterpret_cast<const v8bf*>(a8 + k0 + 32 * u);
 #pragma unroll
             for (int j = 0; j < NB; ++j)
-                b[u][j] = *reinterpret_cast<const v4s*>(b_base + (size_t)(16 * j) * K + k0 + 16 * u);
+                b[u][j] = *reinterpret_cast<const v8bf*>(b8 + (size_t)(16 * j) * K + k0 + 32 * u);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int j = 0; j < NB; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], b[u][j], acc[j], 0, 0, 0);
+            for (int j = 0; j < NB; ++j) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[u], b[u][j], acc[j], 0, 0, 0);
     }
-    for (; k0 < k_hi; k0 += 16) {
-        const v4s a = *reinterpret_cast<const v4s*>(a_row + k0);
+    for (; k0 + 32 <= k_hi; k0 += 32) {
+        const v8bf a = *reinterpret_cast<const v8bf*>(a8 + k0);
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const v4s b = *reinterpret_cast<const v4s*>(b_base + (size_t)(16 * j) * K + k0);
+            const v8bf b = *reinterpret_cast<const v8bf*>(b8 + (size_t)(16 * j) * K + k0);
+            acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
+        }
+    }
+    if (k0 < k_hi) {  // a last 16-wide step
+        const v4s a = *reinterpret_cast<const v4s*>(a_row + 4 * q + k0);
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const v4s b = *reinterpret_cast<const v4s*>(b_row + 4 * q + (size_t)(16 * j) * K + k0);
             acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
         }
     }
@@ -101,8 +116,7 @@ __global__ __launch_bounds__(256) void k_gemm_nt(const uint16_t* __restrict__ A,
     v4f acc[NB];
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
-    const uint16_t* a_row = A + (size_t)(m0 + r) * K + 4 * q;
-    mfma_k_range<NB>(a_row, B + (size_t)(n0 + r) * K + 4 * q, K, 0, K, acc);
+    mfma_k_range<NB>(A + (size_t)(m0 + r) * K, B + (size_t)(n0 + r) * K, q, K, 0, K, acc);
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         const int col = n0 + 16 * j + r;
@@ -159,7 +173,7 @@ __device__ __forceinline__ void wgrad_tile(int t, float* part, const uint16_t* _
 #pragma unroll
     for (int j = 0; j < NB; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
     const int k_lo = split * kc, k_hi = min(K, k_lo + kc);
-    mfma_k_range<NB>(A + (size_t)(m0 + r) * K + 4 * q, B + (size_t)(n0 + r) * K + 4 * q, K, k_lo, k_hi, acc);
+    mfma_k_range<NB>(A + (size_t)(m0 + r) * K, B + (size_t)(n0 + r) * K, q, K, k_lo, k_hi, acc);
     float* P = part + split * 16 * TW;
 #pragma unroll
     for (int j = 0; j < NB; ++j)
@@ -286,7 +300,7 @@ __device__ __forceinline__ void dgelu_chunk(int vb, int tid, float (*tile)[17], 
     const int m0 = chunk * 64 + w * 16;
     const int r = lane & 15, q = lane >> 4;
     v4f acc[1] = {v4f{0.f, 0.f, 0.f, 0.f}};
-    mfma_k_range<1>(dZ + (size_t)(m0 + r) * K + 4 * q, W + (size_t)(n0 + r) * K + 4 * q, K, 0, K, acc);
+    mfma_k_range<1>(dZ + (size_t)(m0 + r) * K, W + (size_t)(n0 + r) * K, q, K, 0, K, acc);
     const int col = n0 + r;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -435,7 +449,7 @@ __global__ __launch_bounds__(256) void k_gemm_head_loss(const uint16_t* __restri
     const int m0 = (int)blockIdx.x * 64 + w * 16;
     const int r = lane & 15, q = lane >> 4;
     v4f acc[1] = {v4f{0.f, 0.f, 0.f, 0.f}};
-    mfma_k_range<1>(A + (size_t)(m0 + r) * K + 4 * q, W + (size_t)r * K + 4 * q, K, 0, K, acc);
+    mfma_k_range<1>(A + (size_t)(m0 + r) * K, W + (size_t)r * K, q, K, 0, K, acc);
     const float bv = bias[r];
 #pragma unroll
     for (int i = 0; i < 4; ++i) Z[(size_t)(m0 + 4 * q + i) * kOutPad + r] = 0.f + acc[0][i] + bv;
@@ -748,14 +762,14 @@ static void gemm(hipStream_t s, const uint16_t* A, const uint16_t* B, float* C, 
 }
 
 // weight-gradient product dW^T[M][N] (small M x N, K = the batch): K split
-// into up to kMaxSplits chunks of a multiple of 16, one wave each, inside one
+// into up to kMaxSplits chunks of a multiple of 32, one wave each, inside one
 // workgroup per output tile; with bias_part the same launch adds the layer's
 // bias-gradient chunk sums ([chunks][M]) into db
 static void wgrad_shape(int M, int N, int K, int* splits_out, int* kc_out, int* nb_out) {
     int splits = K / 256;
     if (splits > kMaxSplits) splits = kMaxSplits;
     if (splits < 1) splits = 1;
-    const int kc = ((K + splits - 1) / splits + 15) / 16 * 16;
+    const int kc = ((K + splits - 1) / splits + 31) / 32 * 32;  // whole double-K steps
     *splits_out = (K + kc - 1) / kc;
     *kc_out = kc;
     // 16 x 16 tiles unless 16 x 64 ones still give a block per CU: these
