@@ -531,6 +531,7 @@ def cpu_baseline_batch(mix, outs64, budget_s, threads):
     rng = np.random.default_rng(12346)
     total = sum(c["count"] for c in mix)
     sec_per_point, sampled, worst = 0.0, {}, 0.0
+    sec_per_point_1, sampled_1 = 0.0, {}
     t_all = 0.0
     for c, o64 in zip(mix, outs64):
         share = c["count"] / total
@@ -538,11 +539,12 @@ def cpu_baseline_batch(mix, outs64, budget_s, threads):
         od = O.make_desc(fd.to_dict())
         order = rng.permutation(c["count"])
         done, t_c, chunk = 0, 0.0, max(threads, 16)
+        params = np.ascontiguousarray(c["params"])
         while t_c < budget_s * max(share, 0.05) and done < len(order):
+            # the points keep their class index: it keys the seeded initial state
             idx = np.sort(order[done:done + chunk])
-            p = np.ascontiguousarray(c["params"][idx])
             t0 = time.perf_counter()
-            lam_o, st_o, _, _ = O.sweep(od, first=0, count=len(idx), params_in=p, n_threads=threads)
+            lam_o, st_o, _, _ = O.sweep(od, idx_list=idx, params_in=params, n_threads=threads)
             t_c += time.perf_counter() - t0
             ok = (st_o == 0) & (o64[1][idx] == 0)
             if ok.any():
@@ -552,9 +554,19 @@ def cpu_baseline_batch(mix, outs64, budget_s, threads):
         sec_per_point += share * t_c / done
         t_all += t_c
         sampled[c["name"]] = done
+        # one core: a few points of the class (Lorenz-96 points cost seconds each)
+        k1 = 8 if c["spec"]["dim"] == 3 else (2 if c["spec"]["dim"] == 8 else 1)
+        idx1 = np.sort(order[:k1])
+        t0 = time.perf_counter()
+        O.sweep(od, idx_list=idx1, params_in=params, n_threads=1)
+        sec_per_point_1 += share * (time.perf_counter() - t0) / len(idx1)
+        sampled_1[c["name"]] = len(idx1)
     rec = {"value": 1.0 / sec_per_point, "unit": "points/s", "cores": threads, "kind": "port",
            "sample": "seeded samples per class " + ", ".join(f"{k} {v}" for k, v in sampled.items())
                      + f", FP64 oracle, OpenMP {threads} threads, {t_all:.1f} s; rate of the 40/30/20/8/2 mix",
+           "single_core_value": 1.0 / sec_per_point_1,
+           "single_core_sample": "1 thread on " + ", ".join(f"{k} {v}" for k, v in sampled_1.items())
+                                 + " points; rate of the mix",
            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
     return rec, {"oracle_sample": int(sum(sampled.values())), "max_dlambda_fp64_vs_oracle": worst}
 
