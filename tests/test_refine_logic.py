@@ -158,3 +158,19 @@ def test_classify():
     lam1 = np.array([0.02, 0.005, -0.3, 0.5])
     st = np.array([0, 0, 0, 4])
     assert R.classify(lam1, st).tolist() == [1, 0, 0, 2]
+
+
+def test_spread_order_is_a_permutation_with_the_kernels_stride():
+    """refine.spread_order is the kernels' default item order (position
+    i * s mod n, s the first integer >= n/phi coprime to n): a permutation of
+    range(n), for sizes with and without a common factor at n/phi."""
+    import math
+
+    for n in (1, 2, 3, 10, 64, 1000, 4096, 65536, 3_980_001):
+        p = R.spread_order(n, "cpu").numpy()
+        assert sorted(p.tolist()) == list(range(n)) if n <= 65536 else len(np.unique(p)) == n
+        if n > 2:
+            s = int(n * 0.6180339887498949)
+            while math.gcd(s, n) != 1:
+                s += 1
+            assert p[1] == s % n and p[2] == (2 * s) % n

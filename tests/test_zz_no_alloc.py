@@ -74,3 +74,33 @@ def test_noalloc_rejects_more_classes_than_its_array(ctx):
     with pytest.raises(L.LyapError) as e:
         L.batch_run_noalloc(0, (classes * 4)[:17])
     assert e.value.code == L.ERR_INVALID and "scratch" in str(e.value)
+
+
+def test_noalloc_under_graph_capture(ctx):
+    """The allocation-free launch captured into a HIP graph (its staging and
+    reset are kernels, no memset nodes) and replayed several times, with eager
+    work between the replays: every replay equals the eager call."""
+    mix = W.batch_mix(700, seed=34, time_scale=0.01)
+    print("case noalloc under graph capture: " + ", ".join(f"{c['name']} {c['count']}" for c in mix),
+          flush=True)
+    classes = device_classes(mix)
+    L.batch_run_noalloc(0, classes)  # eager (also the device's one-time set-up)
+    torch.cuda.synchronize()
+    want = host([c["outputs"] for c in classes])
+    side = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=side, capture_error_mode="thread_local"):
+        L.batch_run_noalloc(0, classes, stream=side)
+    for rep in range(3):
+        for c in classes:  # eager work between replays: poison the outputs
+            for t in c["outputs"]:
+                t.fill_(-5)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        got = host([c["outputs"] for c in classes])
+        for m, a_cls, b_cls in zip(mix, got, want):
+            for a, b in zip(a_cls, b_cls):
+                np.testing.assert_array_equal(a, b, err_msg=f"replay {rep} {m['name']}")
+        print(f"replay {rep}: equal", flush=True)
+    del g
