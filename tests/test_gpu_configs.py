@@ -153,3 +153,40 @@ def test_c5_surrogate_refinement_full_times(gpu_ctx):
     lam_o, st_o, _, _ = O.sweep(od, idx_list=idx, n_threads=THREADS)
     np.testing.assert_array_equal(res.lam[idx], lam_o)
     np.testing.assert_array_equal(res.status[idx].astype(np.uint32), st_o)
+
+
+# The reference's own shipped configuration (ChaosMigration/config.json:2-36):
+# shimizu over alpha in [0.2, 2] x lambda in [0, 1.2] (200 x 200), integration
+# step 1e-2, skips 1e3 (attractor) + 1e3 (slave trajectories), total time 1e4,
+# normalisation every 1e-2 (every step), epsilon 1e-7, initial state
+# (1e-4, 0, 0); using_qr = using_var = false selects the FD mode.
+REFERENCE_SHIMIZU = dict(model=L.MODEL_SHIMIZU, dim=3, n_exps=3, n_params=2, integrator=L.INT_DVERK,
+                         adaptive=0, precision=L.PREC_FP64, h=1e-2, eps=1e-7, t_transient=1e3,
+                         t_slave=1e3, t_average=1e4, t_renorm=1e-2, base_params=[0.0, 0.0],
+                         init_state=[1e-4, 0.0, 0.0], init_kind=L.INIT_NONE, init_param_offset=-1)
+REFERENCE_SHIMIZU.update(W.grid_fields(0, 0.2, 2.0, 200, 1, 0.0, 1.2, 200))
+
+
+@pytest.mark.parametrize("method,k", [(L.METHOD_FD_REF, 12), (L.METHOD_VAR_REF, 8), (L.METHOD_QR_REF, 4)])
+def test_reference_shimizu_config_full_settings(gpu_ctx, method, k):
+    """The reference's shipped shimizu entry at its own settings (1.2 million
+    steps a point; the FD default, and the VAR and QR modes the same entry
+    selects with using_var / using_qr) on seeded points of its 200 x 200
+    grid, bit for bit against the oracle; the float-accumulated loop counts
+    of cuda/lyapunov.dp.cpp are the reference's."""
+    desc = L.finalize(L.make_desc(dict(REFERENCE_SHIMIZU, method=method)))
+    # for (double t = 0; t < 1e3; t += 1e-2) runs 100001 times (the sum falls
+    # short of 1e3 after 100000 additions); the averaging loop 1000000 times
+    assert desc.ref_steps_transient == 100001 and desc.ref_steps_slave == 100001
+    assert desc.ref_steps_average == 1000000
+    idx = sample(desc, k, 107 + method)
+    print(f"case reference shimizu config method {method} points {k}", flush=True)
+    g = host(gpu_ctx.run(desc, idx_list=torch.as_tensor(idx, device="cuda")))
+    torch.cuda.synchronize()
+    o = O.sweep(O.make_desc(desc.to_dict()), idx_list=idx, n_threads=THREADS)
+    np.testing.assert_array_equal(g[0], o[0])
+    # the reference modes carry no escape test: an orbit that diverges (lambda
+    # near 0 leaves z' = x^2 undamped) gives NaN there as in the reference
+    finite = int(np.isfinite(g[0]).all(axis=1).sum())
+    print(f"  {finite}/{k} finite; lambda of the first: " + " ".join(f"{v:.6f}" for v in g[0][0]), flush=True)
+    assert finite >= k // 2
