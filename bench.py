@@ -13,6 +13,9 @@ seeded initial-state rule, so nothing crosses PCIe in the timed region):
       is uncertain or |mu1| < 0.02, and the FP64 judge on a seeded 1 %
       verification subsample.
   rossler-512   the same system on a 512x512 grid; a step is one mixed-form sweep.
+  shimizu-200-ref  the reference's shipped configuration (shimizu 200x200,
+      alpha in [0.2,2] x lambda in [0,1.2], h=1e-2, skips 1e3 + 1e3, total 1e4,
+      renormalisation every step, FD mode, FP64): one sweep of the grid.
   rossler-2048  the 2048x2048 plane in the mixed form alone (no surrogate): one
       mixed-form sweep of every point, the same kernel as the surrogate's rerun.
   lorenz-64     Lorenz (sigma=10) 64x64, Dopri5 fixed h=0.01; one mixed-form sweep.
@@ -277,6 +280,50 @@ def bench_sweep(args, ctx, spec, wl_name):
                                measured_traffic(wl_name))
     if not args.no_cpu:
         cpu, parity = cpu_baseline(desc, lam_c, lam_p, args.cpu_budget, args.cpu_threads)
+        rec["cpu_baseline"] = cpu
+        rec["config"].update(parity)
+    return rec
+
+
+def bench_reference_mode(args, ctx, spec, wl_name):
+    """The reference's own configuration through its own mode (FP64, fixed-step
+    DVERK, the FD default): one sweep of the grid a step; the CPU oracle of the
+    same mode beside it.  There is no other precision to compare with."""
+    import numpy as np
+    import torch
+
+    from chaos_oneapi_amd import flops as F
+    from chaos_oneapi_amd import lyap as L
+
+    desc = L.finalize(L.make_desc(spec))
+    n = desc.n1 * desc.n2
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
+            torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev),
+            torch.empty(n, dtype=torch.int32, device=dev))
+    for _ in range(args.warmup):
+        ctx.run(desc, out=bufs)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ev[i][0].record(stream)
+        ctx.run(desc, out=bufs)
+        ev[i][1].record(stream)
+    torch.cuda.synchronize()
+    ms_per_step = (time.perf_counter() - t0) * 1e3 / args.steps
+    kern_avg = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    lam = bufs[0].cpu().numpy()
+    rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
+    rec["config"] = {"workload": wl_name, "points": n,
+                     "chosen_form": "the reference's FD mode, FP64 (its shipped default)",
+                     "points_finite": int(np.isfinite(lam).all(axis=1).sum())}
+    rec["roofline"] = roofline(F.ref_fd_point_flops(desc) * n, 0.0, kern_avg,
+                               "k_fdref_lane (FP64) over the grid", args.probe, measured_traffic(wl_name))
+    if not args.no_cpu:
+        cpu, parity = cpu_baseline(desc, None, lam, args.cpu_budget, args.cpu_threads)
         rec["cpu_baseline"] = cpu
         rec["config"].update(parity)
     return rec
@@ -679,6 +726,8 @@ def main():
             spec, name = W.rossler_dverk(512, precision=L.PREC_MIXED), "rossler-512x512-dverk65-adaptive"
         elif args.workload == "rossler-2048":
             spec, name = W.rossler_dverk(2048, precision=L.PREC_MIXED), "rossler-2048x2048-dverk65-adaptive"
+        elif args.workload == "shimizu-200-ref":
+            spec, name = W.shimizu_reference(), "shimizu-200x200-reference-fd"
         elif args.workload == "lorenz-64":
             spec, name = W.lorenz_fixed(64, precision=L.PREC_MIXED), "lorenz-64x64-dopri5-fixed"
         elif args.workload == "lorenz96-16":
@@ -691,6 +740,8 @@ def main():
             args.steps = 1 if args.steps is None else args.steps
             args.warmup = 0 if args.warmup is None else args.warmup
             rec = bench_batch(args, ctx, args.batch_points, name)
+        elif spec["method"] != L.METHOD_VAR:
+            rec = bench_reference_mode(args, ctx, spec, name)
         else:
             rec = bench_sweep(args, ctx, spec, name)
     print(json.dumps(rec), flush=True)

@@ -118,3 +118,20 @@ def sweep_flops(desc, n_acc, n_rej) -> float:
     na = np.asarray(n_acc, dtype=np.float64)
     nr = np.asarray(n_rej, dtype=np.float64)
     return float(acc * na.sum() + rej * nr.sum() + mgs(desc.dim, desc.dim) * renorms * len(na))
+
+
+def ref_fd_point_flops(desc) -> float:
+    """Algorithmic flops of one point in the reference's FD mode
+    (cuda/lyapunov.dp.cpp:5-68 restated): the main trajectory alone over the
+    transient, then the main and k slave trajectories (fixed-step DVERK, its
+    seven live stages) over the slave and averaging phases, with the slaves
+    made relative to the main state, Gram-Schmidt-orthogonalised and
+    normalised (every step in the slave phase, every add_skip_steps steps in
+    the averaging phase) and moved back."""
+    n, k = desc.dim, desc.n_exps
+    base = per_step(desc.model, n, 0, L.INT_DVERK, False)[1]
+    renorm = mgs(n, k) + 2 * k * n
+    every = max(1.0, float(desc.add_skip_steps))
+    return float(desc.ref_steps_transient * base
+                 + desc.ref_steps_slave * ((k + 1) * base + renorm)
+                 + desc.ref_steps_average * ((k + 1) * base + renorm / every))

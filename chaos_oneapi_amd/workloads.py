@@ -102,6 +102,21 @@ def batch_mix(n_points: int = 1 << 20, seed: int = 20240604, precision: int = L.
     return out
 
 
+def shimizu_reference(n1: int = 200, n2: int = 200, method: int = L.METHOD_FD_REF) -> dict:
+    """The reference's shipped configuration (ChaosMigration/config.json:2-36):
+    shimizu (alpha, lambda) over [0.2, 2] x [0, 1.2], integration step 1e-2,
+    skip times 1e3 (attractor) and 1e3 (slave trajectories), total time 1e4,
+    normalisation every 1e-2, epsilon 1e-7, initial state (1e-4, 0, 0); the
+    reference's FD mode (using_qr = using_var = false) unless another method
+    is given."""
+    d = dict(model=L.MODEL_SHIMIZU, dim=3, n_exps=3, n_params=2, method=method, integrator=L.INT_DVERK,
+             adaptive=0, precision=L.PREC_FP64, h=1e-2, eps=1e-7, t_transient=1e3, t_slave=1e3,
+             t_average=1e4, t_renorm=1e-2, base_params=[0.0, 0.0], init_state=[1e-4, 0.0, 0.0],
+             init_kind=L.INIT_NONE, init_param_offset=-1)
+    d.update(grid_fields(0, 0.2, 2.0, n1, 1, 0.0, 1.2, n2))
+    return d
+
+
 def henon(n: int = 256, precision: int = L.PREC_FP64, t_transient: float = 1000.0,
           t_average: float = 100000.0, t_renorm: float = 10.0) -> dict:
     """Henon map x' = 1 - a x^2 + y, y' = b x over a in [1.0, 1.42] x b in

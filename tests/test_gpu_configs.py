@@ -155,16 +155,9 @@ def test_c5_surrogate_refinement_full_times(gpu_ctx):
     np.testing.assert_array_equal(res.status[idx].astype(np.uint32), st_o)
 
 
-# The reference's own shipped configuration (ChaosMigration/config.json:2-36):
-# shimizu over alpha in [0.2, 2] x lambda in [0, 1.2] (200 x 200), integration
-# step 1e-2, skips 1e3 (attractor) + 1e3 (slave trajectories), total time 1e4,
-# normalisation every 1e-2 (every step), epsilon 1e-7, initial state
-# (1e-4, 0, 0); using_qr = using_var = false selects the FD mode.
-REFERENCE_SHIMIZU = dict(model=L.MODEL_SHIMIZU, dim=3, n_exps=3, n_params=2, integrator=L.INT_DVERK,
-                         adaptive=0, precision=L.PREC_FP64, h=1e-2, eps=1e-7, t_transient=1e3,
-                         t_slave=1e3, t_average=1e4, t_renorm=1e-2, base_params=[0.0, 0.0],
-                         init_state=[1e-4, 0.0, 0.0], init_kind=L.INIT_NONE, init_param_offset=-1)
-REFERENCE_SHIMIZU.update(W.grid_fields(0, 0.2, 2.0, 200, 1, 0.0, 1.2, 200))
+# The reference's own shipped configuration (ChaosMigration/config.json:2-36),
+# workloads.shimizu_reference: the FD mode by default.
+REFERENCE_SHIMIZU = W.shimizu_reference()
 
 
 @pytest.mark.parametrize("method,k", [(L.METHOD_FD_REF, 12), (L.METHOD_VAR_REF, 8), (L.METHOD_QR_REF, 4)])

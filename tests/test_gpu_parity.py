@@ -150,14 +150,20 @@ def test_in_order_items_over_an_index_list_match_oracle(gpu_ctx, model):
 
 
 @pytest.mark.parametrize("method", [L.METHOD_VAR_REF, L.METHOD_QR_REF, L.METHOD_FD_REF])
-@pytest.mark.parametrize("model", ["lorenz", "shimizu"])
+@pytest.mark.parametrize("model", ["lorenz", "shimizu", "rossler"])
 def test_reference_modes_match_oracle(gpu_ctx, model, method):
-    spec = W.lorenz_fixed(n=4, t_transient=2.0, t_average=5.0)
+    """The reference's VAR, QR and FD modes on 81 points, renormalising every
+    5 steps."""
+    spec = W.lorenz_fixed(n=9, t_transient=2.0, t_average=5.0)
     spec.update(method=method, integrator=L.INT_DVERK, eps=1e-7, t_slave=1.0, t_renorm=0.05)
     if model == "shimizu":
         spec.update(model=L.MODEL_SHIMIZU, n_params=2, base_params=[0.0, 0.0],
                     init_state=[1e-4, 0.0, 0.0], init_kind=L.INIT_NONE)
-        spec.update(W.grid_fields(0, 0.2, 2.0, 4, 1, 0.0, 1.2, 4))
+        spec.update(W.grid_fields(0, 0.2, 2.0, 9, 1, 0.0, 1.2, 9))
+    elif model == "rossler":
+        spec.update(model=L.MODEL_ROSSLER, base_params=[0.0, 0.0, 0.2], init_state=[1.0, 1.0, 1.0],
+                    init_scale=0.1)
+        spec.update(W.grid_fields(0, 0.1, 0.3, 9, 1, 4.0, 6.0, 9))
     g, o = run_both(gpu_ctx, spec)
     assert_match(g, o, TOL_FP64, exact=True)
 
