@@ -178,3 +178,21 @@ def test_finalize_maps():
     with pytest.raises(L.LyapError, match="no map branch") as e:
         L.finalize(L.make_desc(qr))
     assert e.value.code == L.ERR_UNSUPPORTED
+
+
+def test_noalloc_and_reference_entry_reject_before_touching_a_device():
+    """The allocation-free batched launch refuses a NULL class list and more
+    classes than its fixed array holds, and the cudaLyapunov-shaped entry a
+    NULL context, before any device call (these run without a GPU)."""
+    lib = L.load()
+    assert lib.lyap_batch_run_noalloc(0, None, 1, None, None) == L.ERR_INVALID
+    recs = (L.BatchClassRec * 17)()
+    assert lib.lyap_batch_run_noalloc(0, recs, 17, None, None) == L.ERR_INVALID
+    assert b"scratch" in lib.lyap_last_error()
+    assert lib.lyap_cuda_lyapunov(None, 32, 32, 4, 0, 1, L.TYPE_FLOW, None, 3, L.MODEL_SHIMIZU, None, 2,
+                                  1e-7, 0.01, None, 3, 1.0, 1.0, 1.0, 1.0, None) == L.ERR_INVALID
+
+
+def test_ref_option_names_are_checked():
+    lib = L.load()
+    assert lib.lyap_ctx_set_option(None, b"no_such_option", 1) == L.ERR_INVALID

@@ -1,0 +1,52 @@
+"""Host logic of bench.py and the flop model behind its roofline: the
+precision split and weighted peak, the roofline record, the per-exponent
+error-map statistics (no GPU)."""
+import numpy as np
+
+import bench
+from chaos_oneapi_amd import flops as F
+from chaos_oneapi_amd import lyap as L
+from chaos_oneapi_amd import workloads as W
+
+
+def test_precision_split_of_a_rossler_dverk_step():
+    acc, rej = F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True)
+    f32 = F.per_step_split(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, True)
+    # the tangent part is binary32 but the last add of each of the 3 vectors'
+    # increments (3 components each) to the binary64 deviation vector
+    assert f32 == (acc - rej) - 9
+    assert F.per_step_split(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, False) == 0
+    desc = L.finalize(L.make_desc(W.rossler_dverk(4, precision=L.PREC_MIXED)))
+    na, nr = np.full(16, 1000), np.full(16, 10)
+    f64, f32s = F.sweep_flops_split(desc, na, nr)
+    assert abs((f64 + f32s) - F.sweep_flops(desc, na, nr)) < 1e-6
+    assert f32s == f32 * 16000
+
+
+def test_weighted_peak_is_the_harmonic_mix():
+    assert F.weighted_peak(1.0, 0.0) == F.PEAK_FP64_TFLOPS
+    assert abs(F.weighted_peak(0.0, 1.0) - F.PEAK_FP32_TFLOPS) < 1e-12
+    # equal flops: 2 / (1/78.6 + 1/157.3)
+    assert abs(F.weighted_peak(1.0, 1.0) - 2.0 / (1 / 78.6 + 1 / 157.3)) < 1e-9
+
+
+def test_roofline_record():
+    probe = {"fp64_fma_tflops": 60.0, "fp32_unpacked_fma_tflops": 60.0}
+    r = bench.roofline(3e13, 6e13, 1000.0, "k", probe, traffic=123)
+    assert abs(r["achieved"] - 90.0) < 1e-9 and r["unit"] == "TFLOP/s" and r["traffic"] == 123
+    assert abs(r["frac"] - 90.0 / F.weighted_peak(3e13, 6e13)) < 1e-12
+    assert abs(r["measured_issue_peak"] - 60.0) < 1e-9 and abs(r["frac_of_measured_issue_peak"] - 1.5) < 1e-12
+
+
+def test_error_map_statistics():
+    n1, n2 = 8, 8
+    a = np.zeros((64, 2))
+    b = np.zeros((64, 2))
+    b[5, 0] = 2e-3   # p1 index 0, p2 index 5: block (0, 2) of a 4 x 4 block map
+    b[63, 1] = 5e-4  # block (3, 3)
+    ok = np.ones(64, bool)
+    ok[10] = False
+    e = bench.error_map_stats(a, b, ok, n1, n2, blocks=4)
+    assert e["max"] == [2e-3, 5e-4] and e["over_1e-3"] == [1, 0]
+    assert e["block_max"][0][0][2] == 2e-3 and e["block_max"][1][3][3] == 5e-4
+    assert e["block_max"][0][1][1] == 0.0
