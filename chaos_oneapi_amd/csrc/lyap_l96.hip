@@ -6,11 +6,12 @@
 // replicated in every N-lane group (the replicas cost no extra instructions),
 // so the Jacobian at each stage state is available to every vector's lanes.
 //   - neighbour couplings x_{i+1}, x_{i-1}, x_{i-2} (and of each vector) are
-//     moves inside the N-lane group (DPP row rotations for N = 16, lane
-//     shuffles otherwise);
+//     moves inside the N-lane group (DPP row rotations for N = 16; for N = 32
+//     row shifts plus v_permlane16_swap across the row pair; lane shuffles
+//     otherwise);
 //   - the error norm and the Jacobian-norm step cap are butterfly reductions
 //     over the group (Ring::sum / Ring::max: the xor butterfly with offsets
-//     N/2 .. 1, by DPP rotations for N = 16), identical in every lane;
+//     N/2 .. 1, on the VALU for N = 16 and 32), identical in every lane;
 //   - renormalisation stages the N x N deviation matrix in LDS, transposed so
 //     lane r owns vector r, and runs right-looking MGS with one barrier per
 //     vector (double-buffered q_j).
@@ -86,7 +87,8 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
         uint32_t st = LYAP_PT_OK;
         double k1 = 0.0;
         if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
-            const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
+            double xp1, xm1, xm2;
+            ring.moves(x, xp1, xm1, xm2);
             k1 = fma(xp1 - xm2, xm1, fma(-dd, x, F));
         }
 
@@ -101,7 +103,8 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                 }
                 t_next = (double)(mr + 1) * t_renorm;
                 {
-                    const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
+                    double xp1, xm1, xm2;
+                    ring.moves(x, xp1, xm1, xm2);
                     const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
                     const double jn = ring.max(row);
                     if (h_try * jn > 1.0) h_try = 1.0 / jn;
@@ -140,7 +143,8 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                         any = true;
                     }
                     const double arg = any ? fma(h_try, acc, x) : x;
-                    const double xp1 = ring.p1(arg), xm1 = ring.m1(arg), xm2 = ring.m2(arg);
+                    double xp1, xm1, xm2;
+                    ring.moves(arg, xp1, xm1, xm2);
                     jm1[s] = (T)xm1;
                     jd[s] = (T)(xp1 - xm2);
                     if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
@@ -208,7 +212,8 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                             any = true;
                         }
                         const T varg = any ? fma(hT, acc, v0) : v0;
-                        const T vp1 = ring.p1(varg), vm1 = ring.m1(varg), vm2 = ring.m2(varg);
+                        T vp1, vm1, vm2;
+                        ring.moves(varg, vp1, vm1, vm2);
                         if constexpr (MIXED)
                             kv[s] = fma(vp1 - vm2, jm1[s], fma(-dT, varg, jd[s] * vm1));
                         else

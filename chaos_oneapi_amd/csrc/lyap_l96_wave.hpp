@@ -9,7 +9,8 @@
 // slot.  The slot's base state is replicated in each of its G groups (G = 1 at
 // P = 64/N: no replication).
 //   - neighbour couplings are moves inside the N-lane group (Ring: DPP row
-//     rotations for N = 8 and 16, lane shuffles for N = 32);
+//     rotations for N = 8 and 16; for N = 32, row shifts whose edge lanes
+//     take the other row of the pair through v_permlane16_swap);
 //   - the error norm and the Jacobian-norm cap are butterflies over the group
 //     (Ring::sum / Ring::max), bit-identical in every lane of the group;
 //   - renormalisation stages the slot's N x N deviation matrix in the wave's
@@ -44,17 +45,60 @@ LYAP_D double dpp_ror(double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
+// the value of lane l ^ 16 (the other 16-lane row of each row pair), by
+// gfx950's v_permlane16_swap: with both operands v it returns (A, B) where row
+// r of A is row r & ~1 of v and row r of B is row r | 1 of v, so an odd row
+// takes A and an even row B.  Every lane of the wave's row pairs takes part.
+LYAP_D uint32_t row_partner_u32(uint32_t v, bool odd_row) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return odd_row ? r[0] : r[1];
+}
+LYAP_D float row_partner(float v, bool odd_row) {
+    return __builtin_bit_cast(float, row_partner_u32(__builtin_bit_cast(uint32_t, v), odd_row));
+}
+LYAP_D double row_partner(double v, bool odd_row) {
+    const uint64_t b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = row_partner_u32((uint32_t)b, odd_row);
+    const uint32_t hi = row_partner_u32((uint32_t)(b >> 32), odd_row);
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
+// row shift by DPP that keeps `old` where the source lane falls off the row:
+// lane j of each row receives lane j - R (R > 0: row_shr) or j + |R| (R < 0:
+// row_shl) of the same row, and the lanes with no such source keep old
+template <int R>
+LYAP_D uint32_t dpp_shift_u32(uint32_t old, uint32_t v) {
+    constexpr int ctrl = R > 0 ? 0x110 + R : 0x100 - R;
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, ctrl, 0xF, 0xF, false);
+}
+template <int R>
+LYAP_D float dpp_shift(float old, float v) {
+    return __builtin_bit_cast(float, dpp_shift_u32<R>(__builtin_bit_cast(uint32_t, old), __builtin_bit_cast(uint32_t, v)));
+}
+template <int R>
+LYAP_D double dpp_shift(double old, double v) {
+    const uint64_t o = __builtin_bit_cast(uint64_t, old), b = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = dpp_shift_u32<R>((uint32_t)o, (uint32_t)b);
+    const uint32_t hi = dpp_shift_u32<R>((uint32_t)(o >> 32), (uint32_t)(b >> 32));
+    return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
+}
+
 // Couplings x_{i+1}, x_{i-1}, x_{i-2} of component i inside its N-lane group
 // (periodic).  CS is the lane stride between consecutive components: 1 for
 // groups of consecutive lanes, 2 for two N = 8 groups interleaved in one
 // 16-lane row.  Groups that are whole DPP rows (N = 16, CS = 1; N = 8, CS = 2)
-// move by row rotations on the VALU; others by lane shuffles (ds_bpermute).
-// Every lane of a group is active whenever one is (the callers' control is
-// group-uniform).
+// move by row rotations on the VALU.  Groups of 32 consecutive lanes (N = 32)
+// span a row pair: a component's neighbours are the row rotation of its own
+// row, except at the row's edge lanes, which take the rotation of the other
+// row of the pair (v_permlane16_swap) — all on the VALU.  Other groups move by
+// lane shuffles (ds_bpermute).  Every lane of a group is active whenever one
+// is (the callers' control is group-uniform).
 template <int N, int CS = 1>
 struct Ring {
     static constexpr bool kDpp = N * CS == 16;
+    static constexpr bool kPair = N == 32 && CS == 1;
     int src_p1, src_m1, src_m2;
+    bool odd_row;  // rows 1 and 3 of the wave
     LYAP_D explicit Ring(int lane) {
         static_assert(CS == 1 || (CS == 2 && N == 8), "component stride");
         const int i = CS == 1 ? lane % N : (lane & 15) / CS;
@@ -62,20 +106,40 @@ struct Ring {
         src_p1 = base + ((i + 1) % N) * CS;
         src_m1 = base + ((i + N - 1) % N) * CS;
         src_m2 = base + ((i + N - 2) % N) * CS;
+        odd_row = ((lane >> 4) & 1) != 0;
+    }
+    // the three neighbours at once (a row pair's edge lanes share one
+    // v_permlane16_swap of v): the row shift fills the inner lanes and the
+    // edge lanes keep the other row's value rotated into their place
+    template <class T>
+    LYAP_D void moves(T v, T& vp1, T& vm1, T& vm2) const {
+        if constexpr (kPair) {
+            const T pv = row_partner(v, odd_row);
+            vp1 = dpp_shift<-1>(dpp_ror<15>(pv), v);
+            vm1 = dpp_shift<1>(dpp_ror<1>(pv), v);
+            vm2 = dpp_shift<2>(dpp_ror<2>(pv), v);
+        } else {
+            vp1 = p1(v);
+            vm1 = m1(v);
+            vm2 = m2(v);
+        }
     }
     template <class T>
     LYAP_D T p1(T v) const {
         if constexpr (kDpp) return dpp_ror<16 - CS>(v);
+        else if constexpr (kPair) return dpp_shift<-1>(dpp_ror<15>(row_partner(v, odd_row)), v);
         else return __shfl(v, src_p1);
     }
     template <class T>
     LYAP_D T m1(T v) const {
         if constexpr (kDpp) return dpp_ror<CS>(v);
+        else if constexpr (kPair) return dpp_shift<1>(dpp_ror<1>(row_partner(v, odd_row)), v);
         else return __shfl(v, src_m1);
     }
     template <class T>
     LYAP_D T m2(T v) const {
         if constexpr (kDpp) return dpp_ror<2 * CS>(v);
+        else if constexpr (kPair) return dpp_shift<2>(dpp_ror<2>(row_partner(v, odd_row)), v);
         else return __shfl(v, src_m2);
     }
     // Butterfly sum / max over the group, identical in every lane and equal
@@ -84,7 +148,8 @@ struct Ring {
     // (in components), so the value o components away by rotation is the xor
     // partner's value, and a + b == b + a.
     LYAP_D double sum(double v) const {
-        if constexpr (kDpp) {
+        if constexpr (kDpp || kPair) {
+            if constexpr (kPair) v = v + row_partner(v, odd_row);  // offset 16: the other row
             if constexpr (N >= 16) v = v + dpp_ror<8 * CS>(v);
             v = v + dpp_ror<4 * CS>(v);
             v = v + dpp_ror<2 * CS>(v);
@@ -98,7 +163,8 @@ struct Ring {
     }
     LYAP_D double max(double v) const {
         auto mx = [](double a, double b) { return a > b ? a : b; };
-        if constexpr (kDpp) {
+        if constexpr (kDpp || kPair) {
+            if constexpr (kPair) v = mx(v, row_partner(v, odd_row));
             if constexpr (N >= 16) v = mx(v, dpp_ror<8 * CS>(v));
             v = mx(v, dpp_ror<4 * CS>(v));
             v = mx(v, dpp_ror<2 * CS>(v));
@@ -223,7 +289,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 ctrl.log_fac_old = log_floor;
                 ctrl.rejected = false;
                 if (INTEG == LYAP_INT_DOPRI5 && ADAPT) {
-                    const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
+                    double xp1, xm1, xm2;
+                    ring.moves(x, xp1, xm1, xm2);
                     k1 = fma(xp1 - xm2, xm1, fma(-dd, x, F));
                 }
                 active = true;
@@ -245,7 +312,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 go = false;
             } else {
                 t_next = (double)(mr + 1) * t_renorm;
-                const double xp1 = ring.p1(x), xm1 = ring.m1(x), xm2 = ring.m2(x);
+                double xp1, xm1, xm2;
+                ring.moves(x, xp1, xm1, xm2);
                 const double row = (2.0 * fabs(xm1) + fabs(xp1 - xm2)) + fabs(dd);
                 const double jn = ring.max(row);
                 if (h_try * jn > 1.0) h_try = 1.0 / jn;
@@ -285,7 +353,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                     any = true;
                 }
                 const double arg = any ? fma(h_try, acc, x) : x;
-                const double xp1 = ring.p1(arg), xm1 = ring.m1(arg), xm2 = ring.m2(arg);
+                double xp1, xm1, xm2;
+                ring.moves(arg, xp1, xm1, xm2);
                 jm1[s] = (T)xm1;
                 jd[s] = (T)(xp1 - xm2);
                 if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
@@ -364,7 +433,8 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                         }
 #pragma unroll
                         for (int q2 = 0; q2 < NV; ++q2) {
-                            const T vp1 = ring.p1(varg[q2]), vm1 = ring.m1(varg[q2]), vm2 = ring.m2(varg[q2]);
+                            T vp1, vm1, vm2;
+                            ring.moves(varg[q2], vp1, vm1, vm2);
                             // binary32: the jd*v(i-1) product first, so its lane move folds
                             // into a DPP multiply (the FP64 form keeps the fused order)
                             if constexpr (MIXED)
