@@ -145,8 +145,14 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                     const double arg = any ? fma(h_try, acc, x) : x;
                     double xp1, xm1, xm2;
                     ring.moves(arg, xp1, xm1, xm2);
-                    jm1[s] = (T)xm1;
-                    jd[s] = (T)(xp1 - xm2);
+                    if constexpr (MIXED) {
+                        // the binary32 tangent folds h into the Jacobian data
+                        jm1[s] = (T)(h_try * xm1);
+                        jd[s] = (T)(h_try * (xp1 - xm2));
+                    } else {
+                        jm1[s] = (T)xm1;
+                        jd[s] = (T)(xp1 - xm2);
+                    }
                     if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
                     else k[s] = fma(xp1 - xm2, xm1, fma(-dd, arg, F));
                 }
@@ -190,7 +196,11 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                     }
                 }
                 // tangent step of the lane's E vectors, J at every stage state
+                // mixed form: h folded into the Jacobian data (jm1, jd and
+                // dh = h d), so the stage arguments are v + sum_j a_sj K'_j and
+                // the products K'_s = h K_s; the increment is sum_j b_j K'_j
                 const T hT = (T)h_try;
+                const T dh = MIXED ? (T)(h_try * dd) : dT;
                 const RowB<TB, T, S> rbt;
 #pragma unroll
                 for (int m = 0; m < E; ++m) {
@@ -203,19 +213,19 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                             continue;
                         }
                         const RowA<TB, T, S> row(s);
-                        T acc = T(0);
+                        T acc = MIXED ? v0 : T(0);
                         bool any = false;
 #pragma unroll
                         for (int j = 0; j < S; ++j) {
                             if (j >= s || TB::a(s, j) == 0.0) continue;
-                            acc = any ? fma(row.c[j], kv[j], acc) : row.c[j] * kv[j];
+                            acc = (any || MIXED) ? fma(row.c[j], kv[j], acc) : row.c[j] * kv[j];
                             any = true;
                         }
-                        const T varg = any ? fma(hT, acc, v0) : v0;
+                        const T varg = MIXED ? acc : (any ? fma(hT, acc, v0) : v0);
                         T vp1, vm1, vm2;
                         ring.moves(varg, vp1, vm1, vm2);
                         if constexpr (MIXED)
-                            kv[s] = fma(vp1 - vm2, jm1[s], fma(-dT, varg, jd[s] * vm1));
+                            kv[s] = fma(vp1 - vm2, jm1[s], fma(-dh, varg, jd[s] * vm1));
                         else
                             kv[s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg));
                     }
@@ -227,7 +237,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                         accv = anyv ? fma(rbt.c[j], kv[j], accv) : rbt.c[j] * kv[j];
                         anyv = true;
                     }
-                    if constexpr (MIXED) V[m] = V[m] + (double)(hT * accv);
+                    if constexpr (MIXED) V[m] = V[m] + (double)accv;
                     else V[m] = fma(h_try, accv, V[m]);
                 }
                 x = xn;

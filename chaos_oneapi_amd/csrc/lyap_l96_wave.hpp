@@ -355,8 +355,14 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 const double arg = any ? fma(h_try, acc, x) : x;
                 double xp1, xm1, xm2;
                 ring.moves(arg, xp1, xm1, xm2);
-                jm1[s] = (T)xm1;
-                jd[s] = (T)(xp1 - xm2);
+                if constexpr (MIXED) {
+                    // the binary32 tangent folds h into the Jacobian data
+                    jm1[s] = (T)(h_try * xm1);
+                    jd[s] = (T)(h_try * (xp1 - xm2));
+                } else {
+                    jm1[s] = (T)xm1;
+                    jd[s] = (T)(xp1 - xm2);
+                }
                 if (s == 0 && INTEG == LYAP_INT_DOPRI5 && ADAPT) k[0] = k1;
                 else k[s] = fma(xp1 - xm2, xm1, fma(-dd, arg, F));
             }
@@ -402,7 +408,11 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 // tangent step of the lane's E vectors, J at every stage state;
                 // NV vectors advance through the stages together so that one
                 // vector's lane moves fill the others' wait states
+                // mixed form: h folded into the Jacobian data (jm1, jd and
+                // dh = h d), so the stage arguments are v + sum_j a_sj K'_j and
+                // the products K'_s = h K_s; the increment is sum_j b_j K'_j
                 const T hT = (T)h_try;
+                const T dh = MIXED ? (T)(h_try * dd) : dT;
                 const RowB<TB, T, S> rbt;
                 constexpr int NV = E % 2 == 0 ? 2 : 1;
 #pragma unroll
@@ -421,15 +431,25 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                         T varg[NV];
 #pragma unroll
                         for (int q2 = 0; q2 < NV; ++q2) {
-                            T acc = T(0);
-                            bool any = false;
+                            if constexpr (MIXED) {
+                                T acc = v0[q2];
 #pragma unroll
-                            for (int j = 0; j < S; ++j) {
-                                if (j >= s || TB::a(s, j) == 0.0) continue;
-                                acc = any ? fma(row.c[j], kv[q2][j], acc) : row.c[j] * kv[q2][j];
-                                any = true;
+                                for (int j = 0; j < S; ++j) {
+                                    if (j >= s || TB::a(s, j) == 0.0) continue;
+                                    acc = fma(row.c[j], kv[q2][j], acc);
+                                }
+                                varg[q2] = acc;
+                            } else {
+                                T acc = T(0);
+                                bool any = false;
+#pragma unroll
+                                for (int j = 0; j < S; ++j) {
+                                    if (j >= s || TB::a(s, j) == 0.0) continue;
+                                    acc = any ? fma(row.c[j], kv[q2][j], acc) : row.c[j] * kv[q2][j];
+                                    any = true;
+                                }
+                                varg[q2] = any ? fma(hT, acc, v0[q2]) : v0[q2];
                             }
-                            varg[q2] = any ? fma(hT, acc, v0[q2]) : v0[q2];
                         }
 #pragma unroll
                         for (int q2 = 0; q2 < NV; ++q2) {
@@ -438,7 +458,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                             // binary32: the jd*v(i-1) product first, so its lane move folds
                             // into a DPP multiply (the FP64 form keeps the fused order)
                             if constexpr (MIXED)
-                                kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(-dT, varg[q2], jd[s] * vm1));
+                                kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(-dh, varg[q2], jd[s] * vm1));
                             else
                                 kv[q2][s] = fma(vp1 - vm2, jm1[s], fma(jd[s], vm1, -dT * varg[q2]));
                         }
@@ -453,7 +473,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                             accv = anyv ? fma(rbt.c[j], kv[q2][j], accv) : rbt.c[j] * kv[q2][j];
                             anyv = true;
                         }
-                        if constexpr (MIXED) V[m0 + q2] = V[m0 + q2] + (double)(hT * accv);
+                        if constexpr (MIXED) V[m0 + q2] = V[m0 + q2] + (double)accv;
                         else V[m0 + q2] = fma(h_try, accv, V[m0 + q2]);
                     }
                 }

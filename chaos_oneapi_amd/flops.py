@@ -17,7 +17,11 @@ Per point: n_acc * (base + K * tangent) + n_rej * base + renorms * MGS.
 Precision split of the mixed form: the base step, error norm, MGS and log
 accumulation are binary64; a tangent vector's stage arguments, Jacobian
 products and stage sums are binary32 except the last add of its increment to
-the binary64 deviation vector (N flops per vector and step).
+the binary64 deviation vector (N flops per vector and step).  The Lorenz-96
+mixed form folds h into its binary32 Jacobian data (two binary64 products per
+component and stage, h*d once per step): its stage arguments are fma chains
+started on v (2*nnz per stage and component) and its increment sum(b_j K'_j)
+has no multiply by h (2*nnz(b) - 1).
 
 Peaks (MI355X, published): binary64 vector 78.6 TFLOP/s, binary32 vector
 157.3 TFLOP/s (the latter counts packed two-wide FMAs, which this build does
@@ -57,8 +61,9 @@ def _stages(integ: int, adaptive: bool, for_tangent: bool):
     return out
 
 
-def per_step(model: int, n: int, k: int, integ: int, adaptive: bool):
-    """(flops of an accepted step, flops of a rejected step) for one point."""
+def _step_parts(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool):
+    """(base step, binary32 tangent, binary64 tangent) flops of an accepted
+    step of one point; a rejected step costs the base step alone."""
     f_cost, jv_cost = MODEL_COST[model](n)
     nnz_b = DVERK_NNZ_B if integ == L.INT_DVERK else DOPRI_NNZ_B
     nnz_e = DVERK_NNZ_E if integ == L.INT_DVERK else DOPRI_NNZ_E
@@ -69,8 +74,19 @@ def per_step(model: int, n: int, k: int, integ: int, adaptive: bool):
     if adaptive:
         base += n * 2 * nnz_e + 7 * n + 2
     tan_st = _stages(integ, adaptive, for_tangent=True)
+    if mixed and model == L.MODEL_L96:
+        vec32 = len(tan_st) * jv_cost + n * sum(2 * z for z in tan_st if z > 0) + n * (2 * nnz_b - 1)
+        return base, k * vec32, k * n + 2 * n * len(tan_st) + 1
     tangent = len(tan_st) * jv_cost + n * sum(2 * z + 1 for z in tan_st if z > 0) + n * (2 * nnz_b + 1)
-    return base + k * tangent, base
+    if mixed:
+        return base, k * (tangent - n), k * n
+    return base, 0, k * tangent
+
+
+def per_step(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool = False):
+    """(flops of an accepted step, flops of a rejected step) for one point."""
+    base, t32, t64 = _step_parts(model, n, k, integ, adaptive, mixed)
+    return base + t32 + t64, base
 
 
 PEAK_FP64_TFLOPS = 78.6
@@ -80,10 +96,7 @@ PEAK_FP32_TFLOPS = 157.3
 def per_step_split(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool):
     """Binary32 flops of an accepted step (the rest of per_step's accepted
     count is binary64; a rejected step is all binary64)."""
-    if not mixed:
-        return 0
-    acc, base = per_step(model, n, k, integ, adaptive)
-    return (acc - base) - k * n
+    return _step_parts(model, n, k, integ, adaptive, mixed)[1]
 
 
 def weighted_peak(f64: float, f32: float) -> float:
@@ -113,7 +126,8 @@ def sweep_flops(desc, n_acc, n_rej) -> float:
     """Total algorithmic flops of a sweep from its per-point step counters."""
     import numpy as np
 
-    acc, rej = per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
+    acc, rej = per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive),
+                        desc.precision == L.PREC_MIXED)
     renorms = desc.n_renorm_transient + desc.n_renorm_average
     na = np.asarray(n_acc, dtype=np.float64)
     nr = np.asarray(n_rej, dtype=np.float64)

@@ -469,6 +469,44 @@ static void tangent_step_f32(int model, int integ, int n, const double* stage_x,
     }
 }
 
+/* Lorenz-96 mixed form: as tangent_step_f32 with the step h folded into the
+   binary32 Jacobian data (the tangent system is linear), which the kernels
+   form once per stage and share between the vectors of a lane:
+   J'_s = ((float)(h x_{i-1}), (float)(h (x_{i+1} - x_{i-2})), (float)(h d)),
+   stage arguments v + sum_j a_sj K'_j as one fma chain started on v, products
+   K'_s = J'_s arg (h times the stage derivatives), and the increment
+   sum(b_j K'_j) added to the FP64 vector. */
+static void l96_tangent_step_f32(int integ, int n, const double* stage_x, const double* p,
+                                 double h, double* v) {
+    float kv[8 * ORC_MAX_DIM], arg[ORC_MAX_DIM], vf[ORC_MAX_DIM];
+    const int S = n_stages(integ);
+    const float dh = (float)(h * p[1]);
+    for (int i = 0; i < n; ++i) vf[i] = (float)v[i];
+    for (int s = 0; s < S; ++s) {
+        float* ks = kv + s * ORC_MAX_DIM;
+        if (!stage_needed_for_solution(integ, s)) {
+            for (int i = 0; i < n; ++i) ks[i] = 0.0f;
+            continue;
+        }
+        const double* a = tab_a(integ, s);
+        for (int i = 0; i < n; ++i) {
+            float acc = vf[i];
+            for (int j = 0; j < s; ++j)
+                if (a[j] != 0.0) acc = fmaf((float)a[j], kv[j * ORC_MAX_DIM + i], acc);
+            arg[i] = acc;
+        }
+        const double* xs = stage_x + s * n;
+        for (int i = 0; i < n; ++i) {
+            const int ip1 = (i + 1) % n, im1 = (i + n - 1) % n, im2 = (i + n - 2) % n;
+            const float jm1 = (float)(h * xs[im1]);
+            const float jd = (float)(h * (xs[ip1] - xs[im2]));
+            ks[i] = fmaf(arg[ip1] - arg[im2], jm1, fmaf(-dh, arg[i], jd * arg[im1]));
+        }
+    }
+    const double* b = tab_b(integ);
+    for (int i = 0; i < n; ++i) v[i] = v[i] + (double)sum_f(b, S, kv, ORC_MAX_DIM, i);
+}
+
 /* cuda/tools/dverk.dp.cpp:46-80: every stage uses J at the same base point */
 void orc_frozen_tangent_step(int model, int n, const double* x_frozen, const double* p, double h,
                              double* v) {
@@ -916,7 +954,8 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             for (int64_t s = 0; s < R; ++s) {
                 orc_rk_step(d->model, integ, n, x, p, h, xn, NULL, sx);
                 for (int r = 0; r < k; ++r) {
-                    if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h, V + r * n);
+                    if (mixed && d->model == ORC_MODEL_L96) l96_tangent_step_f32(integ, n, sx, p, h, V + r * n);
+                    else if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h, V + r * n);
                     else orc_tangent_step(d->model, integ, n, sx, p, h, V + r * n);
                 }
                 memcpy(x, xn, sizeof(double) * n);
@@ -969,7 +1008,8 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             h = h_next;
             if (!acc) { ++n_rej; continue; }
             for (int r = 0; r < k; ++r) {
-                if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h_try, V + r * n);
+                if (mixed && d->model == ORC_MODEL_L96) l96_tangent_step_f32(integ, n, sx, p, h_try, V + r * n);
+                else if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h_try, V + r * n);
                 else orc_tangent_step(d->model, integ, n, sx, p, h_try, V + r * n);
             }
             memcpy(x, xn, sizeof(double) * n);

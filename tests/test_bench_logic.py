@@ -23,6 +23,24 @@ def test_precision_split_of_a_rossler_dverk_step():
     assert f32s == f32 * 16000
 
 
+def test_l96_mixed_step_counts_the_folded_tangent():
+    # Dopri5 tangent: 6 stages, a-row nonzeros 1..5 (15), 5 nonzero b.  Folding h
+    # into the Jacobian data saves, per vector and component, the fma(h, acc, v)
+    # of the 5 stages with a stage sum and the multiply by h of the increment
+    # (6 flops), and costs 2 binary64 products per component and stage plus h*d.
+    n = k = 16
+    acc64, rej64 = F.per_step(L.MODEL_L96, n, k, L.INT_DOPRI5, True)
+    acc32, rej32 = F.per_step(L.MODEL_L96, n, k, L.INT_DOPRI5, True, True)
+    assert rej32 == rej64
+    assert acc64 - acc32 == 6 * k * n - (2 * n * 6 + 1)
+    f32 = F.per_step_split(L.MODEL_L96, n, k, L.INT_DOPRI5, True, True)
+    # binary32: 6 Jacobian products (7n) + stage chains (2*15n) + b sum (9n) per vector
+    assert f32 == k * (6 * 7 * n + 30 * n + 9 * n)
+    # the 3-D flows' mixed form counts as the FP64 form (only the precision moves)
+    assert F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, True) == \
+        F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True)
+
+
 def test_weighted_peak_is_the_harmonic_mix():
     assert F.weighted_peak(1.0, 0.0) == F.PEAK_FP64_TFLOPS
     assert abs(F.weighted_peak(0.0, 1.0) - F.PEAK_FP32_TFLOPS) < 1e-12
