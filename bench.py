@@ -130,6 +130,8 @@ def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int, idx_
 
     od = O.make_desc(desc.to_dict())
     od.precision = 0
+    if od.method == O.METHOD_VAR_TRACE:
+        od.method = O.METHOD_VAR
     n = desc.n1 * desc.n2
     rng = np.random.default_rng(12345)
     order = rng.permutation(n) if idx_pool is None else rng.permutation(idx_pool)
@@ -170,6 +172,16 @@ def cpu_baseline(desc, lam_chosen, lam_fp64, budget_s: float, threads: int, idx_
     return rec, parity
 
 
+def chosen_form_name(desc):
+    from chaos_oneapi_amd import lyap as L
+
+    name = "mixed" if desc.precision == L.PREC_MIXED else "fp64"
+    if desc.method == L.METHOD_VAR_TRACE:
+        name += (f", {desc.dim - 1} deviation vectors + trace closure (lambda_{desc.dim} = "
+                 "<tr J> - the others)")
+    return name
+
+
 def plain_fp64(ctx, desc):
     """One plain FP64 sweep of the grid: (lam, status, kernel ms, flops)."""
     import torch
@@ -179,6 +191,7 @@ def plain_fp64(ctx, desc):
 
     plain = L.SweepDesc.from_buffer_copy(desc)
     plain.precision = L.PREC_FP64
+    plain.method = L.METHOD_VAR if desc.method == L.METHOD_VAR_TRACE else desc.method
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -252,7 +265,7 @@ def bench_sweep(args, ctx, spec, wl_name):
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
     rec["config"] = {
         "workload": wl_name, "points": n,
-        "chosen_form": "mixed" if desc.precision == L.PREC_MIXED else "fp64",
+        "chosen_form": chosen_form_name(desc),
         "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
         "max_dlambda1_chosen_vs_plain_fp64": d1,
@@ -381,9 +394,9 @@ def bench_surrogate(args, ctx, spec, wl_name):
     # calibration and verification launch beside it, the rerun, and the optional
     # launch beside training): total algorithmic flops over total duration = the
     # average flops per launch over the average launch duration
-    per = F.per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive))
-    f32_acc = F.per_step_split(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive), True)
-    renorms = F.mgs(desc.dim, desc.dim) * (desc.n_renorm_transient + desc.n_renorm_average)
+    acc_fl, rej_fl, f32_acc, n_vec = F.desc_step(desc)
+    per = (acc_fl, rej_fl)
+    renorms = F.mgs(desc.dim, n_vec) * (desc.n_renorm_transient + desc.n_renorm_average)
     acc = st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"]
     rej = st["rerun_rej_steps"] + st["first_rej_steps"] + st["overlap_rej_steps"]
     fl = per[0] * acc + per[1] * rej + renorms * (st["rerun_points"] + st["first_points"] + st["overlap_points"])
@@ -397,7 +410,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
     dl_c = np.abs(res.lam[computed, 0] - lam_p[computed, 0])
     rec["config"] = {
-        "workload": wl_name, "points": n, "chosen_form": "mixed + surrogate refinement",
+        "workload": wl_name, "points": n, "chosen_form": chosen_form_name(desc) + " + surrogate refinement",
         "plain_fp64_points_per_s": n / (plain_ms * 1e-3),
         "speedup_vs_plain_fp64": plain_ms / ms_per_step,
         "mixed_only_points_per_s": n / (mixed_ms * 1e-3),
@@ -692,6 +705,9 @@ def main():
                     help="single-grid workloads: fixed launch shapes instead of the autotuner (profiling)")
     ap.add_argument("--no-plain", action="store_true",
                     help="single-grid workloads: skip the plain FP64 sweep and comparisons (profiling)")
+    ap.add_argument("--full-vectors", action="store_true",
+                    help="3-D flows: the chosen form integrates all N deviation vectors instead of "
+                         "N - 1 and the trace closure")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: every point is independent and this build runs on a "
@@ -716,8 +732,10 @@ def main():
         # three steps: single steps vary by up to ~0.6 s (host stalls, clocks)
         args.steps = 3 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
-        rec = bench_surrogate(args, ctx, W.rossler_dverk(2048, precision=L.PREC_MIXED),
-                              "rossler-2048x2048-dverk65-adaptive-surrogate")
+        spec = W.rossler_dverk(2048, precision=L.PREC_MIXED)
+        if not args.full_vectors:
+            spec = dict(spec, method=L.METHOD_VAR_TRACE)
+        rec = bench_surrogate(args, ctx, spec, "rossler-2048x2048-dverk65-adaptive-surrogate")
     else:
         if args.workload != "batch-mix":
             args.steps = 3 if args.steps is None else args.steps
@@ -736,11 +754,14 @@ def main():
             spec, name = None, f"batch-mix-{args.batch_points}-dopri5-adaptive-1e-9"
         else:
             raise SystemExit(f"unknown workload {args.workload}")
+        if spec is not None and spec["method"] == L.METHOD_VAR and spec["dim"] == 3 \
+                and not args.full_vectors:
+            spec = dict(spec, method=L.METHOD_VAR_TRACE)
         if spec is None:
             args.steps = 1 if args.steps is None else args.steps
             args.warmup = 0 if args.warmup is None else args.warmup
             rec = bench_batch(args, ctx, args.batch_points, name)
-        elif spec["method"] != L.METHOD_VAR:
+        elif spec["method"] not in (L.METHOD_VAR, L.METHOD_VAR_TRACE):
             rec = bench_reference_mode(args, ctx, spec, name)
         else:
             rec = bench_sweep(args, ctx, spec, name)

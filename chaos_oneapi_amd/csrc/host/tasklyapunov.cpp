@@ -74,9 +74,10 @@ int TaskLyapunov::init(std::shared_ptr<Parameters> paramsBase, const std::string
     d.n_exps = P.lyapunov_exponent_num;
     d.n_params = mModelPtr->_param_number;
     d.method = P.method;
-    d.integrator = P.method == LYAP_METHOD_VAR ? P.integrator : LYAP_INT_DVERK;
-    d.adaptive = P.method == LYAP_METHOD_VAR ? (P.adaptive ? 1 : 0) : 0;
-    d.precision = P.method == LYAP_METHOD_VAR ? P.precision : LYAP_PREC_FP64;
+    const bool var = P.method == LYAP_METHOD_VAR || P.method == LYAP_METHOD_VAR_TRACE;
+    d.integrator = var ? P.integrator : LYAP_INT_DVERK;
+    d.adaptive = var ? (P.adaptive ? 1 : 0) : 0;
+    d.precision = var ? P.precision : LYAP_PREC_FP64;
     d.h = P.integration_step;
     d.rtol = P.rtol;
     d.atol = P.atol;

@@ -19,6 +19,7 @@ int32_t method_from_name(const std::string& s) {
     if (s == "var_ref") return LYAP_METHOD_VAR_REF;
     if (s == "qr_ref") return LYAP_METHOD_QR_REF;
     if (s == "fd_ref") return LYAP_METHOD_FD_REF;
+    if (s == "var_trace") return LYAP_METHOD_VAR_TRACE;
     throw std::invalid_argument("unknown method '" + s + "'");
 }
 
@@ -27,6 +28,7 @@ const char* method_name(int32_t m) {
         case LYAP_METHOD_VAR: return "var";
         case LYAP_METHOD_VAR_REF: return "var_ref";
         case LYAP_METHOD_QR_REF: return "qr_ref";
+        case LYAP_METHOD_VAR_TRACE: return "var_trace";
         default: return "fd_ref";
     }
 }
@@ -60,7 +62,7 @@ void LyapunovParameters::Print() const {
     std::cout << "Using QR: " << std::boolalpha << is_qr << std::endl;
     std::cout << "Using VAR: " << std::boolalpha << is_var << std::endl;
     std::cout << "Method: " << method_name(method)
-              << (method == LYAP_METHOD_VAR
+              << ((method == LYAP_METHOD_VAR || method == LYAP_METHOD_VAR_TRACE)
                       ? (std::string(" (") + (integrator == LYAP_INT_DVERK ? "dverk" : "dopri5") +
                          (adaptive ? ", adaptive" : ", fixed step") +
                          (precision == LYAP_PREC_MIXED ? ", mixed)" : ", fp64)"))

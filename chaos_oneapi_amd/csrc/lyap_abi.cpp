@@ -229,8 +229,16 @@ int lyap_desc_finalize(lyap_sweep_desc* d) {
         if (d->method == LYAP_METHOD_QR_REF)
             return fail(LYAP_ERR_UNSUPPORTED, "the reference QR mode has no map branch");
     }
+    if (d->method == LYAP_METHOD_VAR_TRACE) {
+        if (!lane_model(d->model) || map_model(d->model))
+            return fail(LYAP_ERR_UNSUPPORTED, "the trace closure runs the 3-D flows only (model %d)",
+                        d->model);
+        if (d->n_exps != d->dim)
+            return fail(LYAP_ERR_INVALID, "the trace closure needs n_exps == dimension");
+    }
     switch (d->method) {
-        case LYAP_METHOD_VAR: {
+        case LYAP_METHOD_VAR:
+        case LYAP_METHOD_VAR_TRACE: {
             if (!(d->t_renorm > 0.0))
                 return fail(LYAP_ERR_INVALID, "skip_time_normalization must be > 0");
             d->n_renorm_transient = llround((d->t_transient + d->t_slave) / d->t_renorm);

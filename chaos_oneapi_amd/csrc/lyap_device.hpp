@@ -341,6 +341,10 @@ struct Shimizu {  // reference cuda/models/flows/shimizu.dp.cpp:15-27; (alpha, l
         const double m = r0 > r1 ? r0 : r1;
         return m > r2 ? m : r2;
     }
+    // tr J = c(p) + g(x) (the divergence of f): -alpha - lambda
+    static constexpr bool kTraceVar = false;
+    LYAP_D static double trace_const(const double* p) { return -(p[0] + p[1]); }
+    LYAP_D static double trace_var(const double*) { return 0.0; }
 };
 
 struct Lorenz {  // (sigma, rho, beta)
@@ -372,6 +376,10 @@ struct Lorenz {  // (sigma, rho, beta)
         const double m = r0 > r1 ? r0 : r1;
         return m > r2 ? m : r2;
     }
+    // tr J = -sigma - 1 - beta
+    static constexpr bool kTraceVar = false;
+    LYAP_D static double trace_const(const double* p) { return -((p[0] + 1.0) + p[2]); }
+    LYAP_D static double trace_var(const double*) { return 0.0; }
 };
 
 struct Rossler {  // (a, c, b)
@@ -403,6 +411,10 @@ struct Rossler {  // (a, c, b)
         const double m = r0 > r1 ? r0 : r1;
         return m > r2 ? m : r2;
     }
+    // tr J = (a - c) + x
+    static constexpr bool kTraceVar = true;
+    LYAP_D static double trace_const(const double* p) { return p[0] - p[1]; }
+    LYAP_D static double trace_var(const double* x) { return x[0]; }
 };
 
 // Map-type models (ModelType::map, model_factory.h:22-26): rhs() is the next

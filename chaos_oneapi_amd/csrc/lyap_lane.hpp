@@ -11,15 +11,20 @@ namespace lyap {
 // ---------------------------------------------------------------------------
 // base RK step: x -> xn; stage states X[s] (for the coupled tangent); error
 // vector when ERR.  With FSAL (Dopri5 + ERR) k1 comes in from the previous
-// accepted step and the new k1 (= f(xn)) goes out.
+// accepted step and the new k1 (= f(xn)) goes out.  With TRACE, *tr_step is
+// the step's integral of tr J by the solution's quadrature:
+// h * (c + sum_j b_j g(X_j)) for tr J = c(p) + g(x) (h * c when g is absent).
 // ---------------------------------------------------------------------------
-template <class M, int INTEG, bool ERR, class TX>
+template <class M, int INTEG, bool ERR, class TX, bool TRACE = false>
 LYAP_D void base_step(const double* x, const double* p, double h, double* xn, double* ev,
-                      typename M::template J<TX> (&X)[Tab<INTEG>::S], double* k1_io) {
+                      typename M::template J<TX> (&X)[Tab<INTEG>::S], double* k1_io,
+                      double* tr_step = nullptr) {
     using TB = Tab<INTEG>;
     constexpr int S = TB::S, N = M::N;
     constexpr bool FSAL = (INTEG == LYAP_INT_DOPRI5) && ERR;
     double k[S][N];
+    double trq = 0.0;
+    bool tr_any = false;
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         if (!ERR && !TB::for_solution(s)) {
@@ -34,6 +39,13 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
             bool any;
             const double acc = coef_sum_a(s, row, k, i, any);
             arg[i] = any ? fma(h, acc, x[i]) : x[i];
+        }
+        if constexpr (TRACE && M::kTraceVar) {
+            if (TB::b(s) != 0.0) {
+                const double bs = sconst(TB::b(s));
+                trq = tr_any ? fma(bs, M::trace_var(arg), trq) : bs * M::trace_var(arg);
+                tr_any = true;
+            }
         }
         if (ERR || TB::for_solution(s)) X[s] = M::template jac<TX>(arg, p);
         if (s == 0 && FSAL) {
@@ -57,6 +69,7 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
 #pragma unroll
         for (int i = 0; i < N; ++i) k1_io[i] = k[S - 1][i];
     }
+    if constexpr (TRACE) *tr_step = M::kTraceVar ? h * (M::trace_const(p) + trq) : h * M::trace_const(p);
 }
 
 // coupled tangent step for one deviation vector v (FP64 storage).
@@ -185,9 +198,11 @@ LYAP_D int64_t grab_items(bool want, unsigned long long* counter) {
 // the IO record to read the per-point set-up and output fields from (opaque
 // pointers, so those fields are not kept live in SGPRs across the step loop);
 // io supplies only the fields every iteration reads (counter, count, gather).
-template <class M, int INTEG, bool ADAPT, bool MIXED, class GetDesc, class GetIO>
+template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false, class GetDesc, class GetIO>
 LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io, GetIO getio) {
-    constexpr int N = M::N, P = M::P, K = M::N, S = Tab<INTEG>::S;
+    // TRACE: N - 1 deviation vectors; the last exponent is the averaged
+    // integral of tr J (Liouville) minus the others
+    constexpr int N = M::N, P = M::P, K = TRACE ? M::N - 1 : M::N, S = Tab<INTEG>::S;
     using TX = typename std::conditional<MIXED, float, double>::type;
     // scalars the step loop reads; everything else of the descriptor is
     // re-read through an opaque pointer when a lane takes a new point, so the
@@ -207,6 +222,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
     double x[N], V[K][N], sum[K], comp[K], lmant[K], p[P], k1[N];
     int32_t lex[K];
     int lcnt = 0;  // renormalisations in the open log window
+    double tri = 0.0, trs = 0.0, trc = 0.0;  // TRACE: tr J integral (interval, Kahan sum)
     float pf[P];
     double t = 0.0, h = 0.0;
     double t_next = 0.0;  // (m + 1) * t_renorm: the next renormalisation time
@@ -228,11 +244,15 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                                    : (double)dp->n_renorm_average * (double)dp->steps_per_renorm * dp->h;
         const int n_exps = dp->n_exps;
         double* const lam = iop->lambda;
+        double closing = trs;
+#pragma unroll
+        for (int q = 0; q < K; ++q) closing = closing - sum[q];
         for (int r = 0; r < n_exps; ++r) {
             double s = 0.0;
 #pragma unroll
             for (int q = 0; q < K; ++q)
                 if (q == r) s = sum[q];
+            if (TRACE && r == K) s = closing;
             lam[item * n_exps + r] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
         }
         if (iop->status) iop->status[item] = st;
@@ -265,6 +285,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                     for (int i = 0; i < N; ++i) V[r][i] = (r == i) ? 1.0 : 0.0;
                 }
                 lcnt = 0;
+                tri = trs = trc = 0.0;
                 t = 0.0;
                 h = dp->h;
                 m = 0;
@@ -298,7 +319,9 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                                 for (int r = 0; r < K; ++r) lognorm_flush(lmant[r], lex[r], sum[r], comp[r]);
                                 lcnt = 0;
                             }
+                            if constexpr (TRACE) kahan_add(trs, trc, tri);
                         }
+                        if constexpr (TRACE) tri = 0.0;
                         pending = false;
                         if (m == m_total) {
                             write_out(LYAP_PT_OK);
@@ -339,10 +362,10 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                     st = LYAP_PT_STEP_UNDERFLOW;
                     finished = true;
                 } else {
-                    double ev[N], k1_next[N];
+                    double ev[N], k1_next[N], tr_step = 0.0;
 #pragma unroll
                     for (int i = 0; i < N; ++i) k1_next[i] = k1[i];
-                    base_step<M, INTEG, true, TX>(x, p, h_try, xn, ev, X, k1_next);
+                    base_step<M, INTEG, true, TX, TRACE>(x, p, h_try, xn, ev, X, k1_next, &tr_step);
                     const double e2 = err2_seq<N>(ev, x, xn, rtol, atol);
                     if (!(e2 < kInf)) {
                         st = LYAP_PT_NONFINITE;
@@ -359,6 +382,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
 #pragma unroll
                             for (int r = 0; r < K; ++r)
                                 tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_try, V[r]);
+                            if constexpr (TRACE) tri = tri + tr_step;
 #pragma unroll
                             for (int i = 0; i < N; ++i) {
                                 x[i] = xn[i];
@@ -377,9 +401,11 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 }
             }
         } else {
-            base_step<M, INTEG, false, TX>(x, p, h_fixed, xn, nullptr, X, nullptr);
+            double tr_step = 0.0;
+            base_step<M, INTEG, false, TX, TRACE>(x, p, h_fixed, xn, nullptr, X, nullptr, &tr_step);
 #pragma unroll
             for (int r = 0; r < K; ++r) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_fixed, V[r]);
+            if constexpr (TRACE) tri = tri + tr_step;
 #pragma unroll
             for (int i = 0; i < N; ++i) x[i] = xn[i];
             ++nacc;

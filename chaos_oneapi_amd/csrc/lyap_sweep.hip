@@ -10,7 +10,9 @@
 // Kernels:
 //   k_var_lane   coupled variational RK (Jacobian at every stage state), MGS
 //                on exact time boundaries, FP64 or FP32-stage tangent,
-//                DVERK 6(5) / Dopri5, fixed or adaptive   (the sweep's method)
+//                DVERK 6(5) / Dopri5, fixed or adaptive   (the sweep's method);
+//                with TRACE N - 1 vectors and the last exponent from the
+//                integrated Jacobian trace
 //   k_varref_lane  reference VAR mode     cuda/lyapunov.dp.cpp:133-226
 //   k_qrref_lane   reference QR mode      cuda/lyapunov.dp.cpp:229-259
 //   k_fdref_lane   reference FD mode      cuda/lyapunov.dp.cpp:5-129
@@ -23,9 +25,10 @@ namespace lyap {
 // ---------------------------------------------------------------------------
 // the sweep's method: one lane per point, persistent
 // ---------------------------------------------------------------------------
-template <class M, int INTEG, bool ADAPT, bool MIXED>
+// TRACE: N - 1 deviation vectors, the last exponent from the integrated trace
+template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false>
 __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const SweepIO io) {
-    lane_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
+    lane_drain<M, INTEG, ADAPT, MIXED, TRACE>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
 }
 
 // the same method with four lanes per point (latency-bound launches)
@@ -535,6 +538,26 @@ static hipError_t dispatch_model(const lyap_sweep_desc& d, const SweepIO& io, hi
     case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):                                        \
         return quad ? launch_persistent(k_var_quad<M, I, A, X>, d, io, s, num_cus, max_blocks, 4) \
                     : launch_persistent(k_var_lane<M, I, A, X>, d, io, s, num_cus, max_blocks);
+            LYAP_CASE(LYAP_INT_DVERK, false, false)
+            LYAP_CASE(LYAP_INT_DVERK, false, true)
+            LYAP_CASE(LYAP_INT_DVERK, true, false)
+            LYAP_CASE(LYAP_INT_DVERK, true, true)
+            LYAP_CASE(LYAP_INT_DOPRI5, false, false)
+            LYAP_CASE(LYAP_INT_DOPRI5, false, true)
+            LYAP_CASE(LYAP_INT_DOPRI5, true, false)
+            LYAP_CASE(LYAP_INT_DOPRI5, true, true)
+#undef LYAP_CASE
+            default:
+                return hipErrorInvalidValue;
+        }
+    }
+    if (d.method == LYAP_METHOD_VAR_TRACE) {
+        // one lane per point (the four-lane form gives each vector a lane)
+        const int key = (d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0);
+        switch (key) {
+#define LYAP_CASE(I, A, X)                               \
+    case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):     \
+        return launch_persistent(k_var_lane<M, I, A, X, true>, d, io, s, num_cus, max_blocks);
             LYAP_CASE(LYAP_INT_DVERK, false, false)
             LYAP_CASE(LYAP_INT_DVERK, false, true)
             LYAP_CASE(LYAP_INT_DVERK, true, false)

@@ -61,9 +61,19 @@ def _stages(integ: int, adaptive: bool, for_tangent: bool):
     return out
 
 
-def _step_parts(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool):
+# the step's integral of tr J (the trace closure): h * (c + sum_j b_j g(X_j))
+# for Rossler (g = x), h * c for the flows with a constant trace; plus the
+# add into the interval's sum
+def _trace_cost(model: int, integ: int) -> int:
+    nnz_b = DVERK_NNZ_B if integ == L.INT_DVERK else DOPRI_NNZ_B
+    return (2 * nnz_b - 1) + 3 if model == L.MODEL_ROSSLER else 2
+
+
+def _step_parts(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool,
+                trace: bool = False):
     """(base step, binary32 tangent, binary64 tangent) flops of an accepted
-    step of one point; a rejected step costs the base step alone."""
+    step of one point; a rejected step costs the base step alone.  trace: the
+    base step also integrates tr J (k is then the number of vectors, N - 1)."""
     f_cost, jv_cost = MODEL_COST[model](n)
     nnz_b = DVERK_NNZ_B if integ == L.INT_DVERK else DOPRI_NNZ_B
     nnz_e = DVERK_NNZ_E if integ == L.INT_DVERK else DOPRI_NNZ_E
@@ -73,6 +83,8 @@ def _step_parts(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: b
     base = n_f * f_cost + n * sum(2 * z + 1 for z in base_st if z > 0) + n * (2 * nnz_b + 1)
     if adaptive:
         base += n * 2 * nnz_e + 7 * n + 2
+    if trace:
+        base += _trace_cost(model, integ)
     tan_st = _stages(integ, adaptive, for_tangent=True)
     if mixed and model == L.MODEL_L96:
         vec32 = len(tan_st) * jv_cost + n * sum(2 * z for z in tan_st if z > 0) + n * (2 * nnz_b - 1)
@@ -83,20 +95,32 @@ def _step_parts(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: b
     return base, 0, k * tangent
 
 
-def per_step(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool = False):
+def per_step(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool = False,
+             trace: bool = False):
     """(flops of an accepted step, flops of a rejected step) for one point."""
-    base, t32, t64 = _step_parts(model, n, k, integ, adaptive, mixed)
+    base, t32, t64 = _step_parts(model, n, k, integ, adaptive, mixed, trace)
     return base + t32 + t64, base
+
+
+def desc_step(desc):
+    """(accepted-step flops, rejected-step flops, binary32 flops of an accepted
+    step, deviation vectors) of a finalized descriptor's method."""
+    trace = desc.method == L.METHOD_VAR_TRACE
+    k = desc.dim - 1 if trace else desc.dim
+    mixed = desc.precision == L.PREC_MIXED
+    base, t32, t64 = _step_parts(desc.model, desc.dim, k, desc.integrator, bool(desc.adaptive), mixed, trace)
+    return base + t32 + t64, base, t32, k
 
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
 
 
-def per_step_split(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool):
+def per_step_split(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: bool,
+                   trace: bool = False):
     """Binary32 flops of an accepted step (the rest of per_step's accepted
     count is binary64; a rejected step is all binary64)."""
-    return _step_parts(model, n, k, integ, adaptive, mixed)[1]
+    return _step_parts(model, n, k, integ, adaptive, mixed, trace)[1]
 
 
 def weighted_peak(f64: float, f32: float) -> float:
@@ -112,8 +136,7 @@ def sweep_flops_split(desc, n_acc, n_rej):
     import numpy as np
 
     total = sweep_flops(desc, n_acc, n_rej)
-    f32_step = per_step_split(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive),
-                              desc.precision == L.PREC_MIXED)
+    f32_step = desc_step(desc)[2]
     f32 = float(f32_step * np.asarray(n_acc, dtype=np.float64).sum())
     return total - f32, f32
 
@@ -126,12 +149,11 @@ def sweep_flops(desc, n_acc, n_rej) -> float:
     """Total algorithmic flops of a sweep from its per-point step counters."""
     import numpy as np
 
-    acc, rej = per_step(desc.model, desc.dim, desc.dim, desc.integrator, bool(desc.adaptive),
-                        desc.precision == L.PREC_MIXED)
+    acc, rej, _, k = desc_step(desc)
     renorms = desc.n_renorm_transient + desc.n_renorm_average
     na = np.asarray(n_acc, dtype=np.float64)
     nr = np.asarray(n_rej, dtype=np.float64)
-    return float(acc * na.sum() + rej * nr.sum() + mgs(desc.dim, desc.dim) * renorms * len(na))
+    return float(acc * na.sum() + rej * nr.sum() + mgs(desc.dim, k) * renorms * len(na))
 
 
 def ref_fd_point_flops(desc) -> float:

@@ -20,7 +20,7 @@ SUCCESS, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOT_FOUND = 0, -1, -2, -3, -
 MODEL_SHIMIZU, MODEL_LORENZ, MODEL_ROSSLER, MODEL_L96 = 0, 1, 2, 3
 MODEL_HENON, MODEL_LOGISTIC = 4, 5
 TYPE_FLOW, TYPE_MAP = 0, 1
-METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF = 0, 1, 2, 3
+METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF, METHOD_VAR_TRACE = 0, 1, 2, 3, 4
 INT_DVERK, INT_DOPRI5 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 INIT_NONE, INIT_UNIFORM, INIT_NORMAL = 0, 1, 2

@@ -366,6 +366,9 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         with torch.cuda.stream(side_j):
             pdesc = L.SweepDesc.from_buffer_copy(fd)
             pdesc.precision = L.PREC_FP64
+            # the judge is the plain method: FP64, every deviation vector
+            if pdesc.method == L.METHOD_VAR_TRACE:
+                pdesc.method = L.METHOD_VAR
             lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
 
     # 1b. while the surrogate trains, a seeded subset of the other points

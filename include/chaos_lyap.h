@@ -59,7 +59,14 @@ enum {
     LYAP_METHOD_VAR = 0,     /* coupled variational RK, MGS on exact time boundaries */
     LYAP_METHOD_VAR_REF = 1, /* reference VAR mode, cuda/lyapunov.dp.cpp:133-226 */
     LYAP_METHOD_QR_REF = 2,  /* reference QR mode,  cuda/lyapunov.dp.cpp:229-259 */
-    LYAP_METHOD_FD_REF = 3   /* reference FD mode,  cuda/lyapunov.dp.cpp:5-129 */
+    LYAP_METHOD_FD_REF = 3,  /* reference FD mode,  cuda/lyapunov.dp.cpp:5-129 */
+    /* LYAP_METHOD_VAR with N - 1 deviation vectors; the last exponent closes
+       the spectrum through Liouville's formula, sum_i lambda_i = <tr J>, the
+       trace integrated along the base trajectory by the step's own RK
+       quadrature.  The 3-D flows only, n_exps == dimension; the first N - 1
+       exponents are those of LYAP_METHOD_VAR bit for bit.  No counterpart in
+       the reference (which integrates every vector). */
+    LYAP_METHOD_VAR_TRACE = 4
 };
 enum { LYAP_INT_DVERK = 0, LYAP_INT_DOPRI5 = 1 };
 enum { LYAP_PREC_FP64 = 0, LYAP_PREC_MIXED = 1 };

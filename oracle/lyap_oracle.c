@@ -245,6 +245,24 @@ double orc_jac_norm(int model, int n, const double* x, const double* p) {
     return m > r2 ? m : r2;
 }
 
+/* tr J = c(p) + g(x) of the 3-D flows (the divergence of f): shimizu
+   -alpha - lambda, Lorenz -sigma - 1 - beta, Rossler (a - c) + x */
+static double trace_const(int model, const double* p) {
+    switch (model) {
+    case ORC_MODEL_SHIMIZU: return -(p[0] + p[1]);
+    case ORC_MODEL_LORENZ: return -((p[0] + 1.0) + p[2]);
+    case ORC_MODEL_ROSSLER: return p[0] - p[1];
+    default: return NAN;
+    }
+}
+static int trace_has_var(int model) { return model == ORC_MODEL_ROSSLER; }
+static double trace_var(int model, const double* x) { return model == ORC_MODEL_ROSSLER ? x[0] : 0.0; }
+
+double orc_trace(int model, int n, const double* x, const double* p) {
+    (void)n;
+    return trace_has_var(model) ? trace_const(model, p) + trace_var(model, x) : trace_const(model, p);
+}
+
 /* the same products with binary32 arithmetic (the mixed form): Jacobian
    entries that are differences of state and parameter (rho - z, x - c,
    1 - z, x_{i+1} - x_{i-2}) are formed in FP64 from the FP64 stage state and
@@ -381,6 +399,23 @@ static int n_stages(int integ) { return integ == ORC_INT_DVERK ? 8 : 7; }
 static const double* tab_a(int integ, int s) { return integ == ORC_INT_DVERK ? DV_A[s] : DP_A[s]; }
 static const double* tab_b(int integ) { return integ == ORC_INT_DVERK ? DV_B : DP_B; }
 static const double* tab_e(int integ) { return integ == ORC_INT_DVERK ? DV_E : DP_E; }
+/* the step's integral of tr J by the solution's quadrature (the RK method on
+   the added equation y' = tr J(x)): h * (c + sum_j b_j g(X_j)) */
+double orc_trace_step(int model, int integ, int n, const double* stage_x, const double* p, double h) {
+    const double c = trace_const(model, p);
+    if (!trace_has_var(model)) return h * c;
+    const double* b = tab_b(integ);
+    double q = 0.0;
+    int first = 1;
+    for (int j = 0; j < n_stages(integ); ++j) {
+        if (b[j] == 0.0) continue;
+        const double g = trace_var(model, stage_x + j * n);
+        if (first) { q = b[j] * g; first = 0; }
+        else q = fma(b[j], g, q);
+    }
+    return h * (c + q);
+}
+
 /* stages the solution needs: DVERK stage 6 only feeds the error estimate,
    Dopri5 stage 7 only feeds the error estimate (and is the next k1). */
 static int stage_needed_for_solution(int integ, int s) {
@@ -922,7 +957,11 @@ static void lnacc_push(lnacc_t* a, int k, const double* nrm, int last) {
 /* added method: coupled variational RK, MGS on exact time boundaries */
 static void point_var(const orc_desc* d, const double* p, double* x, double* lam, uint32_t* status,
                       uint32_t* n_acc_out, uint32_t* n_rej_out) {
-    const int n = d->dim, k = d->n_exps, integ = d->integrator;
+    /* ORC_METHOD_VAR_TRACE: n - 1 vectors; the last exponent is the averaged
+       integral of tr J over the averaging phase minus the others (Liouville) */
+    const int trace = d->method == ORC_METHOD_VAR_TRACE;
+    const int n = d->dim, k = trace ? d->dim - 1 : d->n_exps, integ = d->integrator;
+    double tri = 0.0, trs = 0.0, trc = 0.0;
     const int tree = is_tree_model(d->model);
     const int mixed = d->precision == ORC_PREC_MIXED;
     const int64_t n_tr = llround((d->t_transient + d->t_slave) / d->t_renorm);
@@ -958,6 +997,7 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
                     else if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h, V + r * n);
                     else orc_tangent_step(d->model, integ, n, sx, p, h, V + r * n);
                 }
+                if (trace) tri = tri + orc_trace_step(d->model, integ, n, sx, p, h);
                 memcpy(x, xn, sizeof(double) * n);
                 ++n_acc;
             }
@@ -966,7 +1006,11 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if (st == ORC_OK && escaped(n, x, d->escape_radius)) st = ORC_ESCAPED;
             if (st != ORC_OK) break;
             ORC_RENORM();
-            if (m + 1 > n_tr) lnacc_push(&lna, k, nrm, m + 1 == m_total);
+            if (m + 1 > n_tr) {
+                lnacc_push(&lna, k, nrm, m + 1 == m_total);
+                if (trace) kahan_add(&trs, &trc, tri);
+            }
+            tri = 0.0;
         }
     } else {
         t_avg_exact = (double)n_av * d->t_renorm;
@@ -1012,6 +1056,7 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
                 else if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h_try, V + r * n);
                 else orc_tangent_step(d->model, integ, n, sx, p, h_try, V + r * n);
             }
+            if (trace) tri = tri + orc_trace_step(d->model, integ, n, sx, p, h_try);
             memcpy(x, xn, sizeof(double) * n);
             t = clipped ? t_next : t + h_try;
             ++n_acc;
@@ -1019,7 +1064,11 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if (clipped) {
                 ORC_RENORM();
                 ++m;
-                if (m > n_tr) lnacc_push(&lna, k, nrm, m == m_total);
+                if (m > n_tr) {
+                    lnacc_push(&lna, k, nrm, m == m_total);
+                    if (trace) kahan_add(&trs, &trc, tri);
+                }
+                tri = 0.0;
                 if (m == m_total) break;
                 t_next = (double)(m + 1) * d->t_renorm;
             }
@@ -1027,6 +1076,11 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
     }
 #undef ORC_RENORM
     for (int r = 0; r < k; ++r) lam[r] = (st == ORC_OK) ? lna.sum[r] / t_avg_exact : NAN;
+    if (trace) {
+        double closing = trs;
+        for (int r = 0; r < k; ++r) closing = closing - lna.sum[r];
+        lam[k] = (st == ORC_OK) ? closing / t_avg_exact : NAN;
+    }
     *status = st;
     *n_acc_out = (uint32_t)n_acc;
     *n_rej_out = (uint32_t)n_rej;
@@ -1321,6 +1375,7 @@ void orc_point(const orc_desc* d, int64_t idx, const double* params_in, const do
     }
     switch (d->method) {
     case ORC_METHOD_VAR:
+    case ORC_METHOD_VAR_TRACE:
         point_var(d, p, x, lam, status, n_acc, n_rej);
         return;
     case ORC_METHOD_VAR_REF: point_var_ref(d, p, x, lam, &steps); break;

@@ -20,7 +20,9 @@
  * RK with the Jacobian at every stage state, modified Gram-Schmidt on exact
  * time boundaries, adaptive DVERK/Dopri5, FP32 tangent stages with FP64
  * renormalisation, norm products over windows of 64 renormalisations
- * (ORC_LOG_WINDOW) and Kahan-compensated sums of the windows' logs).
+ * (ORC_LOG_WINDOW) and Kahan-compensated sums of the windows' logs; for the
+ * 3-D flows optionally N - 1 vectors with the last exponent from the
+ * integrated Jacobian trace, ORC_METHOD_VAR_TRACE).
  *
  * The descriptor below has the same memory layout as lyap_sweep_desc in
  * include/chaos_lyap.h so one Python dictionary can fill both.
@@ -49,7 +51,8 @@ double orc_ctrl_exp(double x);
    (log facOld, rejected flag) is updated in place; returns 1 on accept */
 int orc_ctrl_update(int integ, double err, double* log_fac_old, int* rejected, double h_try,
                     double h_max, double* h_next);
-enum { ORC_METHOD_VAR = 0, ORC_METHOD_VAR_REF = 1, ORC_METHOD_QR_REF = 2, ORC_METHOD_FD_REF = 3 };
+enum { ORC_METHOD_VAR = 0, ORC_METHOD_VAR_REF = 1, ORC_METHOD_QR_REF = 2, ORC_METHOD_FD_REF = 3,
+       ORC_METHOD_VAR_TRACE = 4 };
 enum { ORC_INT_DVERK = 0, ORC_INT_DOPRI5 = 1 };
 enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };
 enum { ORC_INIT_NONE = 0, ORC_INIT_UNIFORM = 1, ORC_INIT_NORMAL = 2 };
@@ -93,6 +96,13 @@ int64_t orc_loop_count(double t0, double t1, double h);
 void orc_rhs(int model, int n, const double* x, const double* p, double* out);
 void orc_jvp(int model, int n, const double* x, const double* p, const double* v, double* out);
 double orc_jac_norm(int model, int n, const double* x, const double* p);
+/* trace of the Jacobian (divergence of f) of a 3-D flow at x, and its
+   integral over one RK step from the step's stage states (stride n):
+   h * (c + sum_j b_j g(X_j)) with tr J = c(p) + g(x), first term a product,
+   later terms fma, stages in order (h * c for the flows whose trace is
+   constant) */
+double orc_trace(int model, int n, const double* x, const double* p);
+double orc_trace_step(int model, int integ, int n, const double* stage_x, const double* p, double h);
 void orc_jvp_f32(int model, int n, const double* x, const double* p, const float* v, float* out);
 
 /* one base RK step; err_vec (may be NULL) receives h*sum(e_j k_j); returns

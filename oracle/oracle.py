@@ -22,7 +22,7 @@ MAX_PARAMS = 8
 
 MODEL_SHIMIZU, MODEL_LORENZ, MODEL_ROSSLER, MODEL_L96 = 0, 1, 2, 3
 MODEL_HENON, MODEL_LOGISTIC = 4, 5
-METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF = 0, 1, 2, 3
+METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF, METHOD_VAR_TRACE = 0, 1, 2, 3, 4
 INT_DVERK, INT_DOPRI5 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 INIT_NONE, INIT_UNIFORM, INIT_NORMAL = 0, 1, 2
@@ -82,6 +82,10 @@ def lib():
                                       ctypes.POINTER(d)]
         L.orc_jac_norm.restype = d
         L.orc_jac_norm.argtypes = [i, i, P, P]
+        L.orc_trace.restype = d
+        L.orc_trace.argtypes = [i, i, P, P]
+        L.orc_trace_step.restype = d
+        L.orc_trace_step.argtypes = [i, i, i, P, P, d]
         L.orc_uniform.restype = d; L.orc_uniform.argtypes = [u64, i64, i64]
         L.orc_loop_count.restype = i64; L.orc_loop_count.argtypes = [d, d, d]
         L.orc_rhs.argtypes = [i, i, P, P, P]
@@ -165,6 +169,20 @@ def jac_norm(model, x, p):
     """max_i sum_j |J_ij(x)|: the adaptive step cap's norm."""
     x, p = f64(x), f64(np.pad(p, (0, MAX_PARAMS - len(p))))
     return lib().orc_jac_norm(model, len(x), _ptr(x), _ptr(p))
+
+
+def trace(model, x, p):
+    """tr J(x) of a 3-D flow (its divergence)."""
+    x, p = f64(x), f64(np.pad(p, (0, MAX_PARAMS - len(p))))
+    return lib().orc_trace(model, len(x), _ptr(x), _ptr(p))
+
+
+def trace_step(model, integ, stage_x, p, h):
+    """The step's integral of tr J from its stage states (rows of stage_x)."""
+    sx = f64(np.asarray(stage_x).reshape(-1))
+    p = f64(np.pad(p, (0, MAX_PARAMS - len(p))))
+    n = np.asarray(stage_x).shape[-1]
+    return lib().orc_trace_step(model, integ, n, _ptr(sx), _ptr(p), float(h))
 
 
 def rk_step(model, integ, x, p, h, want_err=True):

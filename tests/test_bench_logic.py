@@ -41,6 +41,22 @@ def test_l96_mixed_step_counts_the_folded_tangent():
         F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True)
 
 
+def test_trace_closure_step_counts_two_vectors_and_the_trace():
+    desc = L.finalize(L.make_desc(dict(W.rossler_dverk(4, precision=L.PREC_MIXED),
+                                       method=L.METHOD_VAR_TRACE)))
+    acc, rej, f32, k = F.desc_step(desc)
+    acc3, rej3 = F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, True)
+    assert k == 2
+    # DVERK: b has 6 nonzeros -> the quadrature 2*6 - 1, plus c, h and the sum
+    assert rej == rej3 + 14
+    tangent = (acc3 - rej3) // 3
+    assert acc == rej + 2 * tangent
+    assert f32 == 2 * (tangent - 3)
+    na, nr = np.full(4, 100), np.full(4, 2)
+    renorms = desc.n_renorm_transient + desc.n_renorm_average
+    assert F.sweep_flops(desc, na, nr) == acc * 400 + rej * 8 + F.mgs(3, 2) * renorms * 4
+
+
 def test_weighted_peak_is_the_harmonic_mix():
     assert F.weighted_peak(1.0, 0.0) == F.PEAK_FP64_TFLOPS
     assert abs(F.weighted_peak(0.0, 1.0) - F.PEAK_FP32_TFLOPS) < 1e-12

@@ -10,7 +10,11 @@ oracle in FP64 and in the oracle's mixed form:
   C4  the batched 40/30/20/8/2 % mix, adaptive Dopri5 rtol=1e-9, one persistent
       launch replayed from its hipGraph
   C5  the surrogate-refined Rossler sweep at full times (512x512 plane, 64x64
-      lattice), its FP64 judge and computed points against the oracle, its gates
+      lattice) in the chosen form (mixed, trace closure), its FP64 judge (all
+      three vectors) and computed points against the oracle, its gates
+
+C1 and C2 also run the trace closure (two deviation vectors, the last
+exponent from the integrated Jacobian trace).
 
 Each test prints the per-exponent max |dlambda_i| of the mixed form against
 FP64 on its sample (the error map's summary); the bar for those is the
@@ -73,14 +77,16 @@ def run_config(ctx, spec, k, seed, options=()):
     return d
 
 
-@pytest.mark.parametrize("quad", [0, 1])
-def test_c1_lorenz_64_dopri5_fixed(gpu_ctx, quad):
-    run_config(gpu_ctx, W.lorenz_fixed(64), 64, 101, (("lane_quad", quad),))
+@pytest.mark.parametrize("quad,method", [(0, L.METHOD_VAR), (1, L.METHOD_VAR), (0, L.METHOD_VAR_TRACE)])
+def test_c1_lorenz_64_dopri5_fixed(gpu_ctx, quad, method):
+    run_config(gpu_ctx, dict(W.lorenz_fixed(64), method=method), 64, 101, (("lane_quad", quad),))
 
 
-@pytest.mark.parametrize("gather", [0, 4])
-def test_c2_rossler_512_dverk_1e10(gpu_ctx, gather):
-    spec = W.rossler_dverk(512)
+@pytest.mark.parametrize("gather,method", [(0, L.METHOD_VAR), (4, L.METHOD_VAR), (4, L.METHOD_VAR_TRACE)])
+def test_c2_rossler_512_dverk_1e10(gpu_ctx, gather, method):
+    """C2 with the sweep's method and with its trace closure (two vectors, the
+    chosen form of the 3-D flows)."""
+    spec = dict(W.rossler_dverk(512), method=method)
     desc = L.finalize(L.make_desc(spec))
     assert desc.rtol == 1e-10 and desc.atol == 1e-10 and desc.max_steps == 2_000_000
     assert desc.n_renorm_transient == 400 and desc.n_renorm_average == 4000
@@ -126,7 +132,9 @@ def test_c4_batch_mix_full_settings(gpu_ctx, tuning):
 
 
 def test_c5_surrogate_refinement_full_times(gpu_ctx):
-    spec = W.rossler_dverk(512, precision=L.PREC_MIXED)
+    # the chosen form of the headline: mixed, two vectors and the trace closure;
+    # the judge is the plain FP64 method with all three vectors
+    spec = dict(W.rossler_dverk(512, precision=L.PREC_MIXED), method=L.METHOD_VAR_TRACE)
     print("case C5 refine rossler 512x512 full times, 64x64 lattice", flush=True)
     res = R.refine(gpu_ctx, spec, train_n=64, epochs=120, batch=1024)
     st = res.stats
@@ -142,6 +150,8 @@ def test_c5_surrogate_refinement_full_times(gpu_ctx):
                               (L.PREC_MIXED, res.verify_lam[pick], None)):
         od = O.make_desc(fd.to_dict())
         od.precision = prec
+        if prec == L.PREC_FP64:
+            od.method = O.METHOD_VAR
         lam_o, st_o, _, _ = O.sweep(od, idx_list=idx, n_threads=THREADS)
         np.testing.assert_array_equal(lam_g, lam_o)
         if st_g is not None:

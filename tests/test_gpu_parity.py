@@ -49,11 +49,14 @@ def assert_match(g, o, tol, exact):
         np.testing.assert_array_equal(lam_g[ok], lam_o[ok])
 
 
+@pytest.mark.parametrize("method", [L.METHOD_VAR, L.METHOD_VAR_TRACE])
 @pytest.mark.parametrize("integ", [L.INT_DVERK, L.INT_DOPRI5])
 @pytest.mark.parametrize("adaptive", [0, 1])
 @pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
 @pytest.mark.parametrize("model", ["lorenz", "rossler", "shimizu"])
-def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
+def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec, method):
+    """The sweep's method, and its trace closure (two deviation vectors, the
+    last exponent from the integrated Jacobian trace)."""
     if model == "lorenz":
         spec = W.lorenz_fixed(n=6, precision=prec, t_transient=5.0, t_average=20.0)
     elif model == "rossler":
@@ -63,7 +66,7 @@ def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
         spec.update(model=L.MODEL_SHIMIZU, n_params=2, base_params=[0.0, 0.0],
                     init_state=[0.1, 0.0, 0.0], init_scale=1e-3)
         spec.update(W.grid_fields(0, 0.2, 2.0, 6, 1, 0.0, 1.2, 6))
-    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9)
+    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9, method=method)
     g, o = run_both(gpu_ctx, spec)
     assert_match(g, o, TOL_FP64, exact=True)
 
@@ -96,14 +99,16 @@ def test_var_quad_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
 
 
 @pytest.mark.parametrize("gather", [2, 16, 64])
-@pytest.mark.parametrize("integ,adaptive,prec", [(L.INT_DVERK, 1, L.PREC_MIXED),
-                                                 (L.INT_DOPRI5, 1, L.PREC_FP64),
-                                                 (L.INT_DOPRI5, 0, L.PREC_MIXED)])
-def test_var_lane_gathered_renorm_matches_oracle(gpu_ctx, gather, integ, adaptive, prec):
+@pytest.mark.parametrize("integ,adaptive,prec,method", [(L.INT_DVERK, 1, L.PREC_MIXED, L.METHOD_VAR),
+                                                        (L.INT_DOPRI5, 1, L.PREC_FP64, L.METHOD_VAR),
+                                                        (L.INT_DOPRI5, 0, L.PREC_MIXED, L.METHOD_VAR),
+                                                        (L.INT_DVERK, 1, L.PREC_MIXED,
+                                                         L.METHOD_VAR_TRACE)])
+def test_var_lane_gathered_renorm_matches_oracle(gpu_ctx, gather, integ, adaptive, prec, method):
     """Lanes at a renormalisation wait for `gather` lanes of the wave, on a grid
     of 2 blocks so that lanes also refill: every point's bits are unchanged."""
     spec = W.rossler_dverk(n=24, precision=prec, t_transient=5.0, t_average=20.0)
-    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9)
+    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9, method=method)
     gpu_ctx.set_option("lane_gather", gather)
     gpu_ctx.set_option("lane_max_blocks", 2)
     try:

@@ -514,6 +514,89 @@ def test_step_cap_norm_is_the_jacobian_inf_norm(model, n):
         assert abs(O.jac_norm(model, x, p) - ref) <= 4e-16 * ref
 
 
+# ---------------------------------------------------------------- trace closure
+@pytest.mark.parametrize("model,n", [(O.MODEL_SHIMIZU, 3), (O.MODEL_LORENZ, 3), (O.MODEL_ROSSLER, 3)])
+def test_trace_is_the_jacobian_trace(model, n):
+    """orc_trace = the trace of J assembled from the Jacobian-vector product
+    (pinned against central differences of the right-hand side)."""
+    rng = np.random.default_rng(200 + model)
+    npar = {O.MODEL_SHIMIZU: 2, O.MODEL_LORENZ: 3, O.MODEL_ROSSLER: 3}[model]
+    for _ in range(50):
+        x = rng.normal(0, 5, n)
+        p = rng.uniform(-3, 30, npar)
+        J = np.stack([O.jvp(model, x, p, np.eye(n)[j]) for j in range(n)], axis=1)
+        assert abs(O.trace(model, x, p) - np.trace(J)) <= 1e-14 * (1 + np.abs(J).sum())
+
+
+@pytest.mark.parametrize("integ", [O.INT_DVERK, O.INT_DOPRI5])
+@pytest.mark.parametrize("model,p,x0", [(O.MODEL_ROSSLER, [0.2, 5.7, 0.2], [1.0, -2.0, 0.5]),
+                                         (O.MODEL_LORENZ, [10.0, 28.0, 8 / 3], [1.0, 1.0, 20.0])])
+def test_trace_step_integrates_the_divergence(integ, model, p, x0):
+    """The sum of orc_trace_step over 300 steps of h = 0.01 equals the
+    integral of tr J along the trajectory, computed by scipy's DOP853 on the
+    system augmented with y' = tr J(x) (a wrong weight, stage or factor h
+    would miss it by far more than the steps' own 1e-9)."""
+    x = np.array(x0, dtype=np.float64)
+    h, steps, acc = 0.01, 300, 0.0
+    for _ in range(steps):
+        xn, _, sx = O.rk_step(model, integ, x, p, h, want_err=True)
+        acc += O.trace_step(model, integ, sx, p, h)
+        x = xn
+
+    def aug(t, y):
+        return list(O.rhs(model, y[:3], p)) + [O.trace(model, y[:3], p)]
+
+    sol = scipy.integrate.solve_ivp(aug, (0, h * steps), list(x0) + [0.0], method="DOP853",
+                                    rtol=1e-13, atol=1e-14)
+    assert abs(acc - sol.y[3, -1]) < 1e-8 * max(1.0, abs(sol.y[3, -1]))
+
+
+def test_trace_closure_at_a_stable_fixed_point():
+    """Lorenz rho = 0.5 (stable origin, J constant along the converged
+    trajectory): the closed exponent is the third eigenvalue of J(0) (a
+    closed form), the first two are the vector method's bit for bit."""
+    ref = lorenz_fixed_point_spectrum(10, 0.5, 8 / 3)
+    kw = dict(start1=0.5, base_params=[10, 0.5, 8 / 3], t_transient=60, t_average=400,
+              init_state=[1, 1, 1])
+    lam_v, st_v, _, _ = O.sweep(lorenz_desc(method=O.METHOD_VAR, **kw), count=1)
+    lam_t, st_t, _, _ = O.sweep(lorenz_desc(method=O.METHOD_VAR_TRACE, **kw), count=1)
+    assert st_v[0] == O.OK and st_t[0] == O.OK
+    assert np.array_equal(lam_v[0][:2], lam_t[0][:2])
+    assert abs(lam_t[0][2] - ref[2]) < 1e-8
+    np.testing.assert_allclose(lam_t[0], ref, atol=1e-6)
+
+
+@pytest.mark.parametrize("prec", [O.PREC_FP64, O.PREC_MIXED])
+def test_trace_closure_on_chaotic_orbits(prec):
+    """Chaotic Lorenz and Rossler orbits: the closed spectrum sums to the
+    trace's time average (Lorenz: the constant -(sigma + 1 + beta)), its
+    first two exponents are the vector method's (same precision) bit for bit,
+    and at a small fixed step its last exponent agrees with the one the third
+    deviation vector measures in FP64.  In binary32 the third vector itself
+    loses the most contracting direction (off by ~3e-2 on this Lorenz orbit);
+    the closure is off only by the first two exponents' binary32 error."""
+    def pair(**kw):
+        v64 = O.sweep(lorenz_desc(method=O.METHOD_VAR, **dict(kw, precision=O.PREC_FP64)), count=1)
+        v = O.sweep(lorenz_desc(method=O.METHOD_VAR, **dict(kw, precision=prec)), count=1)
+        t = O.sweep(lorenz_desc(method=O.METHOD_VAR_TRACE, **dict(kw, precision=prec)), count=1)
+        assert v64[1][0] == O.OK and v[1][0] == O.OK and t[1][0] == O.OK
+        assert np.array_equal(v[0][0][:2], t[0][0][:2])
+        return v64[0][0], v[0][0], t[0][0]
+
+    tol = 1e-6 if prec == O.PREC_FP64 else 2e-5
+    lam64, lam_v, lam_t = pair(t_transient=50, t_average=400, h=0.002)
+    assert abs(lam_t.sum() + (11 + 8 / 3)) < 1e-9
+    assert abs(lam_t[2] - lam64[2]) < tol
+    if prec == O.PREC_MIXED:
+        assert abs(lam_v[2] - lam64[2]) > 1e-2  # the binary32 third vector
+    lam64, lam_v, lam_t = pair(model=O.MODEL_ROSSLER, base_params=[0.2, 5.7, 0.2], sweep_idx1=0,
+                               sweep_idx2=1, start1=0.2, start2=5.7, integrator=O.INT_DVERK, h=0.005,
+                               t_transient=100, t_average=400, t_renorm=0.5, init_state=[0, 0, 0],
+                               init_scale=1.0)
+    assert abs(lam_t[2] + 5.3943) < 0.05
+    assert abs(lam_t[2] - lam64[2]) < tol
+
+
 def test_status_words():
     # escaping orbit (Rossler with a large a), step underflow, step budget
     d = lorenz_desc(model=O.MODEL_ROSSLER, base_params=[0.45, 4.0, 0.2], sweep_idx1=0, sweep_idx2=1,
