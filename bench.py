@@ -182,16 +182,32 @@ def chosen_form_name(desc):
     return name
 
 
-def plain_fp64(ctx, desc):
-    """One plain FP64 sweep of the grid: (lam, status, kernel ms, flops)."""
-    import torch
-
-    from chaos_oneapi_amd import flops as F
+def plain_desc_of(desc):
+    """The plain method of a chosen form: FP64, every deviation vector."""
     from chaos_oneapi_amd import lyap as L
 
     plain = L.SweepDesc.from_buffer_copy(desc)
     plain.precision = L.PREC_FP64
     plain.method = L.METHOD_VAR if desc.method == L.METHOD_VAR_TRACE else desc.method
+    return plain
+
+
+def set_lane_shape(ctx, shape):
+    """(lane_max_blocks, lane_gather, lane_quad) on ctx."""
+    for key, v in zip(("lane_max_blocks", "lane_gather", "lane_quad"), shape):
+        ctx.set_option(key, int(v))
+
+
+def plain_fp64(ctx, desc, lane_shape=None):
+    """One plain FP64 sweep of the grid: (lam, status, kernel ms, flops);
+    lane_shape: the 3-D flows' launch shape to run it with (its own tuning)."""
+    import torch
+
+    from chaos_oneapi_amd import flops as F
+
+    plain = plain_desc_of(desc)
+    if lane_shape is not None:
+        set_lane_shape(ctx, lane_shape)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -258,7 +274,14 @@ def bench_sweep(args, ctx, spec, wl_name):
         rec["roofline"] = roofline(f64_c, f32_c, kern_avg, kname + " (mixed) over the grid", args.probe,
                                    measured_traffic(wl_name))
         return rec
-    lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
+    # the plain FP64 sweep with a launch shape tuned for it (its own method and
+    # precision), not the chosen form's
+    tuned_plain = None
+    if tuned_lanes is not None:
+        tuned_plain = A.tune_lanes(ctx, plain_desc_of(desc), cache=A.Cache(None), time_scale=0.1,
+                                   apply=False)
+    lam_p, st_p, plain_ms, flops_p = plain_fp64(
+        ctx, desc, None if tuned_plain is None else tuned_plain.best)
     ok = (st_c == 0) & (st_p == 0)
     d1 = float(np.max(np.abs(lam_c[ok, 0] - lam_p[ok, 0]))) if ok.any() else 0.0
     cls = float(np.mean((lam_c[ok, 0] > CHAOS_THRESHOLD) == (lam_p[ok, 0] > CHAOS_THRESHOLD)))
@@ -286,6 +309,9 @@ def bench_sweep(args, ctx, spec, wl_name):
                                        "lane_gather": int(tuned_lanes.best[1]),
                                        "lane_quad": int(tuned_lanes.best[2])}
         rec["config"]["lane_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned_lanes.timings.items()}
+        rec["config"]["plain_fp64_lane_shape"] = {"lane_max_blocks": int(tuned_plain.best[0]),
+                                                  "lane_gather": int(tuned_plain.best[1]),
+                                                  "lane_quad": int(tuned_plain.best[2])}
     if tuned is not None:
         rec["config"]["l96_tile"] = list(tuned.best)
         rec["config"]["l96_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}

@@ -308,7 +308,9 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
     if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     if (lane_model(fd.model)) {
         e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks,
-                                    c->lane_quad != 0 && fd.method == LYAP_METHOD_VAR && fd.dim == 3);
+                                    c->lane_quad != 0 &&
+                                        (fd.method == LYAP_METHOD_VAR || fd.method == LYAP_METHOD_VAR_TRACE) &&
+                                        fd.dim == 3);
     } else if (fd.model == LYAP_MODEL_L96) {
         if (fd.method != LYAP_METHOD_VAR)
             return fail(LYAP_ERR_UNSUPPORTED, "Lorenz-96 runs the coupled variational method only");

@@ -452,9 +452,11 @@ LYAP_D double quad_bcast(double v) {
 // quad_perm), every lane runs the same MGS on the full matrix and keeps its
 // own row; lane r accumulates exponent r.  The arithmetic of each point is
 // that of lane_drain, so both give the same bits.
-template <class M, int INTEG, bool ADAPT, bool MIXED, class GetDesc, class GetIO>
+template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false, class GetDesc, class GetIO>
 LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io, GetIO getio) {
-    constexpr int N = M::N, P = M::P, K = M::N, S = Tab<INTEG>::S;
+    // TRACE: N - 1 vectors (lanes 0 .. N - 2); every lane integrates tr J with
+    // the replicated base step, and lane N - 1 writes the closing exponent
+    constexpr int N = M::N, P = M::P, K = TRACE ? M::N - 1 : M::N, S = Tab<INTEG>::S;
     static_assert(K <= 4, "four lanes per point");
     using TX = typename std::conditional<MIXED, float, double>::type;
     const int64_t n_tr = d.n_renorm_transient;
@@ -470,6 +472,7 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
 
     double x[N], v[N], p[P], k1[N];
     double sum = 0.0, comp = 0.0, lmant = 1.0;
+    double tri = 0.0, trs = 0.0, trc = 0.0;  // TRACE: tr J integral (interval, Kahan sum)
     int32_t lex = 0;
     int lcnt = 0;
     float pf[P];
@@ -484,8 +487,17 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
         const auto iop = getio();
         const double t_avg = ADAPT ? (double)dp->n_renorm_average * dp->t_renorm
                                    : (double)dp->n_renorm_average * (double)dp->steps_per_renorm * dp->h;
+        double s = sum;
+        if constexpr (TRACE) {
+            // the closing exponent: the trace integral minus lanes 0 .. K-1's sums
+            double closing = trs;
+            closing = closing - quad_bcast<0>(sum);
+            if constexpr (K > 1) closing = closing - quad_bcast<1>(sum);
+            if constexpr (K > 2) closing = closing - quad_bcast<2>(sum);
+            if (ql == K) s = closing;
+        }
         if (ql < dp->n_exps)
-            iop->lambda[item * dp->n_exps + ql] = (st == LYAP_PT_OK) ? sum / t_avg : __builtin_nan("");
+            iop->lambda[item * dp->n_exps + ql] = (st == LYAP_PT_OK) ? s / t_avg : __builtin_nan("");
         if (ql == 0) {
             if (iop->status) iop->status[item] = st;
             if (iop->n_acc) iop->n_acc[item] = nacc;
@@ -513,6 +525,7 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                     v[i] = (i == ql) ? 1.0 : 0.0;
                 }
                 sum = comp = 0.0;
+                tri = trs = trc = 0.0;
                 lmant = 1.0;
                 lex = 0;
                 lcnt = 0;
@@ -561,7 +574,9 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                         lognorm_flush(lmant, lex, sum, comp);
                         lcnt = 0;
                     }
+                    if constexpr (TRACE) kahan_add(trs, trc, tri);
                 }
+                if constexpr (TRACE) tri = 0.0;
                 pending = false;
                 if (m == m_total) {
                     write_out(LYAP_PT_OK);
@@ -596,10 +611,10 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                     st = LYAP_PT_STEP_UNDERFLOW;
                     finished = true;
                 } else {
-                    double ev[N], k1_next[N];
+                    double ev[N], k1_next[N], tr_step = 0.0;
 #pragma unroll
                     for (int i = 0; i < N; ++i) k1_next[i] = k1[i];
-                    base_step<M, INTEG, true, TX>(x, p, h_try, xn, ev, X, k1_next);
+                    base_step<M, INTEG, true, TX, TRACE>(x, p, h_try, xn, ev, X, k1_next, &tr_step);
                     const double e2 = err2_seq<N>(ev, x, xn, rtol, atol);
                     if (!(e2 < kInf)) {
                         st = LYAP_PT_NONFINITE;
@@ -614,6 +629,7 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                             ++nrej;
                         } else {
                             if (ql < K) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_try, v);
+                            if constexpr (TRACE) tri = tri + tr_step;
 #pragma unroll
                             for (int i = 0; i < N; ++i) {
                                 x[i] = xn[i];
@@ -632,8 +648,10 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 }
             }
         } else {
-            base_step<M, INTEG, false, TX>(x, p, h_fixed, xn, nullptr, X, nullptr);
+            double tr_step = 0.0;
+            base_step<M, INTEG, false, TX, TRACE>(x, p, h_fixed, xn, nullptr, X, nullptr, &tr_step);
             if (ql < K) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_fixed, v);
+            if constexpr (TRACE) tri = tri + tr_step;
 #pragma unroll
             for (int i = 0; i < N; ++i) x[i] = xn[i];
             ++nacc;

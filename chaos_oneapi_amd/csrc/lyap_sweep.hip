@@ -32,9 +32,9 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
 }
 
 // the same method with four lanes per point (latency-bound launches)
-template <class M, int INTEG, bool ADAPT, bool MIXED>
+template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false>
 __global__ __launch_bounds__(256) void k_var_quad(const lyap_sweep_desc d, const SweepIO io) {
-    quad_drain<M, INTEG, ADAPT, MIXED>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
+    quad_drain<M, INTEG, ADAPT, MIXED, TRACE>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
 }
 
 // ---------------------------------------------------------------------------
@@ -552,12 +552,12 @@ static hipError_t dispatch_model(const lyap_sweep_desc& d, const SweepIO& io, hi
         }
     }
     if (d.method == LYAP_METHOD_VAR_TRACE) {
-        // one lane per point (the four-lane form gives each vector a lane)
         const int key = (d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0);
         switch (key) {
-#define LYAP_CASE(I, A, X)                               \
-    case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):     \
-        return launch_persistent(k_var_lane<M, I, A, X, true>, d, io, s, num_cus, max_blocks);
+#define LYAP_CASE(I, A, X)                                                                          \
+    case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):                                                \
+        return quad ? launch_persistent(k_var_quad<M, I, A, X, true>, d, io, s, num_cus, max_blocks, 4) \
+                    : launch_persistent(k_var_lane<M, I, A, X, true>, d, io, s, num_cus, max_blocks);
             LYAP_CASE(LYAP_INT_DVERK, false, false)
             LYAP_CASE(LYAP_INT_DVERK, false, true)
             LYAP_CASE(LYAP_INT_DVERK, true, false)

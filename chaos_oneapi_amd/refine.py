@@ -155,15 +155,29 @@ def answer_errors(pred, lam, status, answered):
     return np.abs(pred[both, 0].astype(np.float64) - lam[both, 0])
 
 
+# a rule that answers lambda values needs at least this many valued answers on
+# the calibration subsample (all within dlam_cal): a handful passing is no
+# evidence that its values hold elsewhere (on the 2048^2 Rossler plane a rule
+# with a few valued calibration answers missed the gate at 37 of its 50 valued
+# answers over the plane)
+MIN_VALUED_CALIBRATION = 1000
+
+
 def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target,
-                    dlam_cal=0.5 * DLAMBDA1_GATE):
+                    dlam_cal=0.5 * DLAMBDA1_GATE, min_valued=MIN_VALUED_CALIBRATION,
+                    plane_valued=None):
     """The answer rule that answers the most calibration points while (1) the
     final map's class agreement on the calibration subsample stays at or above
     `target` (the method's own points agree by construction, so this bounds
-    the wrong answers: wrong <= (1 - target) * n) and (2) every lambda1 it
+    the wrong answers: wrong <= (1 - target) * n), (2) every lambda1 it
     answers there is within dlam_cal of the method's value (half the 1e-3
     accuracy gate, so the held-out verification subsample, which sees other
-    points of the same error distribution, keeps a margin).  Rules: kappa in
+    points of the same error distribution, keeps a margin) and (3) it answers
+    lambda values at no calibration point or at min_valued of them at least
+    (the evidence that (2) holds beyond the points seen); plane_valued(kappa,
+    escapes), when given, counts the points of the whole plane at which the
+    rule would answer a value, and a rule with no valued calibration answer
+    must answer no value on the plane either.  Rules: kappa in
     KAPPAS, with or without the escape answers (which kappa does not scale).
     Returns (kappa, escapes, agreement, answered, table); kappa inf and no
     escapes: the surrogate answers nothing.  pred [n,5] numpy, lam/status the
@@ -189,6 +203,14 @@ def calibrate_kappa(pred, lam, status, std_thr, lam_thr, bad_lo, bad_hi, target,
             dmax = float(err.max()) if err.size else 0.0
             table.append((kappa, escapes, n_a, wrong, dmax))
             if n_a > 0 and wrong <= budget and dmax <= dlam_cal and n_a > best[3]:
+                if err.size >= min_valued:
+                    evidence = True
+                elif err.size == 0:
+                    evidence = plane_valued is None or plane_valued(kappa, escapes) == 0
+                else:
+                    evidence = False
+                if not evidence:
+                    continue
                 best = (kappa, escapes, 1.0 - wrong / n, n_a)
     return best + (table,)
 
@@ -430,11 +452,16 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     #    surrogate does not answer and that step 1 did not compute
     if side_c is not None:
         main.wait_stream(side_c)
+    mu1, std1, pbad = pred[:, 0], pred[:, 3], pred[:, 4]
+
+    def plane_valued(k, esc):
+        a = answerable(mu1, std1, pbad, k, std_thr, lam_thr, bad_lo, bad_hi, esc)
+        return int((a & torch.isfinite(mu1) & (pbad < 0.5)).sum().item())
+
     kappa, escapes, cal_agree, cal_answered, cal_table = calibrate_kappa(
         pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
-        bad_lo, bad_hi, calib_target, dlam_cal)
+        bad_lo, bad_hi, calib_target, dlam_cal, plane_valued=plane_valued)
     tm["calibrate"] = time.perf_counter() - t
-    mu1, std1, pbad = pred[:, 0], pred[:, 3], pred[:, 4]
     if cal_answered == 0:
         flagged = torch.ones(n, dtype=torch.bool, device=dev)
     else:

@@ -77,7 +77,8 @@ def run_config(ctx, spec, k, seed, options=()):
     return d
 
 
-@pytest.mark.parametrize("quad,method", [(0, L.METHOD_VAR), (1, L.METHOD_VAR), (0, L.METHOD_VAR_TRACE)])
+@pytest.mark.parametrize("quad,method", [(0, L.METHOD_VAR), (1, L.METHOD_VAR), (0, L.METHOD_VAR_TRACE),
+                                         (1, L.METHOD_VAR_TRACE)])
 def test_c1_lorenz_64_dopri5_fixed(gpu_ctx, quad, method):
     run_config(gpu_ctx, dict(W.lorenz_fixed(64), method=method), 64, 101, (("lane_quad", quad),))
 

@@ -71,11 +71,12 @@ def test_var_lane_matches_oracle(gpu_ctx, model, integ, adaptive, prec, method):
     assert_match(g, o, TOL_FP64, exact=True)
 
 
+@pytest.mark.parametrize("method", [L.METHOD_VAR, L.METHOD_VAR_TRACE])
 @pytest.mark.parametrize("integ", [L.INT_DVERK, L.INT_DOPRI5])
 @pytest.mark.parametrize("adaptive", [0, 1])
 @pytest.mark.parametrize("prec", [L.PREC_FP64, L.PREC_MIXED])
 @pytest.mark.parametrize("model", ["lorenz", "rossler", "shimizu"])
-def test_var_quad_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
+def test_var_quad_matches_oracle(gpu_ctx, model, integ, adaptive, prec, method):
     """lane_quad = 1: four lanes per point (one deviation vector per lane, the
     base step and control repeated), on a grid of one block so that quads
     refill: the same bits as the oracle (and so as the one-lane kernel)."""
@@ -87,7 +88,7 @@ def test_var_quad_matches_oracle(gpu_ctx, model, integ, adaptive, prec):
             spec.update(model=L.MODEL_SHIMIZU, n_params=2, base_params=[0.0, 0.0],
                         init_state=[0.1, 0.0, 0.0], init_scale=1e-3)
             spec.update(W.grid_fields(0, 0.2, 2.0, 12, 1, 0.0, 1.2, 12))
-    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9)
+    spec.update(integrator=integ, adaptive=adaptive, rtol=1e-9, atol=1e-9, method=method)
     gpu_ctx.set_option("lane_quad", 1)
     gpu_ctx.set_option("lane_max_blocks", 1)
     try:

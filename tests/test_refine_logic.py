@@ -126,6 +126,52 @@ def test_calibration_falls_back_to_escape_answers_only():
     assert kappa == float("inf") and escapes and answered == 500 and agree == 1.0
 
 
+def test_calibration_needs_evidence_for_valued_answers():
+    """A rule whose lambda answers on the calibration subsample are all within
+    the bound but number fewer than min_valued is not taken: 20 well-predicted
+    values are no evidence for the rest of the plane.  The escapes-only rule
+    is taken instead; with min_valued 10 the 20 values are admitted."""
+    n = 10000
+    lam = np.tile([0.1, 0.0, -1.0], (n, 1))
+    st = np.zeros(n, dtype=np.uint32)
+    st[:500] = 4
+    lam[:500] = np.nan
+    pbad = np.zeros(n)
+    pbad[:500] = 0.995
+    std = np.full(n, 0.05)  # unsure everywhere ...
+    std[500:520] = 1e-5     # ... but at 20 points, whose values are exact
+    p = pred_rows(np.full(n, 0.1), std, pbad)
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995, **THR)
+    assert kappa == float("inf") and escapes and answered == 500
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995,
+                                                          min_valued=10, **THR)
+    assert kappa == 1.0 and escapes and answered == 520
+
+
+def test_calibration_rejects_values_the_calibration_never_saw():
+    """A rule with no valued answer on the calibration subsample, but which
+    would answer values elsewhere on the plane, has no evidence for them: it
+    is not taken (plane_valued counts the plane's valued answers per rule)."""
+    n = 10000
+    lam = np.tile([0.1, 0.0, -1.0], (n, 1))
+    st = np.zeros(n, dtype=np.uint32)
+    st[:500] = 4
+    lam[:500] = np.nan
+    pbad = np.zeros(n)
+    pbad[:500] = 0.995
+    p = pred_rows(np.full(n, 0.1), np.full(n, 0.05), pbad)  # no value answered here
+    seen = []
+
+    def plane_valued(kappa, escapes):
+        seen.append((kappa, escapes))
+        return 0 if kappa >= 4.0 else 50  # the plane has confident values below kappa 4
+
+    kappa, escapes, agree, answered, _ = R.calibrate_kappa(p, lam, st, target=0.9995,
+                                                          plane_valued=plane_valued, **THR)
+    assert kappa == 4.0 and escapes and answered == 500
+    assert (1.0, True) in seen
+
+
 def test_verify_gates_use_the_surrogate_where_it_answers():
     """The chosen form's answer at a verification point is the surrogate's
     where the rule answers it, the method's elsewhere; |dlambda1| is taken
