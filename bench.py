@@ -514,10 +514,12 @@ def bench_batch(args, ctx, n_points, wl_name):
     dev = torch.device("cuda", 0)
     mix = W.batch_mix(n_points)
 
-    def device_classes(precision):
+    def device_classes(precision, trace=False):
         out = []
         for c in mix:
             spec = dict(c["spec"], precision=precision)
+            if trace and spec["dim"] == 3:
+                spec["method"] = L.METHOD_VAR_TRACE
             out.append(dict(desc=L.finalize(L.make_desc(spec)), count=c["count"],
                             params=torch.from_numpy(c["params"]).to(dev).contiguous()))
         return out
@@ -525,7 +527,7 @@ def bench_batch(args, ctx, n_points, wl_name):
     def progress(msg):
         print(f"[batch-mix] {msg}", file=sys.stderr, flush=True)
 
-    classes = device_classes(L.PREC_MIXED)
+    classes = device_classes(L.PREC_MIXED, trace=not args.full_vectors)
     progress("classes " + ", ".join(f"{c['name']} {c['count']}" for c in mix))
     t0 = time.perf_counter()
     tune = A.tune_batch(ctx, classes, cache=A.Cache(None), time_scale=0.1)
@@ -587,7 +589,9 @@ def bench_batch(args, ctx, n_points, wl_name):
     kern_avg = sum(kern) / len(kern)
     rec = base_record(args, n_points / (ms_per_step * 1e-3), ms_per_step, classes[0]["desc"])
     rec["config"] = {
-        "workload": wl_name, "points": n_points, "chosen_form": "mixed, one persistent batched launch",
+        "workload": wl_name, "points": n_points,
+        "chosen_form": "mixed, one persistent batched launch"
+                       + ("" if args.full_vectors else ", 3-D classes with the trace closure"),
         "tuning": A.batch_kwargs(tune.best), "tuning_s": round(tune_s, 2),
         "tuning_timings_ms_sample": {str(k): round(v, 2) for k, v in tune.timings.items()},
         "plain_fp64_points_per_s": n_points / (plain_ms * 1e-3),

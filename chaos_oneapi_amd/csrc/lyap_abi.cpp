@@ -480,10 +480,11 @@ struct lyap_batch {
 namespace {
 
 int batch_kind(const lyap_sweep_desc& d) {
+    const bool trace = d.method == LYAP_METHOD_VAR_TRACE;
     switch (d.model) {
-        case LYAP_MODEL_SHIMIZU: return lyap::BATCH_SHIMIZU;
-        case LYAP_MODEL_LORENZ: return lyap::BATCH_LORENZ;
-        case LYAP_MODEL_ROSSLER: return lyap::BATCH_ROSSLER;
+        case LYAP_MODEL_SHIMIZU: return trace ? lyap::BATCH_SHIMIZU_TRACE : lyap::BATCH_SHIMIZU;
+        case LYAP_MODEL_LORENZ: return trace ? lyap::BATCH_LORENZ_TRACE : lyap::BATCH_LORENZ;
+        case LYAP_MODEL_ROSSLER: return trace ? lyap::BATCH_ROSSLER_TRACE : lyap::BATCH_ROSSLER;
         case LYAP_MODEL_L96:
             return d.dim == 8 ? lyap::BATCH_L96_8
                               : (d.dim == 16 ? lyap::BATCH_L96_16 : (d.dim == 32 ? lyap::BATCH_L96_32 : -1));
@@ -495,9 +496,12 @@ int batch_kind(const lyap_sweep_desc& d) {
 // (one lane runs a whole point), then Lorenz-96 by N descending
 int longest_first_rank(int kind) {
     switch (kind) {
-        case lyap::BATCH_LORENZ: return 0;
-        case lyap::BATCH_ROSSLER: return 1;
-        case lyap::BATCH_SHIMIZU: return 2;
+        case lyap::BATCH_LORENZ:
+        case lyap::BATCH_LORENZ_TRACE: return 0;
+        case lyap::BATCH_ROSSLER:
+        case lyap::BATCH_ROSSLER_TRACE: return 1;
+        case lyap::BATCH_SHIMIZU:
+        case lyap::BATCH_SHIMIZU_TRACE: return 2;
         case lyap::BATCH_L96_32: return 3;
         case lyap::BATCH_L96_16: return 4;
         default: return 5;
@@ -538,8 +542,9 @@ int plan_batch(const lyap_batch_class* classes, int n_classes, const lyap_batch_
         lyap_sweep_desc fd = k.desc;
         int rc = lyap_desc_finalize(&fd);
         if (rc != LYAP_SUCCESS) return fail(rc, "batch class %d: %s", j, g_last_error.c_str());
-        if (fd.method != LYAP_METHOD_VAR)
-            return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: a batch runs the variational method only", j);
+        if (fd.method != LYAP_METHOD_VAR && fd.method != LYAP_METHOD_VAR_TRACE)
+            return fail(LYAP_ERR_UNSUPPORTED,
+                        "batch class %d: a batch runs the variational method (and its trace closure) only", j);
         const int kind = batch_kind(fd);
         if (kind < 0)
             return fail(LYAP_ERR_UNSUPPORTED, "batch class %d: no batch kernel for model %d dimension %d",
@@ -567,7 +572,7 @@ int plan_batch(const lyap_batch_class* classes, int n_classes, const lyap_batch_
                              spread_stride(k.count), tu.gather, 0};
         r.kind = kind;
         rank[n] = tu.order == 0 ? longest_first_rank(kind)
-                                : (tu.order == 2 ? (kind >= lyap::BATCH_L96_8 ? 0 : 1) : 0);
+                                : (tu.order == 2 ? (lyap::batch_kind_is_l96(kind) ? 0 : 1) : 0);
         const int ppw = lyap::batch_points_per_wave(tu.layout, fd.dim);
         plan->wave_tasks += (k.count + ppw - 1) / ppw;
         ++n;

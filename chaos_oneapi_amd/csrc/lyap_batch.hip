@@ -5,7 +5,8 @@
 // the classes in the order the host put them in and drains each one through
 // that class's own work counter:
 //   - 3-D flows (shimizu, Lorenz, Rössler): one point per lane, lanes refill
-//     from the counter as their points finish (lane_drain);
+//     from the counter as their points finish (lane_drain), with N or (trace
+//     closure) N - 1 deviation vectors;
 //   - Lorenz-96 N = 8, 16, 32: P points per wave, each on 64/P lanes, slots
 //     refill independently (l96_wave_drain).
 // A wave moves to the next class once the current class has no items left and
@@ -96,6 +97,9 @@ LYAP_D void batch_body(const BatchClass* __restrict__ cls, int n_cls, double* ld
             case BATCH_SHIMIZU: lane_drain<Shimizu, INTEG, ADAPT, MIXED>(d, getd, io, getio); break;
             case BATCH_LORENZ: lane_drain<Lorenz, INTEG, ADAPT, MIXED>(d, getd, io, getio); break;
             case BATCH_ROSSLER: lane_drain<Rossler, INTEG, ADAPT, MIXED>(d, getd, io, getio); break;
+            case BATCH_SHIMIZU_TRACE: lane_drain<Shimizu, INTEG, ADAPT, MIXED, true>(d, getd, io, getio); break;
+            case BATCH_LORENZ_TRACE: lane_drain<Lorenz, INTEG, ADAPT, MIXED, true>(d, getd, io, getio); break;
+            case BATCH_ROSSLER_TRACE: lane_drain<Rossler, INTEG, ADAPT, MIXED, true>(d, getd, io, getio); break;
             case BATCH_L96_8: l96_wave_drain<8, L.p8, INTEG, ADAPT, MIXED>(d, getd, io, lds); break;
             case BATCH_L96_16: l96_wave_drain<16, L.p16, INTEG, ADAPT, MIXED>(d, getd, io, lds); break;
             case BATCH_L96_32: l96_wave_drain<32, L.p32, INTEG, ADAPT, MIXED>(d, getd, io, lds); break;

@@ -38,7 +38,14 @@ enum BatchKind : int32_t {
     BATCH_L96_8 = 3,
     BATCH_L96_16 = 4,
     BATCH_L96_32 = 5,
+    // the 3-D flows with the trace closure (N - 1 vectors, lambda_N from tr J)
+    BATCH_SHIMIZU_TRACE = 6,
+    BATCH_LORENZ_TRACE = 7,
+    BATCH_ROSSLER_TRACE = 8,
 };
+constexpr bool batch_kind_is_l96(int kind) {
+    return kind == BATCH_L96_8 || kind == BATCH_L96_16 || kind == BATCH_L96_32;
+}
 constexpr int kMaxBatchClasses = 16;
 constexpr int kBatchLayouts = 2;
 

@@ -213,7 +213,8 @@ int lyap_cuda_lyapunov(lyap_ctx* ctx, int block_num, int block_size, size_t task
  * point per lane, Lorenz-96 N = 8/16/32 several points per wave).  The counter
  * reset (a kernel) and the launch are captured once into a hipGraph at create
  * time; lyap_batch_launch replays it on the caller's stream.  All classes share
- * the integrator, the adaptive flag and the precision; the method is VAR.
+ * the integrator, the adaptive flag and the precision; the method is VAR, or
+ * VAR_TRACE for a 3-D class (classes may differ in that).
  * Buffers are device memory owned by the caller and must stay valid for the
  * life of the batch (the graph holds their addresses).
  */

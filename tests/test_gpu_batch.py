@@ -16,13 +16,15 @@ from oracle import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def device_classes(mix, precision=None):
+def device_classes(mix, precision=None, trace=False):
     dev = torch.device("cuda", 0)
     classes = []
     for c in mix:
         spec = dict(c["spec"])
         if precision is not None:
             spec["precision"] = precision
+        if trace and spec["dim"] == 3:
+            spec["method"] = L.METHOD_VAR_TRACE
         desc = L.finalize(L.make_desc(spec))
         classes.append(dict(desc=desc, count=c["count"],
                             params=torch.from_numpy(c["params"]).to(dev).contiguous()))
@@ -42,11 +44,14 @@ def assert_same(a, b, what):
     np.testing.assert_array_equal(lam_a, lam_b, err_msg=what)  # NaN == NaN here
 
 
-@pytest.mark.parametrize("prec", [L.PREC_MIXED, L.PREC_FP64])
-def test_batch_matches_oracle(gpu_ctx, prec):
+@pytest.mark.parametrize("prec,trace", [(L.PREC_MIXED, False), (L.PREC_FP64, False), (L.PREC_MIXED, True)])
+def test_batch_matches_oracle(gpu_ctx, prec, trace):
+    """trace: the 3-D classes run the trace closure (two vectors), the
+    Lorenz-96 classes every vector, in the same launch."""
     mix = W.batch_mix(1500, seed=11, time_scale=0.01)
-    classes = device_classes(mix, prec)
-    print("case batch-vs-oracle prec", prec, [(c["name"], c["count"]) for c in mix], flush=True)
+    classes = device_classes(mix, prec, trace)
+    print("case batch-vs-oracle prec", prec, "trace", trace, [(c["name"], c["count"]) for c in mix],
+          flush=True)
     b = L.Batch(gpu_ctx, classes)
     outs = to_host(b.launch())
     torch.cuda.synchronize()
@@ -65,9 +70,10 @@ TUNINGS = [dict(layout=0, occupancy3=False, order=0), dict(layout=0, occupancy3=
            dict(layout=1, occupancy3=True, order=0, gather=24)]
 
 
-def test_batch_matches_single_class_kernels(gpu_ctx):
+@pytest.mark.parametrize("trace", [False, True])
+def test_batch_matches_single_class_kernels(gpu_ctx, trace):
     mix = W.batch_mix(6000, seed=12, time_scale=0.01)
-    classes = device_classes(mix)
+    classes = device_classes(mix, trace=trace)
     ref = []
     for cl in classes:
         lam, st, na, nr = gpu_ctx.run(cl["desc"], count=cl["count"], params=cl["params"])
