@@ -378,7 +378,8 @@ def bench_surrogate(args, ctx, spec, wl_name):
 
     desc = L.finalize(L.make_desc(spec))
     n = desc.n1 * desc.n2
-    kw = dict(train_n=args.train_n, epochs=args.epochs, fourier_sigma=args.fourier_sigma)
+    kw = dict(train_n=args.train_n, epochs=args.epochs, fourier_sigma=args.fourier_sigma,
+              one_launch=not args.split_schedule)
     for _ in range(args.warmup):
         R.refine(ctx, spec, **kw)
     torch.cuda.synchronize()
@@ -416,23 +417,17 @@ def bench_surrogate(args, ctx, spec, wl_name):
     answered = ~res.rerun
     ans_val = answered & okp & (res.status == 0)
     dl_s = np.abs(res.lam[ans_val, 0] - lam_p[ans_val, 0])
-    # the step's mixed-form launches of k_var_lane (the lattice launch, the
-    # calibration and verification launch beside it, the rerun, and the optional
-    # launch beside training): total algorithmic flops over total duration = the
-    # average flops per launch over the average launch duration
+    # the step's mixed-form launches of k_var_lane (one watched launch over the
+    # plane, or the split schedule's three): total algorithmic flops over total
+    # duration = the average flops per launch over the average launch duration
     acc_fl, rej_fl, f32_acc, n_vec = F.desc_step(desc)
     per = (acc_fl, rej_fl)
     renorms = F.mgs(desc.dim, n_vec) * (desc.n_renorm_transient + desc.n_renorm_average)
-    acc = st["rerun_acc_steps"] + st["first_acc_steps"] + st["overlap_acc_steps"]
-    rej = st["rerun_rej_steps"] + st["first_rej_steps"] + st["overlap_rej_steps"]
-    fl = per[0] * acc + per[1] * rej + renorms * (st["rerun_points"] + st["first_points"] + st["overlap_points"])
+    acc = sum(x["acc_steps"] for x in st["launches"])
+    rej = sum(x["rej_steps"] for x in st["launches"])
+    fl = per[0] * acc + per[1] * rej + renorms * sum(x["points"] for x in st["launches"])
     f32 = f32_acc * acc
-    launch_ms = [st["first_kernel_ms"]]
-    if st.get("split_first"):
-        launch_ms.append(st["split_kernel_ms"])
-    launch_ms.append(st["rerun_kernel_ms"])
-    if st["overlap_points"]:
-        launch_ms.append(st["overlap_kernel_ms"])
+    launch_ms = [x["kernel_ms"] for x in st["launches"]]
     rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
     dl_c = np.abs(res.lam[computed, 0] - lam_p[computed, 0])
     rec["config"] = {
@@ -475,7 +470,9 @@ def bench_surrogate(args, ctx, spec, wl_name):
         # per-exponent error maps of the mixed form against FP64 over the plane
         "dlambda_mixed_vs_plain_fp64_per_exponent": error_map_stats(lam_m, lam_p, (st_m == 0) & okp, desc.n1, desc.n2),
         "rerun_lane_gather": st.get("rerun_gather"),
-        "overlap_points": st.get("overlap_points"),
+        "one_launch": st.get("one_launch"),
+        "computed_points": st.get("computed_points"),
+        "speculative_points": st.get("speculative_points"),
         "timings_s": {k: round(v, 3) for k, v in res.timings.items()},
         "plain_fp64_achieved_tflops": flops_p / (plain_ms * 1e-3) / 1e12,
     }
@@ -483,11 +480,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
         np.savez_compressed(args.error_maps, lam_chosen=res.lam, lam_mixed=lam_m, lam_fp64=lam_p,
                             status_chosen=res.status, status_fp64=st_p, answered=answered,
                             n1=desc.n1, n2=desc.n2)
-    parts = ["lattice launch", "calibration/verification launch beside it"] if st.get("split_first") \
-        else ["first launch"]
-    if st["overlap_points"]:
-        parts.append("overlap launch")
-    parts.append("rerun")
+    parts = [x["name"] for x in st["launches"]]
     # the launches' flops over their summed durations (each charged its own)
     rec["roofline"] = roofline(fl - f32, f32, sum(launch_ms), "k_var_lane (mixed): " + ", ".join(parts),
                                args.probe, measured_traffic(wl_name))
@@ -727,6 +720,8 @@ def main():
     ap.add_argument("--train-n", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--fourier-sigma", type=float, default=6.0)
+    ap.add_argument("--split-schedule", action="store_true",
+                    help="surrogate workload: the split schedule (separate first launches, then a rerun)")
     ap.add_argument("--batch-points", type=int, default=1 << 20)
     ap.add_argument("--error-maps", default=None,
                     help="write the full per-point lambda maps (.npz) to this path")

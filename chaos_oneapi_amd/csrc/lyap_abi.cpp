@@ -8,8 +8,11 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "chaos_lyap.h"
@@ -123,6 +126,15 @@ struct lyap_ctx {
     unsigned long long* counter = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
+    // progress signals and skips of the next run (lyap_ctx_set_watch): device
+    // counters, host-visible flags (pinned, allocated on first use), the
+    // milestone ends, the caller's skip words, and the run's sequence number
+    bool watch_next = false;
+    int64_t watch_end[2] = {0, 0};
+    const uint32_t* watch_skip = nullptr;
+    unsigned long long* watch_done = nullptr;
+    uint32_t* watch_flag = nullptr;
+    uint32_t watch_seq = 0;
 };
 
 extern "C" {
@@ -164,6 +176,8 @@ int lyap_ctx_destroy(lyap_ctx* c) {
     if (c->ev0) hipEventDestroy(c->ev0);
     if (c->ev1) hipEventDestroy(c->ev1);
     if (c->counter) hipFree(c->counter);
+    if (c->watch_done) hipFree(c->watch_done);
+    if (c->watch_flag) hipHostFree(c->watch_flag);
     delete c;
     return LYAP_SUCCESS;
 }
@@ -305,12 +319,26 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
     lyap::SweepIO io{first,    count,    d_idx_list, d_params,   d_inits,
                      d_lambda, d_status, d_n_acc,    d_n_rej,    c->counter,
                      c->item_order ? 1 : spread_stride(count), c->lane_gather, 0};
+    const bool quad = c->lane_quad != 0 &&
+                      (fd.method == LYAP_METHOD_VAR || fd.method == LYAP_METHOD_VAR_TRACE) && fd.dim == 3;
+    lyap::LaneWatch watch{};
+    const bool watched = c->watch_next;
+    c->watch_next = false;  // one run only
+    if (watched) {
+        if (!lane_model(fd.model) || map_model(fd.model) || quad || !fd.adaptive ||
+            (fd.method != LYAP_METHOD_VAR && fd.method != LYAP_METHOD_VAR_TRACE))
+            return fail(LYAP_ERR_UNSUPPORTED,
+                        "progress signals and skips need the sweep's method with adaptive steps on a 3-D "
+                        "flow, one lane per point");
+        if (c->watch_end[1] > count)
+            return fail(LYAP_ERR_INVALID, "watched items [0, %lld) beyond the %lld items of the run",
+                        (long long)c->watch_end[1], (long long)count);
+        watch = lyap::LaneWatch{c->watch_skip, c->watch_done, c->watch_flag,
+                                {c->watch_end[0], c->watch_end[1]}, ++c->watch_seq, 0u};
+    }
     if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
     if (lane_model(fd.model)) {
-        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks,
-                                    c->lane_quad != 0 &&
-                                        (fd.method == LYAP_METHOD_VAR || fd.method == LYAP_METHOD_VAR_TRACE) &&
-                                        fd.dim == 3);
+        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks, quad, watched ? &watch : nullptr);
     } else if (fd.model == LYAP_MODEL_L96) {
         if (fd.method != LYAP_METHOD_VAR)
             return fail(LYAP_ERR_UNSUPPORTED, "Lorenz-96 runs the coupled variational method only");
@@ -392,6 +420,59 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
         return LYAP_SUCCESS;
     }
     return fail(LYAP_ERR_INVALID, "unknown option '%s'", key);
+}
+
+int lyap_ctx_set_watch(lyap_ctx* c, int64_t end0, int64_t end1, const uint32_t* d_skip) {
+    if (!c) return fail(LYAP_ERR_INVALID, "lyap_ctx_set_watch: NULL context");
+    if (end0 < 0 || end1 < end0)
+        return fail(LYAP_ERR_INVALID, "lyap_ctx_set_watch: milestones [0, %lld) and [%lld, %lld) are not ranges",
+                    (long long)end0, (long long)end0, (long long)end1);
+    hipError_t e = hipSetDevice(c->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    if (!c->watch_done) {
+        if ((e = hipMalloc(&c->watch_done, 2 * sizeof(unsigned long long))) != hipSuccess) {
+            c->watch_done = nullptr;
+            return hip_fail(e, "hipMalloc(watch counters)");
+        }
+    }
+    if (!c->watch_flag) {
+        void* h = nullptr;
+        if ((e = hipHostMalloc(&h, 2 * sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped)) !=
+            hipSuccess)
+            return hip_fail(e, "hipHostMalloc(watch flags)");
+        c->watch_flag = static_cast<uint32_t*>(h);
+        c->watch_flag[0] = c->watch_flag[1] = 0u;
+    }
+    c->watch_next = true;
+    c->watch_end[0] = end0;
+    c->watch_end[1] = end1;
+    c->watch_skip = d_skip;
+    return LYAP_SUCCESS;
+}
+
+int lyap_ctx_watch_wait(lyap_ctx* c, int milestone, double timeout_s, int* reached) {
+    if (!c || !reached || milestone < 0 || milestone > 1)
+        return fail(LYAP_ERR_INVALID, "lyap_ctx_watch_wait: bad argument");
+    *reached = 0;
+    if (!c->watch_flag || c->watch_seq == 0)
+        return fail(LYAP_ERR_INVALID, "lyap_ctx_watch_wait: no watched run on this context");
+    const int64_t size = milestone ? c->watch_end[1] - c->watch_end[0] : c->watch_end[0];
+    if (size == 0) {
+        *reached = 1;
+        return LYAP_SUCCESS;
+    }
+    const volatile uint32_t* f = c->watch_flag + milestone;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+        if (*f == c->watch_seq) {
+            std::atomic_thread_fence(std::memory_order_acquire);
+            *reached = 1;
+            return LYAP_SUCCESS;
+        }
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (dt >= timeout_s) return LYAP_SUCCESS;
+        std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
 }
 
 int lyap_last_kernel_ms(lyap_ctx* c, float* ms) {

@@ -698,4 +698,31 @@ LYAP_D KernargIO* kernarg_io() {
     return p;
 }
 
+// Progress signals and per-item skips of one launch of the sweep's method
+// (k_var_lane's third argument; all pointers null: neither).  Items are list
+// positions.  Items [0, end[0]) form milestone 0 and [end[0], end[1])
+// milestone 1: a lane that finishes such an item releases its outputs to the
+// device scope and counts it in done[k]; the lane that finishes the last one
+// stores `seq` to flag[k], memory the host reads while the launch runs.  An
+// item whose skip word is nonzero when a lane takes it is not integrated: its
+// status is LYAP_PT_SKIPPED and its exponents NaN.  The skip words are read
+// past the caches, so words written while the launch runs are seen.
+struct LaneWatch {
+    const uint32_t* skip;
+    unsigned long long* done;  // [2], zero at launch
+    uint32_t* flag;            // [2], host-visible
+    int64_t end[2];
+    uint32_t seq;
+    uint32_t pad;
+};
+using KernargWatch = const __attribute__((address_space(4))) LaneWatch;
+static_assert(sizeof(SweepIO) % alignof(LaneWatch) == 0, "LaneWatch follows SweepIO unpadded");
+LYAP_D KernargWatch* kernarg_watch() {
+    const __attribute__((address_space(4))) char* b =
+        (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+    KernargWatch* p = (KernargWatch*)(b + sizeof(lyap_sweep_desc) + sizeof(SweepIO));
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
+
 }  // namespace lyap

@@ -192,18 +192,59 @@ LYAP_D int64_t grab_items(bool want, unsigned long long* counter) {
     return want ? (int64_t)(base + (unsigned long long)__popcll(below)) : -1;
 }
 
+// no progress signals or skips (the batched launch)
+struct NoWatch {
+    LYAP_D KernargWatch* operator()() const { return nullptr; }
+};
+
+// A lane finished (or skipped) watched item `item`: its outputs are released
+// to the device scope before the count, and the lane that completes a
+// milestone stores the launch's sequence number to the host-visible flag.
+template <class W>
+LYAP_D void watch_count(W* w, int64_t item) {
+    if (!w->done || item >= w->end[1]) return;
+    const int k = item < w->end[0] ? 0 : 1;
+    const int64_t size = k ? w->end[1] - w->end[0] : w->end[0];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned long long c =
+        __hip_atomic_fetch_add(w->done + k, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((int64_t)c + 1 == size) __hip_atomic_store(w->flag + k, w->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Item `item` of a watched launch is skipped: its outputs say so (status
+// LYAP_PT_SKIPPED, NaN exponents, no steps) and it counts for its milestone.
+template <class GetWatch, class GetIO, class GetDesc>
+LYAP_D bool skip_item(GetWatch getw, GetIO getio, GetDesc getd, int64_t item) {
+    const auto w = getw();
+    if (!w->skip) return false;
+    if (__hip_atomic_load(w->skip + item, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0u) return false;
+    const auto dp = getd();
+    const auto iop = getio();
+    for (int r = 0; r < dp->n_exps; ++r) iop->lambda[item * dp->n_exps + r] = __builtin_nan("");
+    if (iop->status) iop->status[item] = LYAP_PT_SKIPPED;
+    if (iop->n_acc) iop->n_acc[item] = 0u;
+    if (iop->n_rej) iop->n_rej[item] = 0u;
+    watch_count(w, item);
+    return true;
+}
+
 // Persistent per-lane loop over the points of one sweep: lanes that finish a
 // point take the next item from io.counter until the items are exhausted and
 // every lane of the wave is done.  getd() and getio() return the descriptor and
 // the IO record to read the per-point set-up and output fields from (opaque
 // pointers, so those fields are not kept live in SGPRs across the step loop);
 // io supplies only the fields every iteration reads (counter, count, gather).
-template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false, class GetDesc, class GetIO>
-LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io, GetIO getio) {
+// getw() returns the launch's LaneWatch record (progress signals, skips) or
+// null.
+template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false, class GetDesc, class GetIO,
+          class GetWatch = NoWatch>
+LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io, GetIO getio,
+                       GetWatch getw = GetWatch()) {
     // TRACE: N - 1 deviation vectors; the last exponent is the averaged
     // integral of tr J (Liouville) minus the others
     constexpr int N = M::N, P = M::P, K = TRACE ? M::N - 1 : M::N, S = Tab<INTEG>::S;
     using TX = typename std::conditional<MIXED, float, double>::type;
+    constexpr bool kWatch = !std::is_same<GetWatch, NoWatch>::value;
     // scalars the step loop reads; everything else of the descriptor is
     // re-read through an opaque pointer when a lane takes a new point, so the
     // compiler does not keep (and spill) the whole descriptor in SGPRs
@@ -258,12 +299,21 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
         if (iop->status) iop->status[item] = st;
         if (iop->n_acc) iop->n_acc[item] = nacc;
         if (iop->n_rej) iop->n_rej[item] = nrej;
+        if constexpr (kWatch) watch_count(getw(), item);
     };
 
     for (;;) {
         const int64_t got = grab_items(!active && !exhausted, counter);
-        if (!active && !exhausted) {
-            if (got < count) {
+        // taking a point and finishing one are rare next to a step: the hints
+        // keep the register allocator's spill code out of the step loop
+        if (__builtin_expect(!active && !exhausted, 0)) {
+            bool skipped = false;
+            if constexpr (kWatch) {
+                if (got < count) skipped = skip_item(getw, getio, getd, item_position(got, getio()->stride, count));
+            }
+            if (skipped) {
+                // not integrated (outputs written); the lane takes the next item
+            } else if (got < count) {
                 const auto dp = getd();
                 const auto iop = getio();
                 item = item_position(got, iop->stride, count);
@@ -300,7 +350,12 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 exhausted = true;
             }
         }
-        if (!__any(active)) break;
+        if constexpr (kWatch) {
+            // a lane whose item was skipped is neither active nor exhausted
+            if (!__any(active || !exhausted)) break;
+        } else {
+            if (!__any(active)) break;
+        }
         {
             const uint64_t pm = __ballot(active && pending);
             if (pm) {
@@ -425,7 +480,7 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 }
             }
         }
-        if (finished) {
+        if (__builtin_expect(finished, 0)) {
             write_out(st);
             active = false;
         }

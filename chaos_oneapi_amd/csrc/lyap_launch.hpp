@@ -14,9 +14,11 @@ namespace lyap {
 int set_error(int code, const char* fmt, ...);
 
 // 3-D flows and maps, one lane per point (lyap_sweep.hip); quad: the sweep's
-// method on the 3-D flows with four lanes per point
+// method on the 3-D flows with four lanes per point; watch: progress signals
+// and skips of the sweep's method with one lane per point (null: none)
 hipError_t launch_lane_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
-                             int num_cus, int max_blocks, bool quad = false);
+                             int num_cus, int max_blocks, bool quad = false,
+                             const LaneWatch* watch = nullptr);
 
 // Lorenz-96 (lyap_l96.hip): ppw > 0 runs the wave-level drain with ppw
 // points per wave64; otherwise one point per block of `waves` wave64s

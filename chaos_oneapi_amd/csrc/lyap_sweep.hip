@@ -31,6 +31,16 @@ __global__ __launch_bounds__(256) void k_var_lane(const lyap_sweep_desc d, const
     lane_drain<M, INTEG, ADAPT, MIXED, TRACE>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); });
 }
 
+// the same with progress signals and skips (w, read from the kernarg segment
+// when a lane takes or finishes a point); adaptive steps only.  Compiled for
+// three waves a SIMD (168 VGPRs), the occupancy of the plain kernel it stands
+// in for: the signal and skip code would otherwise push some variants past it
+template <class M, int INTEG, bool MIXED, bool TRACE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_var_lane_watch(const lyap_sweep_desc d, const SweepIO io, const LaneWatch w) {
+    lane_drain<M, INTEG, true, MIXED, TRACE>(d, [] { return kernarg_desc(); }, io, [] { return kernarg_io(); },
+                                             [] { return kernarg_watch(); });
+}
+
 // the same method with four lanes per point (latency-bound launches)
 template <class M, int INTEG, bool ADAPT, bool MIXED, bool TRACE = false>
 __global__ __launch_bounds__(256) void k_var_quad(const lyap_sweep_desc d, const SweepIO io) {
@@ -506,14 +516,19 @@ __global__ void k_reset_counter(unsigned long long* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0ULL;
 }
 
+// the two milestone counters of a watched launch
+__global__ void k_reset_watch(unsigned long long* done) {
+    if (blockIdx.x == 0 && threadIdx.x < 2) done[threadIdx.x] = 0ULL;
+}
+
 // ---------------------------------------------------------------------------
 // launch table
 // ---------------------------------------------------------------------------
 // lanes_per_point: 1 (k_var_lane and the reference kernels) or 4 (k_var_quad)
-template <class K>
+template <class K, class... Extra>
 static hipError_t launch_persistent(K kern, const lyap_sweep_desc& d, const SweepIO& io,
                                     hipStream_t stream, int num_cus, int max_blocks_hint,
-                                    int lanes_per_point = 1) {
+                                    int lanes_per_point, const Extra&... extra) {
     int per_cu = 0;
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0);
     if (e != hipSuccess) return e;
@@ -524,20 +539,46 @@ static hipError_t launch_persistent(K kern, const lyap_sweep_desc& d, const Swee
     if (max_blocks_hint > 0 && blocks > max_blocks_hint) blocks = max_blocks_hint;
     if (blocks < 1) blocks = 1;
     hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, stream, io.counter);
-    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, d, io);
+    hipLaunchKernelGGL(kern, dim3((unsigned)blocks), dim3(256), 0, stream, d, io, extra...);
     return hipGetLastError();
+}
+
+// watched launches: the sweep's method, adaptive, one lane per point
+template <class M>
+static hipError_t dispatch_watch(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
+                                 int num_cus, int max_blocks, const LaneWatch& w) {
+    if (!d.adaptive) return hipErrorInvalidValue;
+    const bool trace = d.method == LYAP_METHOD_VAR_TRACE;
+    const int key = (d.integrator << 2) | (trace ? 2 : 0) | (d.precision ? 1 : 0);
+    switch (key) {
+#define LYAP_CASE(I, T, X)                             \
+    case ((I) << 2) | ((T) ? 2 : 0) | ((X) ? 1 : 0): \
+        return launch_persistent(k_var_lane_watch<M, I, X, T>, d, io, s, num_cus, max_blocks, 1, w);
+        LYAP_CASE(LYAP_INT_DVERK, false, false)
+        LYAP_CASE(LYAP_INT_DVERK, false, true)
+        LYAP_CASE(LYAP_INT_DVERK, true, false)
+        LYAP_CASE(LYAP_INT_DVERK, true, true)
+        LYAP_CASE(LYAP_INT_DOPRI5, false, false)
+        LYAP_CASE(LYAP_INT_DOPRI5, false, true)
+        LYAP_CASE(LYAP_INT_DOPRI5, true, false)
+        LYAP_CASE(LYAP_INT_DOPRI5, true, true)
+#undef LYAP_CASE
+        default:
+            return hipErrorInvalidValue;
+    }
 }
 
 template <class M>
 static hipError_t dispatch_model(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
-                                 int num_cus, int max_blocks, bool quad) {
+                                 int num_cus, int max_blocks, bool quad, const LaneWatch* w) {
+    if (w) return dispatch_watch<M>(d, io, s, num_cus, max_blocks, *w);
     if (d.method == LYAP_METHOD_VAR) {
         const int key = (d.integrator << 2) | (d.adaptive ? 2 : 0) | (d.precision ? 1 : 0);
         switch (key) {
 #define LYAP_CASE(I, A, X)                                                                  \
     case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):                                        \
         return quad ? launch_persistent(k_var_quad<M, I, A, X>, d, io, s, num_cus, max_blocks, 4) \
-                    : launch_persistent(k_var_lane<M, I, A, X>, d, io, s, num_cus, max_blocks);
+                    : launch_persistent(k_var_lane<M, I, A, X>, d, io, s, num_cus, max_blocks, 1);
             LYAP_CASE(LYAP_INT_DVERK, false, false)
             LYAP_CASE(LYAP_INT_DVERK, false, true)
             LYAP_CASE(LYAP_INT_DVERK, true, false)
@@ -557,7 +598,7 @@ static hipError_t dispatch_model(const lyap_sweep_desc& d, const SweepIO& io, hi
 #define LYAP_CASE(I, A, X)                                                                          \
     case ((I) << 2) | ((A) ? 2 : 0) | ((X) ? 1 : 0):                                                \
         return quad ? launch_persistent(k_var_quad<M, I, A, X, true>, d, io, s, num_cus, max_blocks, 4) \
-                    : launch_persistent(k_var_lane<M, I, A, X, true>, d, io, s, num_cus, max_blocks);
+                    : launch_persistent(k_var_lane<M, I, A, X, true>, d, io, s, num_cus, max_blocks, 1);
             LYAP_CASE(LYAP_INT_DVERK, false, false)
             LYAP_CASE(LYAP_INT_DVERK, false, true)
             LYAP_CASE(LYAP_INT_DVERK, true, false)
@@ -595,8 +636,8 @@ static hipError_t dispatch_map(const lyap_sweep_desc& d, const SweepIO& io, hipS
     if (d.method == LYAP_METHOD_VAR) {
         if (d.adaptive) return hipErrorInvalidValue;
         if (d.precision == LYAP_PREC_MIXED)
-            return launch_persistent(k_map_lane<M, true>, d, io, s, num_cus, max_blocks);
-        return launch_persistent(k_map_lane<M, false>, d, io, s, num_cus, max_blocks);
+            return launch_persistent(k_map_lane<M, true>, d, io, s, num_cus, max_blocks, 1);
+        return launch_persistent(k_map_lane<M, false>, d, io, s, num_cus, max_blocks, 1);
     }
     const int64_t grid = (io.count + 255) / 256 > 4096 ? 4096 : (io.count + 255) / 256;
     const dim3 g((unsigned)(grid < 1 ? 1 : grid)), b(256);
@@ -614,18 +655,19 @@ static hipError_t dispatch_map(const lyap_sweep_desc& d, const SweepIO& io, hipS
 }
 
 hipError_t launch_lane_sweep(const lyap_sweep_desc& d, const SweepIO& io, hipStream_t s,
-                             int num_cus, int max_blocks, bool quad) {
+                             int num_cus, int max_blocks, bool quad, const LaneWatch* watch) {
+    if (watch && watch->done) hipLaunchKernelGGL(k_reset_watch, dim3(1), dim3(64), 0, s, watch->done);
     switch (d.model) {
         case LYAP_MODEL_HENON:
             return dispatch_map<Henon>(d, io, s, num_cus, max_blocks);
         case LYAP_MODEL_LOGISTIC:
             return dispatch_map<Logistic>(d, io, s, num_cus, max_blocks);
         case LYAP_MODEL_SHIMIZU:
-            return dispatch_model<Shimizu>(d, io, s, num_cus, max_blocks, quad);
+            return dispatch_model<Shimizu>(d, io, s, num_cus, max_blocks, quad, watch);
         case LYAP_MODEL_LORENZ:
-            return dispatch_model<Lorenz>(d, io, s, num_cus, max_blocks, quad);
+            return dispatch_model<Lorenz>(d, io, s, num_cus, max_blocks, quad, watch);
         case LYAP_MODEL_ROSSLER:
-            return dispatch_model<Rossler>(d, io, s, num_cus, max_blocks, quad);
+            return dispatch_model<Rossler>(d, io, s, num_cus, max_blocks, quad, watch);
         default:
             return hipErrorInvalidValue;
     }

@@ -24,7 +24,7 @@ METHOD_VAR, METHOD_VAR_REF, METHOD_QR_REF, METHOD_FD_REF, METHOD_VAR_TRACE = 0, 
 INT_DVERK, INT_DOPRI5 = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 INIT_NONE, INIT_UNIFORM, INIT_NORMAL = 0, 1, 2
-PT_OK, PT_NONFINITE, PT_STEP_UNDERFLOW, PT_MAXSTEPS, PT_ESCAPED = 0, 1, 2, 3, 4
+PT_OK, PT_NONFINITE, PT_STEP_UNDERFLOW, PT_MAXSTEPS, PT_ESCAPED, PT_SKIPPED = 0, 1, 2, 3, 4, 5
 
 # every entry point include/chaos_lyap.h declares
 EXPORTS = (
@@ -35,7 +35,7 @@ EXPORTS = (
     "lyap_mlp_forward", "lyap_mlp_param_count", "lyap_mlp_get", "lyap_mlp_set_params",
     "lyap_batch_create", "lyap_batch_launch", "lyap_batch_last_ms", "lyap_batch_info",
     "lyap_batch_layout_ppw",
-    "lyap_batch_destroy", "lyap_batch_run_noalloc",
+    "lyap_batch_destroy", "lyap_batch_run_noalloc", "lyap_ctx_set_watch", "lyap_ctx_watch_wait",
 )
 
 
@@ -143,6 +143,8 @@ def load(path: str = LIB_PATH):
     L.lyap_sweep_run.argtypes = [vp, P(SweepDesc), i64, i64, vp, vp, vp, vp, vp, vp, vp, vp]
     L.lyap_last_kernel_ms.argtypes = [vp, P(ctypes.c_float)]
     L.lyap_ctx_set_option.argtypes = [vp, ctypes.c_char_p, i64]
+    L.lyap_ctx_set_watch.argtypes = [vp, i64, i64, vp]
+    L.lyap_ctx_watch_wait.argtypes = [vp, ctypes.c_int, d, P(ctypes.c_int)]
     L.lyap_mlp_create.argtypes = [ctypes.c_int, P(MlpConfig), P(vp)]
     L.lyap_mlp_destroy.argtypes = [vp]
     L.lyap_mlp_backward.argtypes = [vp, vp, vp, vp, i64, vp, vp]
@@ -256,6 +258,12 @@ class Context:
                 raise ValueError(f"{name} must be a contiguous {dt} CUDA tensor")
         if idx_list is not None and count > idx_list.numel():
             raise ValueError(f"count {count} exceeds the {idx_list.numel()} entries of idx_list")
+        watch, self._watch = getattr(self, "_watch", None), None
+        if watch is not None:
+            if watch[1] > count:
+                raise ValueError(f"watched items [0, {watch[1]}) beyond the {count} items of the run")
+            if watch[2] is not None and watch[2].numel() < count:
+                raise ValueError(f"skip holds {watch[2].numel()} words, {count} needed")
         n_rows = first + count
         if params is not None and params.numel() < n_rows * desc.n_params and idx_list is None:
             raise ValueError(f"params holds {params.numel()} values, {n_rows * desc.n_params} needed")
@@ -267,6 +275,8 @@ class Context:
                 if t is not None and (tuple(t.shape[:len(w)]) != w or not t.is_contiguous()):
                     raise ValueError(f"output tensor of shape {tuple(t.shape)}, {w} expected")
         s = stream if stream is not None else torch.cuda.current_stream(dev)
+        if watch is not None:
+            check(load().lyap_ctx_set_watch(self._h, watch[0], watch[1], _ptr(watch[2])))
         check(load().lyap_sweep_run(self._h, ctypes.byref(desc), int(first), int(count),
                                     _ptr(idx_list), _ptr(params), _ptr(inits), _ptr(lam),
                                     _ptr(st), _ptr(na), _ptr(nr), ctypes.c_void_p(s.cuda_stream)))
@@ -274,6 +284,28 @@ class Context:
 
     def set_option(self, key: str, value: int) -> None:
         check(load().lyap_ctx_set_option(self._h, key.encode(), int(value)))
+
+    def set_watch(self, end0: int, end1: int, skip=None) -> None:
+        """Progress signals and skips for the next run() on this context:
+        milestones items [0, end0) and [end0, end1); skip, an optional
+        contiguous int32/uint32 CUDA tensor of one word per item, read while
+        the launch runs (nonzero: the item is not integrated, status
+        PT_SKIPPED)."""
+        import torch
+
+        if skip is not None and not (skip.is_cuda and skip.is_contiguous() and skip.dtype in (torch.int32, torch.uint32)):
+            raise ValueError("skip must be a contiguous 32-bit CUDA tensor")
+        if not 0 <= end0 <= end1:
+            raise ValueError(f"milestones [0, {end0}) and [{end0}, {end1}) are not ranges")
+        # armed in the C context by the next run(), which checks the sizes
+        self._watch = (int(end0), int(end1), skip)
+
+    def watch_wait(self, milestone: int, timeout_s: float = 120.0) -> bool:
+        """Waits on the host until milestone 0 or 1 of the last watched run is
+        reached (True) or timeout_s passes (False)."""
+        r = ctypes.c_int(0)
+        check(load().lyap_ctx_watch_wait(self._h, int(milestone), float(timeout_s), ctypes.byref(r)))
+        return bool(r.value)
 
     def kernel_ms(self) -> float:
         ms = ctypes.c_float()

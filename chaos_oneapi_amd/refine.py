@@ -16,25 +16,29 @@ Pipeline, all on the device through liblyap.so:
      lambda1 it answers there is within dlam_cal (half the 1e-3 gate) of the
      method's value (the predicted std alone is not calibrated);
   5. the plain FP64 method judges the verification subsample.
-With longest_first the rerun takes its points longest first: a point's
-expected cost is the step count of its nearest training-lattice point, and
-the kernel takes the sorted list in order (context option item_order).  Off
-by default: on the 2048^2 Rossler plane (3.6 M rerun points, ~18 per lane)
-it measured no gain (10.40 s against 10.36 s spread over the plane).
-rerun_gather: the lane_gather option of the rerun (2^20 points of that
-plane: 2,537 ms at 4 against 2,564 at 0, 2,579 at 2, 2,561 at 6, 2,591-2,620
-at 8 and 2,685 at 12; bit-identical).  With the renormalisation's Newton
-inverse square root and 64-renormalisation log windows the block a wave runs
-for its gathered lanes is cheap, so little gathering pays.
-The training map and the calibration and verification runs do not depend on
-the surrogate: they are one launch; the FP64 judge runs on a side stream (with
-a context of its own) while the surrogate trains; the rerun of step 4 skips
-every point already computed.  Optionally (overlap_points > 0), while steps 2-4 run, a
-seeded subset of the other points is computed on another side stream with
-its persistent grid capped so the training kernels keep room.  Off by
-default: on the 2048^2 Rossler plane it saved at most 0.16 s of 9.3 s and
-made training time erratic (0.99 s, once 1.98 s, against 0.75 s alone).
-The returned map takes the method's values where it ran and the surrogate's
+rerun_gather: the lane_gather option of the method's launches (2^20 points
+of the 2048^2 Rossler plane: 2,537 ms at 4 against 2,564 at 0, 2,579 at 2,
+2,561 at 6, 2,591-2,620 at 8 and 2,685 at 12; bit-identical).
+
+Schedules.  With one_launch (the default, for the sweep's method with adaptive
+steps on a 3-D flow) the method runs as one persistent launch over the
+training lattice, the calibration points, the verification points and then
+every other point, on a side stream (lyap_ctx_set_watch): it signals the host
+when the lattice and when the calibration points are done, while it goes on;
+training, inference and the choice of the rule follow on the main stream, and
+the points the rule answers are skipped from the moment their skip words are
+written.  The FP64 judge runs on the main stream first, on a few blocks
+(judge_blocks), beside the launch.  So no wave idles on the slowest point of a short
+launch: on the 2048^2 Rossler plane the lattice, calibration and verification
+points as launches of their own (one point per lane) kept their waves mostly
+masked for ~0.8 s, and a launch beside them could only use the issue slots
+they left.  The split schedule (one_launch False, or a fine grid without the
+lattice): the training map on the main stream with the calibration and
+verification points beside it on a side stream, each capped at one block per
+CU, the judge beside training, then a rerun of the flagged points.
+The returned map takes the surrogate's answers where the rule answers a point
+outside the three first sets (also where the one launch computed it before the
+rule was known, so the map does not depend on timing) and the method's values
 elsewhere.  Gates reported on the held-out verification subsample
 (verify_gates): max |dlambda1| between the chosen form's answer (the
 surrogate's where the rule answers the point, the method's elsewhere) and
@@ -89,7 +93,9 @@ class RefineResult:
 # process (GPU_MAX_HW_QUEUES) consecutive pool streams land on different
 # queues from call to call, sometimes the main stream's, which serialises a
 # side launch behind the main one (measured: 0.6-0.8 s stalls in every other
-# step).  The same three streams every call keep one placement.
+# step).  The same two streams every call keep one placement; a third one
+# shared the main stream's queue (a launch on it started only when the main
+# stream's launch ended), so refine() uses two.
 _SIDE_STREAMS = {}
 
 
@@ -117,7 +123,7 @@ def side_streams(dev):
 
     key = dev.index if dev.index is not None else torch.cuda.current_device()
     if key not in _SIDE_STREAMS:
-        _SIDE_STREAMS[key] = tuple(torch.cuda.Stream(dev) for _ in range(3))
+        _SIDE_STREAMS[key] = tuple(torch.cuda.Stream(dev) for _ in range(2))
     return _SIDE_STREAMS[key]
 
 
@@ -262,17 +268,92 @@ def verify_gates(flagged, pred, lam_m, st_m, lam_j, st_j):
     return out
 
 
+def _judge_desc(fd):
+    """The plain method judging the chosen form: FP64, every deviation vector."""
+    pdesc = L.SweepDesc.from_buffer_copy(fd)
+    pdesc.precision = L.PREC_FP64
+    if pdesc.method == L.METHOD_VAR_TRACE:
+        pdesc.method = L.METHOD_VAR
+    return pdesc
+
+
+def _train(dev, fd, p_t, lam_t, st_t, epochs, batch, lr, seed, hidden, fourier_sigma, max_batch):
+    """The surrogate trained on the training map (one shuffle per epoch, batches
+    contiguous slices of it) on the current stream; returns (net, losses)."""
+    import torch
+
+    n_t = int(p_t.shape[0])
+    end1 = fd.start1 + fd.delta1 * (fd.n1 - 1)
+    end2 = fd.start2 + fd.delta2 * (fd.n2 - 1)
+    ok = st_t == 0
+    lam_ok = lam_t[ok]
+    ym = lam_ok.mean(dim=0).tolist() if lam_ok.shape[0] else [0.0, 0.0, 0.0]
+    ys = lam_ok.std(dim=0).clamp_min(1e-6).tolist() if lam_ok.shape[0] > 1 else [1.0, 1.0, 1.0]
+    net = L.Mlp(device=dev.index, max_batch=max_batch, hidden=hidden, fourier_sigma=fourier_sigma, lr=lr,
+                seed=seed, ranges=(min(fd.start1, end1), max(fd.start1, end1), min(fd.start2, end2),
+                                   max(fd.start2, end2)), y_mean=ym, y_std=ys)
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    losses = []
+    for ep in range(epochs):
+        perm = torch.randperm(n_t, generator=gen).to(dev)
+        pp, lp, sp = p_t[perm], lam_t[perm], st_t[perm]
+        for b0 in range(0, n_t - batch + 1, batch):
+            # the epoch's loss is its last batch's (no copy on the other steps)
+            last = b0 + 2 * batch > n_t
+            net.train_step(pp[b0:b0 + batch], lp[b0:b0 + batch], sp[b0:b0 + batch], loss if last else None)
+        losses.append(loss)
+        loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    return net, losses
+
+
+def _infer(net, fd, n, infer_batch, dev):
+    import torch
+
+    pred = torch.empty((n, 5), dtype=torch.float32, device=dev)
+    for c0 in range(0, n, infer_batch):
+        c1 = min(n, c0 + infer_batch)
+        net.forward(grid_params(fd, torch.arange(c0, c1, device=dev)), pred[c0:c1])
+    return pred
+
+
+def _rule(pred, calib, lam_c, st_c, std_thr, lam_thr, bad_lo, bad_hi, calib_target, dlam_cal):
+    """The calibrated answer rule and the points it leaves to the method
+    (flagged): (kappa, escapes, agreement, answered, table, flagged)."""
+    import torch
+
+    mu1, std1, pbad = pred[:, 0], pred[:, 3], pred[:, 4]
+
+    def plane_valued(k, esc):
+        a = answerable(mu1, std1, pbad, k, std_thr, lam_thr, bad_lo, bad_hi, esc)
+        return int((a & torch.isfinite(mu1) & (pbad < 0.5)).sum().item())
+
+    kappa, escapes, cal_agree, cal_answered, cal_table = calibrate_kappa(
+        pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
+        bad_lo, bad_hi, calib_target, dlam_cal, plane_valued=plane_valued)
+    if cal_answered == 0:
+        flagged = torch.ones(pred.shape[0], dtype=torch.bool, device=pred.device)
+    else:
+        flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes)
+        flagged |= ~torch.isfinite(mu1)
+    return kappa, escapes, cal_agree, cal_answered, cal_table, flagged
+
+
+def one_launch_supported(fd) -> bool:
+    """Whether refine() can run the fine grid as one watched launch: the
+    sweep's method with adaptive steps on a 3-D flow (lyap_ctx_set_watch)."""
+    return (fd.method in (L.METHOD_VAR, L.METHOD_VAR_TRACE) and bool(fd.adaptive) and fd.dim == 3
+            and fd.model in (L.MODEL_SHIMIZU, L.MODEL_LORENZ, L.MODEL_ROSSLER))
+
+
 def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2048,
            lr: float = 1e-3, seed: int = 7, verify_frac: float = 0.01, std_thr: float = 0.01,
            lam_thr: float = 0.02, bad_lo: float = 0.02, bad_hi: float = 0.98,
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, dlam_cal: float = 0.5 * DLAMBDA1_GATE,
-           longest_first: bool = False,
-           rerun_gather: int = 4, overlap_points: int = 0, overlap_frac: float = 1 / 16,
-           overlap_blocks_per_cu: int = 2, first_blocks_per_cu: int = 2,
-           split_first: bool = True, split_blocks_per_cu: int = 1,
-           judge_quad: bool = False) -> RefineResult:
+           rerun_gather: int = 4, one_launch: bool = True, judge_blocks: int = 32,
+           split_blocks_per_cu: int = 1, watch_timeout_s: float = 600.0) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -289,12 +370,10 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     end2 = fd.start2 + fd.delta2 * (fd.n2 - 1)
     t_all = t = time.perf_counter()
 
-    # 1. the training map, and the calibration and verification subsamples: none
-    #    depends on the surrogate.  When the fine grid holds a train_n x train_n
-    #    lattice (stride s = (n-1) // (train_n-1) >= 1 in both directions), the
-    #    training map is that lattice, so the three sets are one chosen-form
-    #    launch over the fine grid (one tail) and their values enter the final
-    #    map
+    # the seeded calibration and verification subsamples (disjoint) and, when
+    # the fine grid holds a train_n x train_n lattice (stride s = (n-1) //
+    # (train_n-1) >= 1 in both directions), the training map as that lattice,
+    # so its values enter the final map
     gv = torch.Generator(device="cpu").manual_seed(seed + 1)
     n_ver = max(1, int(round(verify_frac * n)))
     n_cal = max(1, int(round(calib_frac * n)))
@@ -307,246 +386,190 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
     s2 = (fd.n2 - 1) // (train_n - 1) if train_n > 1 else 0
     n_t = train_n * train_n
     lattice = s1 >= 1 and s2 >= 1
-    cv_ctx, side_c = None, None
+    first_mask = torch.zeros(n, dtype=torch.bool, device=dev)
+    first_mask[verify] = True
+    first_mask[calib] = True
+    train_idx = None
     if lattice:
         ar = torch.arange(train_n, device=dev, dtype=torch.int64)
         train_idx = ((ar * s1)[:, None] * fd.n2 + (ar * s2)[None, :]).reshape(-1)
-        first = torch.cat([train_idx, calib, verify]).contiguous()
-        if split_first:
-            # training needs only the lattice: it runs on the main stream while
-            # the calibration and verification points run beside it on a side
-            # stream with a context of its own (needed only after training),
-            # each persistent grid capped at split_blocks_per_cu 256-lane blocks
-            # per CU (on the 2048^2 Rossler plane the lattice alone takes 594 ms
-            # where the three sets in one launch take 819 ms)
-            cv = first[n_t:]
-            side_c = side_streams(dev)[0]
-            side_c.wait_stream(main)
-            cv_ctx = L.Context(ctx.device)
-            cv_ctx.set_option("lane_max_blocks", split_blocks_per_cu * n_cus)
+        first_mask[train_idx] = True
+    # every other point in the kernels' spread order
+    rest = torch.nonzero(~first_mask).flatten()
+    rest = rest[spread_order(rest.numel(), dev)].contiguous()
+    launches = []  # per sweep launch: name, kernel ms, points computed, accepted / rejected steps
+
+    def launch_stats(name, kernel_ms, st, na, nr):
+        computed = st != L.PT_SKIPPED
+        launches.append(dict(name=name, kernel_ms=kernel_ms, points=int(computed.sum().item()),
+                             acc_steps=float(na.double().sum().item()),
+                             rej_steps=float(nr.double().sum().item())))
+
+    lam_j = st_j = None
+    if one_launch and lattice and one_launch_supported(fd):
+        # One launch of the method over the lattice, the calibration and the
+        # verification points and then every other point, on a side stream.
+        # The FP64 judge goes first, on the main stream, on judge_blocks
+        # 256-lane blocks, so its blocks hold their slots from the start (on
+        # the 2048^2 Rossler plane its 41,943 points on 32 blocks, about five
+        # a lane, end well before the launch: 7.88 s a step against 7.99 s on
+        # 256 blocks, one point a lane, whose mostly idle waves held a third
+        # of every SIMD's slots); the launch reports when the
+        # lattice (milestone 0) and the calibration points (milestone 1) are
+        # done, training and inference follow the judge on the main stream, and
+        # the answer rule's points are skipped from the moment their skip
+        # words land: no wave waits on a short launch's slowest point.  Points
+        # taken before the rule was known are computed all the same; the map
+        # takes the rule's answers there too, so it does not depend on timing.
+        n_first = n_t + n_cal + n_ver
+        items = torch.cat([train_idx, calib, verify, rest]).contiguous()
+        skip = torch.zeros(items.numel(), dtype=torch.int32, device=dev)
+        side = side_streams(dev)[0]
+        side.wait_stream(main)
+        judge_ctx = None
+        if judge:
+            judge_ctx = L.Context(ctx.device)
+            judge_ctx.set_option("lane_max_blocks", judge_blocks)
+            lam_j, st_j, _, _ = judge_ctx.run(_judge_desc(fd), idx_list=verify, stream=main)
+        ctx.set_option("item_order", 1)
+        ctx.set_option("lane_gather", rerun_gather)
+        ctx.set_watch(n_t, n_t + n_cal, skip)
+        try:
+            with torch.cuda.stream(side):
+                lam_a, st_a, na_a, nr_a = ctx.run(fd, idx_list=items, stream=side)
+        finally:
+            ctx.set_option("lane_gather", 0)
+            ctx.set_option("item_order", 0)
+        if not ctx.watch_wait(0, watch_timeout_s):
+            raise RuntimeError(f"the training lattice did not finish within {watch_timeout_s} s")
+        tm["train_map"] = time.perf_counter() - t
+        t = time.perf_counter()
+        lam_t, st_t = lam_a[:n_t], st_a[:n_t]
+        net, losses = _train(dev, fd, grid_params(fd, train_idx), lam_t, st_t, epochs, batch, lr, seed,
+                             hidden, fourier_sigma, max(batch, infer_batch))
+        t = mark_stream("train", t, main)
+        pred = _infer(net, fd, n, infer_batch, dev)
+        t = mark_stream("infer", t, main)
+        if not ctx.watch_wait(1, watch_timeout_s):
+            raise RuntimeError(f"the calibration points did not finish within {watch_timeout_s} s")
+        lam_c, st_c = lam_a[n_t:n_t + n_cal], st_a[n_t:n_t + n_cal]
+        kappa, escapes, cal_agree, cal_answered, cal_table, flagged = _rule(
+            pred, calib, lam_c, st_c, std_thr, lam_thr, bad_lo, bad_hi, calib_target, dlam_cal)
+        # the skip words of the points the rule answers (a kernel on the main
+        # stream; the launch reads them past the caches)
+        skip[n_first:] = (~flagged[rest]).to(torch.int32)
+        tm["calibrate"] = time.perf_counter() - t
+        t = time.perf_counter()
+        side.synchronize()
+        main.synchronize()
+        tm["rest"] = time.perf_counter() - t
+        launch_stats("one launch", ctx.kernel_ms(), st_a, na_a, nr_a)
+        if judge_ctx is not None:
+            judge_ctx.close()
+        lam_v, st_v = lam_a[n_t + n_cal:n_first], st_a[n_t + n_cal:n_first]
+        # the map: the method's values wherever it ran, then the rule's answers
+        # on the points outside the first three sets
+        lam = torch.full((n, fd.n_exps), float("nan"), dtype=torch.float64, device=dev)
+        status = torch.full((n,), L.PT_SKIPPED, dtype=torch.int32, device=dev)
+        lam[items] = lam_a
+        status[items] = st_a
+    else:
+        # The split schedule: the lattice (or a coarse training grid of its
+        # own) on the main stream with the calibration and verification points
+        # beside it on a side stream, each capped at split_blocks_per_cu
+        # 256-lane blocks per CU; the FP64 judge on the other side stream while
+        # the surrogate trains; then the rerun of the flagged points.
+        side_c = side_streams(dev)[0]
+        side_c.wait_stream(main)
+        cv = torch.cat([calib, verify]).contiguous()
+        cv_ctx = L.Context(ctx.device)
+        cv_ctx.set_option("lane_max_blocks", split_blocks_per_cu * n_cus)
+        if lattice:
             ctx.set_option("lane_max_blocks", split_blocks_per_cu * n_cus)
             try:
                 lam_t, st_t, na_t, nr_t = ctx.run(fd, idx_list=train_idx)
             finally:
                 ctx.set_option("lane_max_blocks", 0)
-            with torch.cuda.stream(side_c):
-                lam_cv, st_cv, na_cv, nr_cv = cv_ctx.run(fd, idx_list=cv, stream=side_c)
-            lam_f, st_f, na_f, nr_f = None, None, na_t, nr_t
+            p_t = grid_params(fd, train_idx)
         else:
-            # about one point per lane: the launch lasts as long as its slowest
-            # point, and lanes advance faster with fewer waves per SIMD (two
-            # 256-lane blocks per CU: 820 ms against 1,083 ms at three for
-            # 149,422 points of the 2048^2 Rossler plane)
-            ctx.set_option("lane_max_blocks", first_blocks_per_cu * n_cus)
-            try:
-                lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
-            finally:
-                ctx.set_option("lane_max_blocks", 0)
-            lam_t, st_t = lam_f[:n_t], st_f[:n_t]
-        p_t = grid_params(fd, train_idx)
-    else:
-        tspec = dict(fine)
-        tspec.update(W.grid_fields(fd.sweep_idx1, fd.start1, end1, train_n, fd.sweep_idx2, fd.start2,
-                                   end2, train_n))
-        td = L.finalize(L.make_desc(tspec))
-        lam_t, st_t, _, _ = ctx.run(td)
-        p_t = grid_params(td, torch.arange(n_t, device=dev))
-        first = torch.cat([calib, verify]).contiguous()
-        lam_f, st_f, na_f, nr_f = ctx.run(fd, idx_list=first)
-        lam_f = torch.cat([lam_t, lam_f])
-        st_f = torch.cat([st_t, st_f])
-    if side_c is None:
-        lam_c, st_c = lam_f[n_t:n_t + n_cal], st_f[n_t:n_t + n_cal]
-        lam_v, st_v = lam_f[n_t + n_cal:], st_f[n_t + n_cal:]
-    else:
+            tspec = dict(fine)
+            tspec.update(W.grid_fields(fd.sweep_idx1, fd.start1, end1, train_n, fd.sweep_idx2, fd.start2,
+                                       end2, train_n))
+            td = L.finalize(L.make_desc(tspec))
+            lam_t, st_t, na_t, nr_t = ctx.run(td)
+            p_t = grid_params(td, torch.arange(n_t, device=dev))
+        with torch.cuda.stream(side_c):
+            lam_cv, st_cv, na_cv, nr_cv = cv_ctx.run(fd, idx_list=cv, stream=side_c)
+        t = mark_stream("train_map", t, main)
+        launch_stats("training map", ctx.kernel_ms(), st_t, na_t, nr_t)
+        judge_ctx = None
+        if judge:
+            side_j = side_streams(dev)[1]
+            side_j.wait_stream(main)
+            judge_ctx = L.Context(ctx.device)
+            with torch.cuda.stream(side_j):
+                lam_j, st_j, _, _ = judge_ctx.run(_judge_desc(fd), idx_list=verify, stream=side_j)
+        net, losses = _train(dev, fd, p_t, lam_t, st_t, epochs, batch, lr, seed, hidden, fourier_sigma,
+                             max(batch, infer_batch))
+        t = mark_stream("train", t, main)
+        pred = _infer(net, fd, n, infer_batch, dev)
+        t = mark_stream("infer", t, main)
+        main.wait_stream(side_c)
         lam_c, st_c = lam_cv[:n_cal], st_cv[:n_cal]
         lam_v, st_v = lam_cv[n_cal:], st_cv[n_cal:]
-    t = mark_stream("train_map", t, main)
-    first_kernel_ms = ctx.kernel_ms()
-    vmask = torch.zeros(n, dtype=torch.bool, device=dev)
-    vmask[verify] = True
-    cmask = torch.zeros(n, dtype=torch.bool, device=dev)
-    cmask[calib] = True
-    tmask = torch.zeros(n, dtype=torch.bool, device=dev)
-    if lattice:
-        tmask[train_idx] = True
-
-    # the FP64 judge of the verification points, on a side stream with a
-    # context of its own while the surrogate trains (training's small kernels
-    # leave most of the GPU idle; beside the latency-bound first launch the
-    # judge slowed it by ~40 %, beside the rerun it cost the rerun ~0.3 s)
-    judge_ctx = None
-    if judge:
-        side_j = side_streams(dev)[1]
-        side_j.wait_stream(main)
-        judge_ctx = L.Context(ctx.device)
-        # judge_quad: four lanes per point shorten the judge alone (2048^2
-        # Rossler, 41,943 FP64 points: 713 ms against 836 ms) but its four
-        # times as many waves slow the training kernels beside it (training
-        # 0.49 -> 0.75 s), so it is off by default
-        judge_ctx.set_option("lane_quad", 1 if judge_quad else 0)
-        with torch.cuda.stream(side_j):
-            pdesc = L.SweepDesc.from_buffer_copy(fd)
-            pdesc.precision = L.PREC_FP64
-            # the judge is the plain method: FP64, every deviation vector
-            if pdesc.method == L.METHOD_VAR_TRACE:
-                pdesc.method = L.METHOD_VAR
-            lam_j, st_j, _, _ = judge_ctx.run(pdesc, idx_list=verify, stream=side_j)
-
-    # 1b. while the surrogate trains, a seeded subset of the other points
-    #     (overlap_frac of them, at most overlap_points) runs on a side stream
-    #     with a context of its own, its persistent grid capped at
-    #     overlap_blocks_per_cu 256-lane blocks per CU so the training kernels
-    #     keep room on every CU.  These points are computed whatever the
-    #     surrogate later says, so the final map does not depend on timing.
-    omask = torch.zeros(n, dtype=torch.bool, device=dev)
-    ov_ctx, ov_idx = None, None
-    rest = torch.nonzero(~(vmask | cmask | tmask)).flatten()
-    n_ov = min(int(overlap_points), int(overlap_frac * rest.numel()))
-    if n_ov > 0 and lattice:
-        go = torch.Generator(device="cpu").manual_seed(seed + 2)
-        pick = torch.randperm(int(rest.numel()), generator=go)[:n_ov].to(dev)
-        ov_idx = rest[pick].sort().values.contiguous()
-        omask[ov_idx] = True
-        ov_ctx = L.Context(ctx.device)
-        ov_ctx.set_option("lane_max_blocks", overlap_blocks_per_cu * n_cus)
-        ov_ctx.set_option("lane_gather", rerun_gather)
-        side_o = side_streams(dev)[2]
-        side_o.wait_stream(main)
-        with torch.cuda.stream(side_o):
-            lam_o, st_o, na_o, nr_o = ov_ctx.run(fd, idx_list=ov_idx, stream=side_o)
-
-    # 2. training (one shuffle per epoch; batches are contiguous slices)
-    ok = st_t == 0
-    lam_ok = lam_t[ok]
-    ym = lam_ok.mean(dim=0).tolist() if lam_ok.shape[0] else [0.0, 0.0, 0.0]
-    ys = lam_ok.std(dim=0).clamp_min(1e-6).tolist() if lam_ok.shape[0] > 1 else [1.0, 1.0, 1.0]
-    net = L.Mlp(device=ctx.device, max_batch=max(batch, infer_batch), hidden=hidden,
-                fourier_sigma=fourier_sigma, lr=lr, seed=seed,
-                ranges=(min(fd.start1, end1), max(fd.start1, end1), min(fd.start2, end2),
-                        max(fd.start2, end2)), y_mean=ym, y_std=ys)
-    gen = torch.Generator(device="cpu").manual_seed(seed)
-    loss = torch.zeros(1, dtype=torch.float32, device=dev)
-    losses = []
-    for ep in range(epochs):
-        perm = torch.randperm(n_t, generator=gen).to(dev)
-        pp, lp, sp = p_t[perm], lam_t[perm], st_t[perm]
-        for b0 in range(0, n_t - batch + 1, batch):
-            # the epoch's loss is its last batch's (no copy on the other steps)
-            last = b0 + 2 * batch > n_t
-            net.train_step(pp[b0:b0 + batch], lp[b0:b0 + batch], sp[b0:b0 + batch], loss if last else None)
-        losses.append(loss)
-        loss = torch.zeros(1, dtype=torch.float32, device=dev)
-    t = mark_stream("train", t, main)
-
-    # 3. inference
-    pred = torch.empty((n, 5), dtype=torch.float32, device=dev)
-    for c0 in range(0, n, infer_batch):
-        c1 = min(n, c0 + infer_batch)
-        net.forward(grid_params(fd, torch.arange(c0, c1, device=dev)), pred[c0:c1])
-    t = mark_stream("infer", t, main)
-
-    # 4. calibration: the std multiplier at which the surrogate's answers on the
-    #    calibration subsample agree with the method (the verification
-    #    subsample stays out of it), then the rerun set: every point the
-    #    surrogate does not answer and that step 1 did not compute
-    if side_c is not None:
-        main.wait_stream(side_c)
-    mu1, std1, pbad = pred[:, 0], pred[:, 3], pred[:, 4]
-
-    def plane_valued(k, esc):
-        a = answerable(mu1, std1, pbad, k, std_thr, lam_thr, bad_lo, bad_hi, esc)
-        return int((a & torch.isfinite(mu1) & (pbad < 0.5)).sum().item())
-
-    kappa, escapes, cal_agree, cal_answered, cal_table = calibrate_kappa(
-        pred[calib].cpu().numpy(), lam_c.cpu().numpy(), st_c.cpu().numpy(), std_thr, lam_thr,
-        bad_lo, bad_hi, calib_target, dlam_cal, plane_valued=plane_valued)
-    tm["calibrate"] = time.perf_counter() - t
-    if cal_answered == 0:
-        flagged = torch.ones(n, dtype=torch.bool, device=dev)
-    else:
-        flagged = ~answerable(mu1, std1, pbad, kappa, std_thr, lam_thr, bad_lo, bad_hi, escapes)
-        flagged |= ~torch.isfinite(mu1)
-    rerun_idx = torch.nonzero(flagged & ~vmask & ~cmask & ~tmask & ~omask).flatten().to(torch.int64).contiguous()
-    ordered = longest_first and lattice and rerun_idx.numel() > 0
-    if ordered:
-        # longest first: each point's expected cost is the step count of its
-        # nearest lattice point, and the kernel takes the list in order, so the
-        # launch ends on short points instead of waiting for a long one
-        cost_t = (na_f[:n_t].to(torch.int64) + nr_f[:n_t].to(torch.int64))
-        j1 = torch.clamp(torch.round(torch.div(rerun_idx, fd.n2, rounding_mode="floor") / s1), 0, train_n - 1)
-        j2 = torch.clamp(torch.round(torch.remainder(rerun_idx, fd.n2) / s2), 0, train_n - 1)
-        cost = cost_t[(j1 * train_n + j2).to(torch.int64)]
-        rerun_idx = rerun_idx[torch.sort(cost, descending=True, stable=True).indices].contiguous()
-    elif rerun_idx.numel() > 0:
-        # the kernel's spread order applied to the list itself, taken in order
-        rerun_idx = rerun_idx[spread_order(rerun_idx.numel(), dev)].contiguous()
-    ctx.set_option("item_order", 1)
-    # the rerun keeps every lane busy for many points: lanes at a
-    # renormalisation wait until rerun_gather lanes of the wave do (same bits)
-    ctx.set_option("lane_gather", rerun_gather)
-    try:
-        lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
-    finally:
-        ctx.set_option("lane_gather", 0)
-        ctx.set_option("item_order", 0)
-    t = mark_stream("rerun", t, main)
-    rerun_kernel_ms = ctx.kernel_ms() if rerun_idx.numel() else 0.0
-    torch.cuda.synchronize(dev)
-    if judge_ctx is not None:
-        judge_ctx.close()
-    split_kernel_ms = 0.0
-    if cv_ctx is not None:
-        split_kernel_ms = cv_ctx.kernel_ms()
+        kappa, escapes, cal_agree, cal_answered, cal_table, flagged = _rule(
+            pred, calib, lam_c, st_c, std_thr, lam_thr, bad_lo, bad_hi, calib_target, dlam_cal)
+        tm["calibrate"] = time.perf_counter() - t
+        rerun_idx = rest[flagged[rest]].contiguous()
+        ctx.set_option("item_order", 1)
+        ctx.set_option("lane_gather", rerun_gather)
+        try:
+            lam_r, st_r, na_r, nr_r = ctx.run(fd, idx_list=rerun_idx)
+        finally:
+            ctx.set_option("lane_gather", 0)
+            ctx.set_option("item_order", 0)
+        t = mark_stream("rest", t, main)
+        torch.cuda.synchronize(dev)
+        launch_stats("calibration and verification", cv_ctx.kernel_ms(), st_cv, na_cv, nr_cv)
+        if rerun_idx.numel():
+            launch_stats("rerun", ctx.kernel_ms(), st_r, na_r, nr_r)
         cv_ctx.close()
-        na_f, nr_f = torch.cat([na_t, na_cv]), torch.cat([nr_t, nr_cv])
-    overlap_kernel_ms = 0.0
-    if ov_ctx is not None:
-        overlap_kernel_ms = ov_ctx.kernel_ms()
-        ov_ctx.close()
+        if judge_ctx is not None:
+            judge_ctx.close()
+        lam = torch.full((n, fd.n_exps), float("nan"), dtype=torch.float64, device=dev)
+        status = torch.full((n,), L.PT_SKIPPED, dtype=torch.int32, device=dev)
+        lam[rerun_idx] = lam_r
+        status[rerun_idx] = st_r
+        lam[cv] = torch.cat([lam_c, lam_v])
+        status[cv] = torch.cat([st_c, st_v])
+        if lattice:
+            lam[train_idx] = lam_t
+            status[train_idx] = st_t
 
-    # 5. assemble
-    lam = pred[:, :3].to(torch.float64)
-    status = torch.where(pbad >= 0.5, L.PT_ESCAPED, L.PT_OK).to(torch.int32)
-    lam[pbad >= 0.5] = float("nan")
-    lam[rerun_idx] = lam_r
-    status[rerun_idx] = st_r
-    lam[verify] = lam_v
-    status[verify] = st_v
-    lam[calib] = lam_c
-    status[calib] = st_c
-    if lattice:
-        lam[train_idx] = lam_t
-        status[train_idx] = st_t
-    if ov_idx is not None:
-        lam[ov_idx] = lam_o
-        status[ov_idx] = st_o
-    rerun_mask = flagged | vmask | cmask | tmask | omask
+    # the rule's answers on the points outside the first three sets (the
+    # surrogate's mu and its escape class); the method's values elsewhere
+    answered = ~flagged & ~first_mask
+    pbad = pred[:, 4]
+    sur_lam = pred[:, :3].to(torch.float64)
+    sur_lam[pbad >= 0.5] = float("nan")
+    sur_status = torch.where(pbad >= 0.5, L.PT_ESCAPED, L.PT_OK).to(torch.int32)
+    computed = status != L.PT_SKIPPED
+    speculative = int((answered & computed).sum().item())
+    lam[answered] = sur_lam[answered]
+    status[answered] = sur_status[answered]
+    rerun_mask = ~answered
     tm["total_chosen"] = time.perf_counter() - t_all
 
-    n_rerun = int(rerun_mask.sum().item())  # distinct points the method computed
+    n_rerun = int(rerun_mask.sum().item())  # points whose value is the method's
     stats = dict(points=n, train_points=n_t, rerun=n_rerun,
                  flagged=int(flagged.sum().item()), verify=n_ver, calib=n_cal,
                  kappa=kappa, escape_answers=bool(escapes), calib_agreement=cal_agree,
                  calib_answered=cal_answered, calib_table=cal_table,
-                 train_on_lattice=bool(lattice),
+                 train_on_lattice=bool(lattice), one_launch=launches[0]["name"] == "one launch",
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
-                 rerun_kernel_ms=rerun_kernel_ms, rerun_points=int(rerun_idx.numel()),
-                 rerun_longest_first=bool(ordered), rerun_gather=int(rerun_gather),
-                 overlap_points=int(ov_idx.numel()) if ov_idx is not None else 0,
-                 overlap_kernel_ms=overlap_kernel_ms,
-                 overlap_acc_steps=float(na_o.double().sum().item()) if ov_idx is not None else 0.0,
-                 overlap_rej_steps=float(nr_o.double().sum().item()) if ov_idx is not None else 0.0,
-                 first_kernel_ms=first_kernel_ms, first_points=int(first.numel()),
-                 split_first=cv_ctx is not None, split_kernel_ms=split_kernel_ms,
-                 first_acc_steps=float(na_f.double().sum().item()),
-                 first_rej_steps=float(nr_f.double().sum().item()),
-                 rerun_acc_steps=float(na_r.double().sum().item()),
-                 rerun_rej_steps=float(nr_r.double().sum().item()))
+                 computed_points=int(computed.sum().item()), speculative_points=speculative,
+                 rerun_gather=int(rerun_gather), launches=launches)
     if judge:
-        v = verify.cpu().numpy()
-        lj, sj = lam_j.cpu().numpy(), st_j.cpu().numpy()
-        lc, sc = lam_v.cpu().numpy(), st_v.cpu().numpy()
         stats.update(verify_gates(flagged[verify].cpu().numpy(), pred[verify].cpu().numpy(),
                                   lam_v.cpu().numpy(), st_v.cpu().numpy(), lam_j.cpu().numpy(),
                                   st_j.cpu().numpy()))

@@ -78,7 +78,8 @@ enum {
     LYAP_PT_NONFINITE = 1,
     LYAP_PT_STEP_UNDERFLOW = 2,
     LYAP_PT_MAXSTEPS = 3,
-    LYAP_PT_ESCAPED = 4
+    LYAP_PT_ESCAPED = 4,
+    LYAP_PT_SKIPPED = 5 /* not integrated: its skip word was set (lyap_ctx_set_watch) */
 };
 
 /* One sweep.  Field meanings follow LyapunovParameters (tool_params.h:29-43);
@@ -186,6 +187,31 @@ int lyap_ctx_set_option(lyap_ctx* ctx, const char* key, int64_t value);
 /* duration of the last sweep kernel launched by lyap_sweep_run on ctx
    (HIP events on the launch stream; valid once that stream has drained) */
 int lyap_last_kernel_ms(lyap_ctx* ctx, float* ms);
+
+/*
+ * Progress signals and skips for the next lyap_sweep_run on ctx (that run
+ * only).  No counterpart in the reference, whose launches were synchronous
+ * (tasklyapunov.cpp:200-215); it lets a caller consume the first results of a
+ * long launch while it runs, and drop points it no longer needs.  The run must
+ * be the sweep's method (LYAP_METHOD_VAR or _VAR_TRACE) with adaptive steps on
+ * a 3-D flow, one lane per point (not lane_quad).  Items are work items (list positions;
+ * item_order 1 takes them in that order):
+ *   items [0, end0) form milestone 0 and [end0, end1) milestone 1; once every
+ *       item of a milestone has finished, its outputs are in device memory and
+ *       lyap_ctx_watch_wait reports the milestone reached, while the launch
+ *       goes on (end1 <= the run's count);
+ *   d_skip (optional, [count] words, device memory): an item whose word is
+ *       nonzero when a lane takes it is not integrated (status
+ *       LYAP_PT_SKIPPED, exponents NaN, step counts 0); words written while
+ *       the launch runs (by a kernel on another stream) are seen by the items
+ *       taken after they land.
+ */
+int lyap_ctx_set_watch(lyap_ctx* ctx, int64_t end0, int64_t end1, const uint32_t* d_skip);
+
+/* Waits on the host, up to timeout_s seconds, until milestone 0 or 1 of the
+   last watched run on ctx is reached; *reached = 1 if it was (an empty
+   milestone is reached at once), 0 on time-out. */
+int lyap_ctx_watch_wait(lyap_ctx* ctx, int milestone, double timeout_s, int* reached);
 
 /*
  * Reference-shaped launch: the arguments of cudaLyapunov

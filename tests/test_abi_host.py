@@ -196,3 +196,22 @@ def test_noalloc_and_reference_entry_reject_before_touching_a_device():
 def test_ref_option_names_are_checked():
     lib = L.load()
     assert lib.lyap_ctx_set_option(None, b"no_such_option", 1) == L.ERR_INVALID
+
+
+def test_watch_entry_points_reject_bad_arguments():
+    """lyap_ctx_set_watch and lyap_ctx_watch_wait refuse a NULL context, bad
+    milestone ranges and bad milestone numbers before any device call; the
+    Python wrapper checks the ranges too (no GPU needed)."""
+    lib = L.load()
+    assert lib.lyap_ctx_set_watch(None, 1, 2, None) == L.ERR_INVALID
+    r = ctypes.c_int(7)
+    assert lib.lyap_ctx_watch_wait(None, 0, 1.0, ctypes.byref(r)) == L.ERR_INVALID
+    assert lib.lyap_ctx_watch_wait(None, 2, 1.0, ctypes.byref(r)) == L.ERR_INVALID
+    assert lib.lyap_ctx_watch_wait(None, 0, 1.0, None) == L.ERR_INVALID
+    ctx = L.Context.__new__(L.Context)  # no device: only the argument checks run
+    ctx._h = ctypes.c_void_p()
+    with pytest.raises(ValueError, match="not ranges"):
+        ctx.set_watch(5, 3)
+    with pytest.raises(ValueError, match="not ranges"):
+        ctx.set_watch(-1, 3)
+    assert L.PT_SKIPPED == 5

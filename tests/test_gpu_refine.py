@@ -23,25 +23,22 @@ pytestmark = pytest.mark.gpu
 def test_refinement_gain_and_agreement(gpu_ctx):
     spec = W.rossler_dverk(256, precision=L.PREC_MIXED, t_transient=50.0, t_average=500.0)
     print("case refine rossler 256x256 short times", flush=True)
-    # with the optional launch that overlaps training (a seeded 1/16 of the rest)
-    res = R.refine(gpu_ctx, spec, train_n=64, epochs=300, batch=1024, infer_batch=65536,
-                   overlap_points=1 << 18)
+    # the default schedule: one watched launch over the plane
+    res = R.refine(gpu_ctx, spec, train_n=64, epochs=300, batch=1024, infer_batch=65536)
     st = res.stats
     print({k: v for k, v in st.items() if k != "losses"}, flush=True)
     print("timings", res.timings, flush=True)
     # the gain: points the surrogate answered, with no integration
     answered = 1.0 - st["rerun_fraction"]
     print(f"surrogate answered {answered:.3%} of the plane (kappa {st['kappa']})", flush=True)
-    assert st["train_on_lattice"]
-    # the launch that overlaps training took its seeded 1/16 of the points that
-    # the first launch did not compute (the lattice may share a few points with
-    # the calibration and verification subsamples)
+    assert st["train_on_lattice"] and st["one_launch"]
+    # the map's method values are the points the rule leaves to the method and
+    # the three first sets; the launch also computed the points it took before
+    # the rule was known (speculative), whose map values are the rule's answers
     first = st["train_points"] + st["verify"] + st["calib"]
-    assert int((st["points"] - first) / 16) <= st["overlap_points"] <= int((st["points"] - st["train_points"]) / 16)
-    assert st["overlap_points"] > 0
     assert int(res.rerun.sum()) == st["rerun"]
-    assert st["train_points"] + st["overlap_points"] + st["rerun_points"] <= st["rerun"]
-    assert st["rerun"] <= first + st["overlap_points"] + st["rerun_points"]
+    assert st["train_points"] <= st["rerun"] and st["rerun"] <= first + st["flagged"]
+    assert st["computed_points"] == st["rerun"] + st["speculative_points"]
     assert st["losses"][-1] < st["losses"][0]
     # the gates, with the surrogate's answers in: |dlambda1| and classes
     assert st["max_dlambda1_chosen_vs_fp64"] <= R.DLAMBDA1_GATE
