@@ -265,7 +265,9 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
         const bool want = !active && !exhausted;
         int64_t got = grab_items(want && u == 0, io.counter);
         got = __shfl(got, lead);
-        if (want) {
+        // taking and finishing a point are rare next to a step (branch hints
+        // keep the register allocator's spill code out of the step loop)
+        if (__builtin_expect(want, 0)) {
             if (got < io.count) {
                 const auto dp = getd();
                 pos = item_position(io, got);
@@ -574,7 +576,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 }
             }
         }
-        if (finished) {
+        if (__builtin_expect(finished, 0)) {
             if (u < n_exps) io.lambda[pos * n_exps + u] = (st == LYAP_PT_OK) ? sum / t_avg : __builtin_nan("");
             if (u == 0) {
                 if (io.status) io.status[pos] = st;

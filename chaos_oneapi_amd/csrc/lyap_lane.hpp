@@ -564,7 +564,7 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
         const bool want = !active && !exhausted;
         int64_t got = grab_items(want && ql == 0, counter);
         got = __shfl(got, lead);
-        if (want) {
+        if (__builtin_expect(want, 0)) {
             if (got < count) {
                 const auto dp = getd();
                 const auto iop = getio();
@@ -726,7 +726,7 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                 }
             }
         }
-        if (finished) {
+        if (__builtin_expect(finished, 0)) {
             write_out(st);
             active = false;
         }
@@ -758,7 +758,7 @@ LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io)
 
     for (;;) {
         const int64_t got = grab_items(!active && !exhausted, io.counter);
-        if (!active && !exhausted) {
+        if (__builtin_expect(!active && !exhausted, 0)) {
             if (got < io.count) {
                 const auto dp = getd();
                 item = item_position(io, got);
@@ -827,7 +827,7 @@ LYAP_D void map_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io)
             }
             if (m == m_total) finished = true;
         }
-        if (finished) {
+        if (__builtin_expect(finished, 0)) {
             for (int r = 0; r < n_exps; ++r) {
                 double s = 0.0;
 #pragma unroll
