@@ -48,6 +48,21 @@ LYAP_HD double sconst(double v) {
 }
 LYAP_HD float sconst(float v) { return v; }
 
+// fma(a, b, c) for a binary64 constant c: one v_fma_f64 with c in an SGPR pair
+// materialised at the use site.  Left to itself the compiler copies c into
+// the destination VGPRs and issues v_fmac_f64: three VALU instructions for
+// one.  The same rounding as fma(), so host and device agree bit for bit.
+LYAP_HD double fma_c(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double cs = sconst(c);
+    double r;
+    __asm__("v_fma_f64 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "s"(cs));
+    return r;
+#else
+    return fma(a, b, c);
+#endif
+}
+
 constexpr double kLn2Hi = 6.93147180369123816490e-01;
 constexpr double kLn2Lo = 1.90821492927058770002e-10;
 constexpr double kInf = __builtin_huge_val();
@@ -78,17 +93,17 @@ LYAP_HD double det_log(double x) {
     const double s = f / (2.0 + f);
     const double z = s * s;
     double r = sconst(2.0 / 25.0);
-    r = fma(r, z, sconst(2.0 / 23.0));
-    r = fma(r, z, sconst(2.0 / 21.0));
-    r = fma(r, z, sconst(2.0 / 19.0));
-    r = fma(r, z, sconst(2.0 / 17.0));
-    r = fma(r, z, sconst(2.0 / 15.0));
-    r = fma(r, z, sconst(2.0 / 13.0));
-    r = fma(r, z, sconst(2.0 / 11.0));
-    r = fma(r, z, sconst(2.0 / 9.0));
-    r = fma(r, z, sconst(2.0 / 7.0));
-    r = fma(r, z, sconst(2.0 / 5.0));
-    r = fma(r, z, sconst(2.0 / 3.0));
+    r = fma_c(r, z, 2.0 / 23.0);
+    r = fma_c(r, z, 2.0 / 21.0);
+    r = fma_c(r, z, 2.0 / 19.0);
+    r = fma_c(r, z, 2.0 / 17.0);
+    r = fma_c(r, z, 2.0 / 15.0);
+    r = fma_c(r, z, 2.0 / 13.0);
+    r = fma_c(r, z, 2.0 / 11.0);
+    r = fma_c(r, z, 2.0 / 9.0);
+    r = fma_c(r, z, 2.0 / 7.0);
+    r = fma_c(r, z, 2.0 / 5.0);
+    r = fma_c(r, z, 2.0 / 3.0);
     r = r * z;
     const double lg = f - s * (f - r);
     const double dk = (double)k;
@@ -104,19 +119,19 @@ LYAP_HD double det_exp(double x) {
     kd = (kd >= 0.0) ? floor(kd + 0.5) : ceil(kd - 0.5);
     const double r = fma(-kd, sconst(kLn2Lo), fma(-kd, sconst(kLn2Hi), x));
     double p = sconst(1.0 / 6227020800.0);
-    p = fma(p, r, sconst(1.0 / 479001600.0));
-    p = fma(p, r, sconst(1.0 / 39916800.0));
-    p = fma(p, r, sconst(1.0 / 3628800.0));
-    p = fma(p, r, sconst(1.0 / 362880.0));
-    p = fma(p, r, sconst(1.0 / 40320.0));
-    p = fma(p, r, sconst(1.0 / 5040.0));
-    p = fma(p, r, sconst(1.0 / 720.0));
-    p = fma(p, r, sconst(1.0 / 120.0));
-    p = fma(p, r, sconst(1.0 / 24.0));
-    p = fma(p, r, sconst(1.0 / 6.0));
-    p = fma(p, r, sconst(0.5));
-    p = fma(p, r, sconst(1.0));
-    p = fma(p, r, sconst(1.0));
+    p = fma_c(p, r, 1.0 / 479001600.0);
+    p = fma_c(p, r, 1.0 / 39916800.0);
+    p = fma_c(p, r, 1.0 / 3628800.0);
+    p = fma_c(p, r, 1.0 / 362880.0);
+    p = fma_c(p, r, 1.0 / 40320.0);
+    p = fma_c(p, r, 1.0 / 5040.0);
+    p = fma_c(p, r, 1.0 / 720.0);
+    p = fma_c(p, r, 1.0 / 120.0);
+    p = fma_c(p, r, 1.0 / 24.0);
+    p = fma_c(p, r, 1.0 / 6.0);
+    p = fma_c(p, r, 0.5);
+    p = fma_c(p, r, 1.0);
+    p = fma_c(p, r, 1.0);
     int k = (int)kd;
     if (k > 1023) {
         p = p * double_of((uint64_t)(1023 + 1023) << 52);
@@ -494,10 +509,10 @@ LYAP_HD double ctrl_log(double x) {
     const double s = f / (2.0 + f);
     const double z = s * s;
     double r = sconst(2.0 / 11.0);
-    r = fma(r, z, sconst(2.0 / 9.0));
-    r = fma(r, z, sconst(2.0 / 7.0));
-    r = fma(r, z, sconst(2.0 / 5.0));
-    r = fma(r, z, sconst(2.0 / 3.0));
+    r = fma_c(r, z, 2.0 / 9.0);
+    r = fma_c(r, z, 2.0 / 7.0);
+    r = fma_c(r, z, 2.0 / 5.0);
+    r = fma_c(r, z, 2.0 / 3.0);
     r = r * z;
     const double lg = f - s * (f - r);
     const double dk = (double)k;
@@ -505,16 +520,15 @@ LYAP_HD double ctrl_log(double x) {
 }
 
 LYAP_HD double ctrl_exp(double x) {
-    x = x < -1.6 ? -1.6 : x;
-    x = x > 2.5 ? 2.5 : x;
+    x = fmin(fmax(x, -1.6), 2.5);  // never NaN here (see ctrl_update)
     const double kd = floor(x * 1.4426950408889634 + 0.5);
     const double r = fma(-kd, sconst(kLn2Lo), fma(-kd, sconst(kLn2Hi), x));
     double p = sconst(1.0 / 40320.0);
-    p = fma(p, r, sconst(1.0 / 5040.0));
-    p = fma(p, r, sconst(1.0 / 720.0));
-    p = fma(p, r, sconst(1.0 / 120.0));
-    p = fma(p, r, sconst(1.0 / 24.0));
-    p = fma(p, r, sconst(1.0 / 6.0));
+    p = fma_c(p, r, 1.0 / 5040.0);
+    p = fma_c(p, r, 1.0 / 720.0);
+    p = fma_c(p, r, 1.0 / 120.0);
+    p = fma_c(p, r, 1.0 / 24.0);
+    p = fma_c(p, r, 1.0 / 6.0);
     p = fma(p, r, 0.5);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
@@ -540,14 +554,16 @@ LYAP_D bool ctrl_update(Ctrl& c, bool ok, bool above_floor, double lerr, double 
     constexpr double kSafe = 0.9, kBeta = 0.04;
     const double e_rej = Tab<INTEG>::expo * lerr;
     const double e = ok ? fma(-kBeta, c.log_fac_old, e_rej) : e_rej;
+    // e is finite or -inf (err = 0), never NaN; r, h_try and hn are finite
+    // and positive, so min/max (one instruction each) give the clamps' bits
     double r = kSafe * ctrl_exp(-e);
-    r = r < 0.2 ? 0.2 : r;
+    r = fmax(r, 0.2);
     if (ok) {
-        r = r > 10.0 ? 10.0 : r;
+        r = fmin(r, 10.0);
         double hn = h_try * r;
         c.log_fac_old = above_floor ? lerr : log_floor;
-        if (fabs(hn) > h_max) hn = h_max;
-        if (c.rejected) hn = fabs(hn) < fabs(h_try) ? fabs(hn) : fabs(h_try);
+        hn = fmin(hn, h_max);
+        if (c.rejected) hn = fmin(hn, h_try);
         c.rejected = false;
         h_next = hn;
         return true;
@@ -603,9 +619,8 @@ LYAP_D double init_component(const D& d, int64_t idx, int i, const double* p) {
 LYAP_HD double inv_sqrt_det(double s) {
     double y = double_of(0x5FE6EB50C7B537A9ULL - (bits_of(s) >> 1));
     const double hs = 0.5 * s;
-    const double c15 = sconst(1.5);
 #pragma unroll
-    for (int it = 0; it < 4; ++it) y = y * fma(-hs, y * y, c15);
+    for (int it = 0; it < 4; ++it) y = y * fma_c(-hs, y * y, 1.5);
     return y;
 }
 
