@@ -208,7 +208,13 @@ LYAP_D void watch_count(W* w, int64_t item) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     const unsigned long long c =
         __hip_atomic_fetch_add(w->done + k, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if ((int64_t)c + 1 == size) __hip_atomic_store(w->flag + k, w->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    if ((int64_t)c + 1 == size) {
+        // the other lanes' release fences synchronise with this acquire fence
+        // through the counter's increments, so their outputs are ordered before
+        // the flag the host reads
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __hip_atomic_store(w->flag + k, w->seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // Item `item` of a watched launch is skipped: its outputs say so (status
