@@ -198,16 +198,22 @@ def set_lane_shape(ctx, shape):
         ctx.set_option(key, int(v))
 
 
-def plain_fp64(ctx, desc, lane_shape=None):
-    """One plain FP64 sweep of the grid: (lam, status, kernel ms, flops);
-    lane_shape: the 3-D flows' launch shape to run it with (its own tuning)."""
+def plain_fp64(ctx, desc, lane_shape=None, resident=None):
+    """One plain FP64 sweep of the grid: (lam, status, ms, flops, order tuning);
+    lane_shape: the 3-D flows' launch shape to run it with (its own tuning);
+    resident: the launch's resident points, to tune its own heavy-first order
+    (None: the order set on ctx)."""
     import torch
 
+    from chaos_oneapi_amd import autotune as A
     from chaos_oneapi_amd import flops as F
 
     plain = plain_desc_of(desc)
     if lane_shape is not None:
         set_lane_shape(ctx, lane_shape)
+    order = None
+    if resident is not None:
+        order = A.tune_order(ctx, plain, resident, cache=A.Cache(None))
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
@@ -216,7 +222,7 @@ def plain_fp64(ctx, desc, lane_shape=None):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     return (lam.cpu().numpy(), st.cpu().numpy(), ms,
-            F.sweep_flops(plain, na.cpu().numpy(), nr.cpu().numpy()))
+            F.sweep_flops(plain, na.cpu().numpy(), nr.cpu().numpy()), order)
 
 
 def bench_sweep(args, ctx, spec, wl_name):
@@ -242,6 +248,12 @@ def bench_sweep(args, ctx, spec, wl_name):
         # persistent grid cap and renormalisation gather, set on ctx for the
         # chosen form and the plain FP64 sweep alike
         tuned_lanes = A.tune_lanes(ctx, desc, cache=A.Cache(None), time_scale=0.1)
+    # heavy-first order of adaptive grids that start their points in a few
+    # rounds of the launch, timed on full runs (probe, sort and scatter included)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    resident = A.resident_points(desc, cus, int(tuned.best[1]) if tuned is not None and tuned.best[0] == "ppw"
+                                 else 0)
+    tuned_order = None if args.no_tune else A.tune_order(ctx, desc, resident, cache=A.Cache(None))
     stream = torch.cuda.current_stream()
     dev = torch.device("cuda", 0)
     bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
@@ -252,14 +264,16 @@ def bench_sweep(args, ctx, spec, wl_name):
     torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
           for _ in range(args.steps)]
+    kms = []
     t0 = time.perf_counter()
     for i in range(args.steps):
         ev[i][0].record(stream)
         ctx.run(desc, out=bufs)
         ev[i][1].record(stream)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        kms.append(ctx.kernel_ms())  # the sweep's own launch (an ordered run's probe excluded)
     wall = time.perf_counter() - t0
-    kern_avg = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    kern_avg = sum(kms) / args.steps
     ms_per_step = wall * 1e3 / args.steps
     lam_c, st_c = bufs[0].cpu().numpy().copy(), bufs[1].cpu().numpy().copy()
     na_c, nr_c = bufs[2].cpu().numpy().copy(), bufs[3].cpu().numpy().copy()
@@ -280,8 +294,8 @@ def bench_sweep(args, ctx, spec, wl_name):
     if tuned_lanes is not None:
         tuned_plain = A.tune_lanes(ctx, plain_desc_of(desc), cache=A.Cache(None), time_scale=0.1,
                                    apply=False)
-    lam_p, st_p, plain_ms, flops_p = plain_fp64(
-        ctx, desc, None if tuned_plain is None else tuned_plain.best)
+    lam_p, st_p, plain_ms, flops_p, order_plain = plain_fp64(
+        ctx, desc, None if tuned_plain is None else tuned_plain.best, None if args.no_tune else resident)
     ok = (st_c == 0) & (st_p == 0)
     d1 = float(np.max(np.abs(lam_c[ok, 0] - lam_p[ok, 0]))) if ok.any() else 0.0
     cls = float(np.mean((lam_c[ok, 0] > CHAOS_THRESHOLD) == (lam_p[ok, 0] > CHAOS_THRESHOLD)))
@@ -312,6 +326,12 @@ def bench_sweep(args, ctx, spec, wl_name):
         rec["config"]["plain_fp64_lane_shape"] = {"lane_max_blocks": int(tuned_plain.best[0]),
                                                   "lane_gather": int(tuned_plain.best[1]),
                                                   "lane_quad": int(tuned_plain.best[2])}
+    if tuned_order is not None:
+        rec["config"]["order_probe"] = int(tuned_order.best[0])
+        rec["config"]["order_tuning_ms"] = {str(k[0]): round(v, 1) for k, v in tuned_order.timings.items()}
+        rec["config"]["plain_fp64_order_probe"] = int(order_plain.best[0])
+        rec["config"]["plain_fp64_order_tuning_ms"] = {str(k[0]): round(v, 1)
+                                                       for k, v in order_plain.timings.items()}
     if tuned is not None:
         rec["config"]["l96_tile"] = list(tuned.best)
         rec["config"]["l96_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned.timings.items()}
@@ -392,7 +412,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
         print(f"step {k}: {times[-1]:.3f} s, phases {res.timings}", file=sys.stderr, flush=True)
     ms_per_step = 1e3 * sum(times) / len(times)
     st = res.stats
-    lam_p, st_p, plain_ms, flops_p = plain_fp64(ctx, desc)
+    lam_p, st_p, plain_ms, flops_p, _ = plain_fp64(ctx, desc)
     print(f"plain FP64 sweep {plain_ms:.0f} ms", file=sys.stderr, flush=True)
     # the mixed form alone over the whole plane (no surrogate), with the
     # rerun's renormalisation gather: what the surrogate adds to the mixed form

@@ -8,8 +8,10 @@ Knobs, all exposed by liblyap.so:
   - single Lorenz-96 sweeps (lyap_ctx_set_option): the LDS tile shape, i.e.
     one point per block of W waves (l96_waves_N) or P points per wave
     (l96_ppw_N);
-  - single 3-D sweeps: the persistent grid size (lane_max_blocks) and the
-    renormalisation gather (lane_gather).
+  - single 3-D sweeps: the persistent grid size (lane_max_blocks), the
+    renormalisation gather (lane_gather) and four lanes per point (lane_quad);
+  - adaptive single sweeps: heavy-first order (order_probe), timed on full
+    runs of the grid (tune_order).
 
 Each candidate is timed on a sample of the workload — a seeded subset of the
 points with the full integration times (per-point costs keep their
@@ -313,4 +315,69 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
         ctx.set_option("lane_max_blocks", int(res.best[0]))
         ctx.set_option("lane_gather", int(res.best[1]))
         ctx.set_option("lane_quad", int(res.best[2]))
+    return res
+
+
+ORDER_CANDIDATES = (0, 10, 30)
+
+
+def order_candidates(desc, n_points: int, resident_points: int, max_rounds: int = 8) -> tuple:
+    """order_probe values worth timing for a sweep: heavy-first order pays where
+    an adaptive grid's points start in a few rounds of the launch's resident
+    points (a heavy point of a later round sets the launch's length); with more
+    rounds than max_rounds, or fixed steps (every point alike), it stays off."""
+    if not desc.adaptive or n_points > max_rounds * max(1, resident_points):
+        return (0,)
+    return ORDER_CANDIDATES
+
+
+def resident_points(desc, cus: int, l96_ppw: int = 0, waves_per_simd: int = 3) -> int:
+    """Points a persistent launch of the sweep holds at once (about): one per
+    lane for the 3-D flows and maps, l96_ppw per wave for the Lorenz-96 wave
+    drain, one per block of waves otherwise."""
+    waves = cus * 4 * waves_per_simd
+    if desc.model == L.MODEL_L96:
+        return waves * l96_ppw if l96_ppw > 0 else cus * waves_per_simd
+    return waves * 64
+
+
+def tune_order(ctx, desc, resident: int, cache: Cache | None = None, force: bool = False,
+               apply: bool = True, timer=None) -> TuneResult:
+    """Pick order_probe for a sweep over desc's whole grid (and, apply=True, set
+    it on ctx): each candidate is one full run timed on the stream, probe, sort
+    and scatter included — the gain is the schedule of the whole launch, which
+    a shortened or sampled run does not show."""
+    import torch
+
+    cache = cache if cache is not None else Cache()
+    n_total = desc.n1 * desc.n2
+    cands = order_candidates(desc, n_total, resident)
+    name = device_name(ctx.device) if timer is None else "test"
+    key = f"order|{name}|{desc_signature(desc)}x{_bucket(n_total)}r{_bucket(resident)}"
+    hit = cache.get(key)
+    if len(cands) == 1:
+        res = TuneResult(best=(cands[0],), timings={})
+    elif hit and not force:
+        res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
+    else:
+        timings = {}
+        for c in cands:
+            if timer:
+                timings[(c,)] = timer(c)
+                continue
+            ctx.set_option("order_probe", c)
+            try:
+                stream = torch.cuda.current_stream()
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                ctx.run(desc)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                timings[(c,)] = e0.elapsed_time(e1)
+            finally:
+                ctx.set_option("order_probe", 0)
+        res = TuneResult(best=pick_best(timings), timings=timings)
+        cache.put(key, res.best, timings)
+    if apply and timer is None:
+        ctx.set_option("order_probe", int(res.best[0]))
     return res

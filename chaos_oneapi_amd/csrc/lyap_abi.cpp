@@ -123,7 +123,11 @@ struct lyap_ctx {
     int lane_gather = 0;
     int item_order = 0;  // 0: items spread over the grid; 1: in list order
     int lane_quad = 0;   // 1: the sweep's method on the 3-D flows with four lanes per point
+    int order_probe = 0; // > 0: adaptive sweeps run heavy-first, ordered by a probe run over
+                         // this many thousandths of the integration times
     unsigned long long* counter = nullptr;
+    void* order_buf = nullptr;  // staging, keys, index lists and sort scratch of ordered runs
+    size_t order_cap = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool timed = false;
     // progress signals and skips of the next run (lyap_ctx_set_watch): device
@@ -178,6 +182,7 @@ int lyap_ctx_destroy(lyap_ctx* c) {
     if (c->counter) hipFree(c->counter);
     if (c->watch_done) hipFree(c->watch_done);
     if (c->watch_flag) hipHostFree(c->watch_flag);
+    if (c->order_buf) hipFree(c->order_buf);
     delete c;
     return LYAP_SUCCESS;
 }
@@ -309,6 +314,48 @@ int lyap_desc_finalize(lyap_sweep_desc* d) {
 
 namespace {
 
+// device buffers of a heavy-first run, carved from the context's order_buf
+struct OrderBufs {
+    double* lam;
+    uint32_t *st, *na, *nr, *keys_in, *keys_out;
+    int64_t *vals_in, *vals_out;
+    void* temp;
+    size_t temp_bytes;
+};
+
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+int order_buffers(lyap_ctx* c, int64_t count, int n_exps, OrderBufs* ob) {
+    if (count > (int64_t)0x7fffffff)
+        return fail(LYAP_ERR_INVALID, "heavy-first order takes at most 2^31 - 1 points, got %lld",
+                    (long long)count);
+    const size_t n = (size_t)count;
+    const size_t temp_bytes = lyap::order_sort_bytes(count);
+    if (temp_bytes == 0) return fail(LYAP_ERR_HIP, "heavy-first order: sort scratch query failed");
+    const size_t sizes[9] = {n * (size_t)n_exps * sizeof(double), n * 4, n * 4, n * 4, n * 4, n * 4,
+                             n * 8, n * 8, temp_bytes};
+    size_t need = 0;
+    for (size_t z : sizes) need += align256(z);
+    if (need > c->order_cap) {
+        if (c->order_buf) hipFree(c->order_buf);
+        c->order_buf = nullptr;
+        c->order_cap = 0;
+        hipError_t e = hipMalloc(&c->order_buf, need);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(order buffers)");
+        c->order_cap = need;
+    }
+    char* p = (char*)c->order_buf;
+    void* parts[9];
+    for (int i = 0; i < 9; ++i) {
+        parts[i] = p;
+        p += align256(sizes[i]);
+    }
+    *ob = OrderBufs{(double*)parts[0],   (uint32_t*)parts[1], (uint32_t*)parts[2], (uint32_t*)parts[3],
+                    (uint32_t*)parts[4], (uint32_t*)parts[5], (int64_t*)parts[6],  (int64_t*)parts[7],
+                    parts[8],            temp_bytes};
+    return LYAP_SUCCESS;
+}
+
 int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t count,
                   const int64_t* d_idx_list, const double* d_params, const double* d_inits,
                   double* d_lambda, uint32_t* d_status, uint32_t* d_n_acc, uint32_t* d_n_rej,
@@ -336,20 +383,64 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
         watch = lyap::LaneWatch{c->watch_skip, c->watch_done, c->watch_flag,
                                 {c->watch_end[0], c->watch_end[1]}, ++c->watch_seq, 0u};
     }
-    if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
-    if (lane_model(fd.model)) {
-        e = lyap::launch_lane_sweep(fd, io, s, c->num_cus, c->lane_max_blocks, quad, watched ? &watch : nullptr);
-    } else if (fd.model == LYAP_MODEL_L96) {
+    if (!lane_model(fd.model)) {
+        if (fd.model != LYAP_MODEL_L96)
+            return fail(LYAP_ERR_UNSUPPORTED, "no kernel for model %d (dimension %d)", fd.model, fd.dim);
         if (fd.method != LYAP_METHOD_VAR)
             return fail(LYAP_ERR_UNSUPPORTED, "Lorenz-96 runs the coupled variational method only");
-        const int j = fd.dim == 8 ? 0 : (fd.dim == 16 ? 1 : 2);
-        e = lyap::launch_l96_sweep(fd, io, s, c->num_cus, c->l96_waves[j], c->l96_ppw[j],
-                                   c->l96_max_blocks);
-    } else {
-        return fail(LYAP_ERR_UNSUPPORTED, "no kernel for model %d (dimension %d)", fd.model, fd.dim);
     }
-    if (e != hipSuccess) return hip_fail(e, "sweep launch");
-    if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    auto launch = [&](const lyap_sweep_desc& d, const lyap::SweepIO& x) -> hipError_t {
+        if (lane_model(d.model))
+            return lyap::launch_lane_sweep(d, x, s, c->num_cus, c->lane_max_blocks, quad,
+                                           watched ? &watch : nullptr);
+        const int j = d.dim == 8 ? 0 : (d.dim == 16 ? 1 : 2);
+        return lyap::launch_l96_sweep(d, x, s, c->num_cus, c->l96_waves[j], c->l96_ppw[j], c->l96_max_blocks);
+    };
+    // heavy-first order: adaptive runs of the grid (no caller's list, not watched)
+    const bool ordered = c->order_probe > 0 && !d_idx_list && !watched && fd.adaptive &&
+                         (fd.method == LYAP_METHOD_VAR || fd.method == LYAP_METHOD_VAR_TRACE);
+    OrderBufs ob{};
+    if (ordered) {
+        const int rc = order_buffers(c, count, fd.n_exps, &ob);
+        if (rc != LYAP_SUCCESS) return rc;
+    }
+    // ev0/ev1 bracket the run's own launch (not an ordered run's probe, sort
+    // and scatter)
+    if (ordered) {
+        // the probe: the same integration over order_probe thousandths of the
+        // times (at least one renormalisation interval of averaging)
+        lyap_sweep_desc pd = fd;
+        const double f = (double)c->order_probe / 1000.0;
+        pd.t_transient *= f;
+        pd.t_slave *= f;
+        pd.t_average = std::max(fd.t_average * f, fd.t_renorm);
+        int rc = lyap_desc_finalize(&pd);
+        if (rc != LYAP_SUCCESS) return rc;
+        lyap::SweepIO pio = io;
+        pio.idx_list = nullptr;
+        pio.lambda = ob.lam;
+        pio.status = ob.st;
+        pio.n_acc = ob.na;
+        pio.n_rej = ob.nr;
+        pio.stride = spread_stride(count);
+        if ((e = launch(pd, pio)) != hipSuccess) return hip_fail(e, "probe launch");
+        if ((e = lyap::order_heavy_first(ob.na, ob.nr, first, count, ob.keys_in, ob.keys_out, ob.vals_in,
+                                         ob.vals_out, ob.temp, ob.temp_bytes, s)) != hipSuccess)
+            return hip_fail(e, "heavy-first sort");
+        lyap::SweepIO mio = pio;
+        mio.idx_list = ob.vals_out;
+        mio.stride = 1;  // list order
+        if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = launch(fd, mio)) != hipSuccess) return hip_fail(e, "sweep launch");
+        if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = lyap::order_scatter(ob.vals_out, first, count, fd.n_exps, ob.lam, ob.st, ob.na, ob.nr,
+                                     d_lambda, d_status, d_n_acc, d_n_rej, s)) != hipSuccess)
+            return hip_fail(e, "output scatter");
+    } else {
+        if ((e = hipEventRecord(c->ev0, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+        if ((e = launch(fd, io)) != hipSuccess) return hip_fail(e, "sweep launch");
+        if ((e = hipEventRecord(c->ev1, s)) != hipSuccess) return hip_fail(e, "hipEventRecord");
+    }
     c->timed = true;
     return LYAP_SUCCESS;
 }
@@ -407,6 +498,11 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
     if (k == "lane_quad") {
         if (value != 0 && value != 1) return fail(LYAP_ERR_INVALID, "lane_quad must be 0 or 1");
         c->lane_quad = (int)value;
+        return LYAP_SUCCESS;
+    }
+    if (k == "order_probe") {
+        if (value < 0 || value > 500) return fail(LYAP_ERR_INVALID, "order_probe must be in [0, 500]");
+        c->order_probe = (int)value;
         return LYAP_SUCCESS;
     }
     if (k == "item_order") {

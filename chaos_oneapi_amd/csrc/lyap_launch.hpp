@@ -30,6 +30,20 @@ bool l96_supported(int n, int waves);
 bool l96_wave_supported(int n, int ppw);
 int l96_default_waves(int n);
 
+// heavy-first order (lyap_order.hip): keys n_acc + n_rej of a probe run over
+// points [first, first + count), sorted descending with the point indices
+// (vals_out: the index list of the full run); temp: order_sort_bytes(count)
+// bytes of sort scratch (0 when the query fails)
+size_t order_sort_bytes(int64_t count);
+hipError_t order_heavy_first(const uint32_t* n_acc, const uint32_t* n_rej, int64_t first, int64_t count,
+                             uint32_t* keys_in, uint32_t* keys_out, int64_t* vals_in, int64_t* vals_out,
+                             void* temp, size_t temp_bytes, hipStream_t s);
+// outputs of a run in list order (idx: the list) back to point order; null
+// st/na/nr outputs are skipped
+hipError_t order_scatter(const int64_t* idx, int64_t first, int64_t count, int n_exps, const double* lam_s,
+                         const uint32_t* st_s, const uint32_t* na_s, const uint32_t* nr_s, double* lam,
+                         uint32_t* st, uint32_t* na, uint32_t* nr, hipStream_t s);
+
 // ---------------------------------------------------------------------------
 // batched mixed-model launch (lyap_batch.hip)
 // ---------------------------------------------------------------------------

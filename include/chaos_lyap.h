@@ -176,6 +176,17 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *       s ~ count/phi coprime to count), 1 in order (point i, i.e. the order of
  *       idx_list: a list sorted by decreasing expected cost schedules the
  *       longest points first); results are the same bits either way
+ *   "order_probe": 1..500 runs adaptive sweeps of the grid (the sweep's
+ *       method, no idx_list, not watched) heavy-first: a probe run of every
+ *       point over this many thousandths of the integration times counts its
+ *       steps, the points are sorted by that count, most first, and the full
+ *       run takes them in that order (its outputs scattered back to point
+ *       order); 0 (default) off.  Pays where the grid holds more points than
+ *       the launch has lanes, so that points start in later rounds; the run
+ *       allocates staging and sort buffers on the context (kept for later
+ *       runs).  lyap_last_kernel_ms times the full run's launch alone (not
+ *       the probe, sort or scatter);
+ *       results are the same bits either way
  *   "lane_quad": 1 runs the sweep's method on the 3-D flows with four lanes
  *       per point (each lane one deviation vector; base step and control
  *       repeated in the four lanes), for launches of about one point per lane

@@ -168,3 +168,32 @@ def test_tune_lanes_time_scaled_runs_the_whole_grid():
     r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=timer, time_scale=0.1)
     assert r.best == (512, 8, 0)
     assert len(seen) == 6 and all(i is None for i in seen)
+
+
+def test_order_candidates_need_an_adaptive_grid_of_few_rounds():
+    c2 = L.finalize(L.make_desc(W.rossler_dverk(n=512)))
+    assert A.resident_points(c2, 256) == 256 * 4 * 3 * 64
+    assert A.order_candidates(c2, 512 * 512, A.resident_points(c2, 256)) == A.ORDER_CANDIDATES
+    # 21 points a lane: every heavy point starts early enough, no ordering
+    assert A.order_candidates(c2, 2048 * 2048, A.resident_points(c2, 256)) == (0,)
+    # fixed steps: every point costs the same
+    c1 = L.finalize(L.make_desc(W.lorenz_fixed(64)))
+    assert A.order_candidates(c1, 4096, A.resident_points(c1, 256)) == (0,)
+    c3 = L.finalize(L.make_desc(W.lorenz96_dopri(n=256, dim=16)))
+    assert A.resident_points(c3, 256, l96_ppw=4) == 256 * 4 * 3 * 4
+    assert A.order_candidates(c3, 65536, A.resident_points(c3, 256, l96_ppw=4)) == A.ORDER_CANDIDATES
+
+
+def test_tune_order_times_every_candidate_and_keeps_the_fastest():
+    c2 = L.finalize(L.make_desc(W.rossler_dverk(n=512)))
+    seen = []
+
+    def timer(c):
+        seen.append(c)
+        return {0: 950.0, 10: 800.0, 30: 770.0}[c]
+
+    r = A.tune_order(None, c2, A.resident_points(c2, 256), cache=A.Cache(None), timer=timer)
+    assert r.best == (30,) and seen == list(A.ORDER_CANDIDATES)
+    big = L.finalize(L.make_desc(W.rossler_dverk(n=2048)))
+    r = A.tune_order(None, big, A.resident_points(big, 256), cache=A.Cache(None), timer=timer)
+    assert r.best == (0,) and r.timings == {} and len(seen) == 3
