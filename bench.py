@@ -399,7 +399,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
     desc = L.finalize(L.make_desc(spec))
     n = desc.n1 * desc.n2
     kw = dict(train_n=args.train_n, epochs=args.epochs, fourier_sigma=args.fourier_sigma,
-              one_launch=not args.split_schedule)
+              one_launch=not args.split_schedule, order_probe=args.order_probe)
     for _ in range(args.warmup):
         R.refine(ctx, spec, **kw)
     torch.cuda.synchronize()
@@ -412,12 +412,19 @@ def bench_surrogate(args, ctx, spec, wl_name):
         print(f"step {k}: {times[-1]:.3f} s, phases {res.timings}", file=sys.stderr, flush=True)
     ms_per_step = 1e3 * sum(times) / len(times)
     st = res.stats
-    lam_p, st_p, plain_ms, flops_p, _ = plain_fp64(ctx, desc)
+    # the comparison sweeps take their points in the same heavy-first order as
+    # the refinement's launch (the probe's time included in theirs)
+    ctx.set_option("order_probe", args.order_probe)
+    try:
+        lam_p, st_p, plain_ms, flops_p, _ = plain_fp64(ctx, desc)
+    finally:
+        ctx.set_option("order_probe", 0)
     print(f"plain FP64 sweep {plain_ms:.0f} ms", file=sys.stderr, flush=True)
     # the mixed form alone over the whole plane (no surrogate), with the
     # rerun's renormalisation gather: what the surrogate adds to the mixed form
     stream = torch.cuda.current_stream()
     ctx.set_option("lane_gather", 4)
+    ctx.set_option("order_probe", args.order_probe)
     try:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
@@ -427,6 +434,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
         mixed_ms = e0.elapsed_time(e1)
     finally:
         ctx.set_option("lane_gather", 0)
+        ctx.set_option("order_probe", 0)
     lam_m, st_m = lam_m.cpu().numpy(), st_m.cpu().numpy()
     print(f"mixed-only sweep {mixed_ms:.0f} ms", file=sys.stderr, flush=True)
     # the final map against the plain FP64 map over the whole plane
@@ -457,6 +465,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
         "mixed_only_points_per_s": n / (mixed_ms * 1e-3),
         "speedup_mixed_only_vs_plain_fp64": plain_ms / mixed_ms,
         "surrogate_gain_vs_mixed_only": mixed_ms / ms_per_step,
+        "order_probe": args.order_probe,
         "rerun_fraction": st["rerun_fraction"], "surrogate_answered_fraction": 1.0 - st["rerun_fraction"],
         "train_points": st["train_points"],
         "verify_points": st["verify"], "calibration_points": st["calib"],
@@ -740,6 +749,9 @@ def main():
     ap.add_argument("--train-n", type=int, default=256)
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--fourier-sigma", type=float, default=6.0)
+    ap.add_argument("--order-probe", type=int, default=10,
+                    help="surrogate workload: heavy-first order of the refinement's launch and of the "
+                         "comparison sweeps, by a probe over this many thousandths of the times (0: off)")
     ap.add_argument("--split-schedule", action="store_true",
                     help="surrogate workload: the split schedule (separate first launches, then a rerun)")
     ap.add_argument("--batch-points", type=int, default=1 << 20)

@@ -321,7 +321,7 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
 ORDER_CANDIDATES = (0, 10, 30)
 
 
-def order_candidates(desc, n_points: int, resident_points: int, max_rounds: int = 8) -> tuple:
+def order_candidates(desc, n_points: int, resident_points: int, max_rounds: int = 32) -> tuple:
     """order_probe values worth timing for a sweep: heavy-first order pays where
     an adaptive grid's points start in a few rounds of the launch's resident
     points (a heavy point of a later round sets the launch's length); with more

@@ -125,6 +125,7 @@ struct lyap_ctx {
     int lane_quad = 0;   // 1: the sweep's method on the 3-D flows with four lanes per point
     int order_probe = 0; // > 0: adaptive sweeps run heavy-first, ordered by a probe run over
                          // this many thousandths of the integration times
+    int order_radius = 2;  // grid neighbourhood whose heaviest probe count orders a point
     unsigned long long* counter = nullptr;
     void* order_buf = nullptr;  // staging, keys, index lists and sort scratch of ordered runs
     size_t order_cap = 0;
@@ -424,8 +425,11 @@ int run_finalized(lyap_ctx* c, const lyap_sweep_desc& fd, int64_t first, int64_t
         pio.n_rej = ob.nr;
         pio.stride = spread_stride(count);
         if ((e = launch(pd, pio)) != hipSuccess) return hip_fail(e, "probe launch");
-        if ((e = lyap::order_heavy_first(ob.na, ob.nr, first, count, ob.keys_in, ob.keys_out, ob.vals_in,
-                                         ob.vals_out, ob.temp, ob.temp_bytes, s)) != hipSuccess)
+        // the grid neighbourhood is the parameter plane's only where the
+        // points are the grid's own (no caller's parameter rows)
+        const int32_t radius = d_params ? 0 : c->order_radius;
+        if ((e = lyap::order_heavy_first(ob.na, ob.nr, first, count, fd.n2, radius, ob.keys_in, ob.keys_out,
+                                         ob.vals_in, ob.vals_out, ob.temp, ob.temp_bytes, s)) != hipSuccess)
             return hip_fail(e, "heavy-first sort");
         lyap::SweepIO mio = pio;
         mio.idx_list = ob.vals_out;
@@ -503,6 +507,11 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
     if (k == "order_probe") {
         if (value < 0 || value > 500) return fail(LYAP_ERR_INVALID, "order_probe must be in [0, 500]");
         c->order_probe = (int)value;
+        return LYAP_SUCCESS;
+    }
+    if (k == "order_radius") {
+        if (value < 0 || value > 8) return fail(LYAP_ERR_INVALID, "order_radius must be in [0, 8]");
+        c->order_radius = (int)value;
         return LYAP_SUCCESS;
     }
     if (k == "item_order") {

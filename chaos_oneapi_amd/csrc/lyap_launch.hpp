@@ -31,13 +31,16 @@ bool l96_wave_supported(int n, int ppw);
 int l96_default_waves(int n);
 
 // heavy-first order (lyap_order.hip): keys n_acc + n_rej of a probe run over
-// points [first, first + count), sorted descending with the point indices
-// (vals_out: the index list of the full run); temp: order_sort_bytes(count)
-// bytes of sort scratch (0 when the query fails)
+// points [first, first + count) of a grid with n2 points per row, with
+// radius > 0 the largest over the (2 radius + 1)^2 grid neighbourhood, sorted
+// descending with the point indices (vals_out: the index list of the full
+// run); temp: order_sort_bytes(count) bytes of sort scratch (0 when the query
+// fails)
 size_t order_sort_bytes(int64_t count);
 hipError_t order_heavy_first(const uint32_t* n_acc, const uint32_t* n_rej, int64_t first, int64_t count,
-                             uint32_t* keys_in, uint32_t* keys_out, int64_t* vals_in, int64_t* vals_out,
-                             void* temp, size_t temp_bytes, hipStream_t s);
+                             int32_t n2, int32_t radius, uint32_t* keys_in, uint32_t* keys_out,
+                             int64_t* vals_in, int64_t* vals_out, void* temp, size_t temp_bytes,
+                             hipStream_t s);
 // outputs of a run in list order (idx: the list) back to point order; null
 // st/na/nr outputs are skipped
 hipError_t order_scatter(const int64_t* idx, int64_t first, int64_t count, int n_exps, const double* lam_s,

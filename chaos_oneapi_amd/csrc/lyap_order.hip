@@ -4,8 +4,9 @@
 // long as the lane that ends last: a heavy point taken in the second round
 // finishes long after the rest.  A short probe run of every point (the same
 // integration over a small fraction of its times) gives each point's step
-// count; the points are then sorted by it, heaviest first, and the full run
-// takes them in that order.  Its outputs land in list order in staging
+// count; the points are then sorted by it (or by the largest count of their
+// grid neighbourhood), heaviest first, and the full run takes them in that
+// order.  Its outputs land in list order in staging
 // buffers and are scattered back to point order.  Every point's integration
 // is the same whatever the order, so the results are the same bits.
 #include <hip/hip_runtime.h>
@@ -45,6 +46,31 @@ __global__ void __launch_bounds__(256) k_order_scatter(const int64_t* __restrict
     if (nr) nr[p] = nr_s[j];
 }
 
+// keys of grid points: the largest raw key over the (2r+1)^2 grid points
+// around each point that lie in the run's range [first, first + count) of the
+// n1 x n2 grid (index p1 * n2 + p2).  Points whose neighbourhood holds a heavy
+// point count as heavy, so that a point the short probe takes for light is
+// not left for a late round when its neighbours are heavy.
+__global__ void __launch_bounds__(256) k_order_nbr_max(const uint32_t* __restrict__ raw, int64_t first,
+                                                       int64_t count, int32_t n2, int32_t r,
+                                                       uint32_t* __restrict__ keys) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int64_t g = first + i;
+    const int64_t g1 = g / n2, g2 = g % n2;
+    uint32_t m = raw[i];
+    for (int64_t a = g1 - r; a <= g1 + r; ++a) {
+        for (int64_t b = g2 - r; b <= g2 + r; ++b) {
+            if (b < 0 || b >= n2) continue;
+            const int64_t j = a * n2 + b - first;
+            if (a < 0 || j < 0 || j >= count) continue;
+            const uint32_t v = raw[j];
+            m = v > m ? v : m;
+        }
+    }
+    keys[i] = m;
+}
+
 unsigned grid_of(int64_t count) { return (unsigned)((count + 255) / 256); }
 
 }  // namespace
@@ -59,12 +85,19 @@ size_t order_sort_bytes(int64_t count) {
 }
 
 hipError_t order_heavy_first(const uint32_t* n_acc, const uint32_t* n_rej, int64_t first, int64_t count,
-                             uint32_t* keys_in, uint32_t* keys_out, int64_t* vals_in, int64_t* vals_out,
-                             void* temp, size_t temp_bytes, hipStream_t s) {
+                             int32_t n2, int32_t radius, uint32_t* keys_in, uint32_t* keys_out,
+                             int64_t* vals_in, int64_t* vals_out, void* temp, size_t temp_bytes,
+                             hipStream_t s) {
+    // raw keys into keys_out (the sort's output, free until the sort)
     hipLaunchKernelGGL(k_order_keys, dim3(grid_of(count)), dim3(256), 0, s, n_acc, n_rej, first, count,
-                       keys_in, vals_in);
+                       radius > 0 ? keys_out : keys_in, vals_in);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (radius > 0) {
+        hipLaunchKernelGGL(k_order_nbr_max, dim3(grid_of(count)), dim3(256), 0, s, keys_out, first, count, n2,
+                           radius, keys_in);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     return hipcub::DeviceRadixSort::SortPairsDescending(temp, temp_bytes, keys_in, keys_out, vals_in,
                                                         vals_out, (int)count, 0, 32, s);
 }

@@ -23,7 +23,10 @@ of the 2048^2 Rossler plane: 2,537 ms at 4 against 2,564 at 0, 2,579 at 2,
 Schedules.  With one_launch (the default, for the sweep's method with adaptive
 steps on a 3-D flow) the method runs as one persistent launch over the
 training lattice, the calibration points, the verification points and then
-every other point, on a side stream (lyap_ctx_set_watch): it signals the host
+every other point — heavy-first by a probe run of the plane over order_probe
+thousandths of the integration times (heavy_first), so that no heavy point
+starts in the launch's last round — on a side stream (lyap_ctx_set_watch): it
+signals the host
 when the lattice and when the calibration points are done, while it goes on;
 training, inference and the choice of the rule follow on the main stream, and
 the points the rule answers are skipped from the moment their skip words are
@@ -116,6 +119,30 @@ def spread_order(n: int, device):
     while math.gcd(s, n) != 1:
         s += 1
     return (torch.arange(n, device=device, dtype=torch.int64) * s) % n
+
+
+def heavy_first(ctx, fd, idx, permille: int, radius: int = 2, stream=None):
+    """Grid point indices idx reordered most expected steps first, by the rule
+    of lyap_sweep_run's heavy-first order (options order_probe, order_radius):
+    a probe run of the whole grid over permille thousandths of the integration
+    times (at least one renormalisation interval of averaging) counts every
+    point's steps, and a point's key is the largest count within `radius` rows
+    and columns of it.  Ties keep idx's order."""
+    import torch
+    import torch.nn.functional as Fn
+
+    pd = L.SweepDesc.from_buffer_copy(fd)
+    f = permille / 1000.0
+    pd.t_transient *= f
+    pd.t_slave *= f
+    pd.t_average = max(fd.t_average * f, fd.t_renorm)
+    pd = L.finalize(pd)
+    _, _, na, nr = ctx.run(pd, stream=stream)
+    keys = (na.to(torch.float32) + nr.to(torch.float32)).reshape(1, 1, fd.n1, fd.n2)
+    if radius > 0:
+        keys = Fn.max_pool2d(keys, 2 * radius + 1, stride=1, padding=radius)
+    order = torch.argsort(keys.reshape(-1)[idx], descending=True, stable=True)
+    return idx[order]
 
 
 def side_streams(dev):
@@ -353,7 +380,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, dlam_cal: float = 0.5 * DLAMBDA1_GATE,
            rerun_gather: int = 4, one_launch: bool = True, judge_blocks: int = 32,
-           split_blocks_per_cu: int = 1, watch_timeout_s: float = 600.0) -> RefineResult:
+           split_blocks_per_cu: int = 1, watch_timeout_s: float = 600.0,
+           order_probe: int = 10, order_radius: int = 2) -> RefineResult:
     import torch
 
     dev = torch.device("cuda", ctx.device)
@@ -422,6 +450,15 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
         # taken before the rule was known are computed all the same; the map
         # takes the rule's answers there too, so it does not depend on timing.
         n_first = n_t + n_cal + n_ver
+        if order_probe > 0:
+            # the points after the first milestones heavy-first (a probe run
+            # of the plane), so that no heavy point starts in the launch's last
+            # round
+            t_o = time.perf_counter()
+            rest = heavy_first(ctx, fd, torch.nonzero(~first_mask).flatten(), order_probe, order_radius,
+                               stream=main)
+            main.synchronize()
+            tm["order_probe"] = time.perf_counter() - t_o
         items = torch.cat([train_idx, calib, verify, rest]).contiguous()
         skip = torch.zeros(items.numel(), dtype=torch.int32, device=dev)
         side = side_streams(dev)[0]

@@ -179,7 +179,8 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *   "order_probe": 1..500 runs adaptive sweeps of the grid (the sweep's
  *       method, no idx_list, not watched) heavy-first: a probe run of every
  *       point over this many thousandths of the integration times counts its
- *       steps, the points are sorted by that count, most first, and the full
+ *       steps, the points are sorted by the largest count in their grid
+ *       neighbourhood ("order_radius"), most first, and the full
  *       run takes them in that order (its outputs scattered back to point
  *       order); 0 (default) off.  Pays where the grid holds more points than
  *       the launch has lanes, so that points start in later rounds; the run
@@ -187,6 +188,11 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *       runs).  lyap_last_kernel_ms times the full run's launch alone (not
  *       the probe, sort or scatter);
  *       results are the same bits either way
+ *   "order_radius": 0..8 (default 2): with order_probe, a point of the grid
+ *       is ordered by the largest probe count over the grid points within
+ *       this many rows and columns of it (0: its own count; runs with caller's
+ *       parameter rows use their own counts): a short probe misjudges single
+ *       points, not the regions of the plane that are heavy
  *   "lane_quad": 1 runs the sweep's method on the 3-D flows with four lanes
  *       per point (each lane one deviation vector; base step and control
  *       repeated in the four lanes), for launches of about one point per lane

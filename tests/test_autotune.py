@@ -174,8 +174,9 @@ def test_order_candidates_need_an_adaptive_grid_of_few_rounds():
     c2 = L.finalize(L.make_desc(W.rossler_dverk(n=512)))
     assert A.resident_points(c2, 256) == 256 * 4 * 3 * 64
     assert A.order_candidates(c2, 512 * 512, A.resident_points(c2, 256)) == A.ORDER_CANDIDATES
-    # 21 points a lane: every heavy point starts early enough, no ordering
-    assert A.order_candidates(c2, 2048 * 2048, A.resident_points(c2, 256)) == (0,)
+    # 21 points a lane still pays; 85 do not (every heavy point starts early enough)
+    assert A.order_candidates(c2, 2048 * 2048, A.resident_points(c2, 256)) == A.ORDER_CANDIDATES
+    assert A.order_candidates(c2, 8192 * 2048, A.resident_points(c2, 256)) == (0,)
     # fixed steps: every point costs the same
     c1 = L.finalize(L.make_desc(W.lorenz_fixed(64)))
     assert A.order_candidates(c1, 4096, A.resident_points(c1, 256)) == (0,)
@@ -194,6 +195,6 @@ def test_tune_order_times_every_candidate_and_keeps_the_fastest():
 
     r = A.tune_order(None, c2, A.resident_points(c2, 256), cache=A.Cache(None), timer=timer)
     assert r.best == (30,) and seen == list(A.ORDER_CANDIDATES)
-    big = L.finalize(L.make_desc(W.rossler_dverk(n=2048)))
-    r = A.tune_order(None, big, A.resident_points(big, 256), cache=A.Cache(None), timer=timer)
+    big = L.finalize(L.make_desc(W.rossler_dverk(n=4096)))
+    r = A.tune_order(None, big, A.resident_points(big, 256) // 4, cache=A.Cache(None), timer=timer)
     assert r.best == (0,) and r.timings == {} and len(seen) == 3

@@ -2,7 +2,8 @@
 steps, the full run takes the points most steps first and scatters its outputs
 back to point order.  Every point's results are the same bits as the run in
 the kernels' own order (and so as the oracle's), over the whole grid and over
-a sub-range, for the one-lane, four-lane and Lorenz-96 kernels."""
+a sub-range, for the one-lane, four-lane and Lorenz-96 kernels, whatever the
+probe's length and the grid neighbourhood its keys take the maximum over."""
 import ctypes
 
 import numpy as np
@@ -49,19 +50,48 @@ CASES = {
 
 
 @pytest.mark.parametrize("case", list(CASES))
-@pytest.mark.parametrize("permille", [10, 200])
-def test_heavy_first_matches_spread_order(gpu_ctx, case, permille):
+@pytest.mark.parametrize("permille,radius", [(10, 2), (200, 0), (30, 8)])
+def test_heavy_first_matches_spread_order(gpu_ctx, case, permille, radius):
     spec, opts = CASES[case]
     desc = L.finalize(L.make_desc(spec))
     n = desc.n1 * desc.n2
-    print(f"case order {case} permille {permille} n={n}", flush=True)
+    print(f"case order {case} permille {permille} radius {radius} n={n}", flush=True)
     base = run_opts(gpu_ctx, desc, opts)
-    got = run_opts(gpu_ctx, desc, dict(opts, order_probe=permille))
+    gpu_ctx.set_option("order_radius", radius)
+    try:
+        got = run_opts(gpu_ctx, desc, dict(opts, order_probe=permille))
+        # a sub-range of the grid (neighbourhoods cut at its ends)
+        first, count = n // 3, n // 2
+        sub = run_opts(gpu_ctx, desc, dict(opts, order_probe=permille), first=first, count=count)
+    finally:
+        gpu_ctx.set_option("order_radius", 2)
     assert_same(got, base)
-    # a sub-range of the grid
-    first, count = n // 3, n // 2
-    sub = run_opts(gpu_ctx, desc, dict(opts, order_probe=permille), first=first, count=count)
     assert_same(sub, tuple(x[first:first + count] for x in base))
+
+
+def test_refine_heavy_first_is_a_sorted_permutation(gpu_ctx):
+    """The refinement's own ordering: a permutation of the given points whose
+    neighbourhood-maximum probe counts do not increase along the list."""
+    import torch.nn.functional as Fn
+
+    from chaos_oneapi_amd import refine as R
+
+    spec = dict(W.rossler_dverk(40, precision=L.PREC_MIXED, t_transient=5.0, t_average=20.0),
+                method=L.METHOD_VAR_TRACE)
+    fd = L.finalize(L.make_desc(spec))
+    n = fd.n1 * fd.n2
+    idx = torch.arange(0, n, 3, device="cuda", dtype=torch.int64)
+    got = R.heavy_first(gpu_ctx, fd, idx, 100, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(got.sort().values, idx)
+    pd = L.SweepDesc.from_buffer_copy(fd)
+    pd.t_transient *= 0.1
+    pd.t_average = max(fd.t_average * 0.1, fd.t_renorm)
+    _, _, na, nr = gpu_ctx.run(L.finalize(pd))
+    keys = (na + nr).to(torch.float32).reshape(1, 1, fd.n1, fd.n2)
+    keys = Fn.max_pool2d(keys, 3, stride=1, padding=1).reshape(-1)
+    k = keys[got].cpu().numpy()
+    assert np.all(np.diff(k) <= 0) and k[0] > k[-1]
 
 
 def test_heavy_first_matches_oracle(gpu_ctx):
@@ -103,3 +133,6 @@ def test_order_probe_range(gpu_ctx):
     for bad in (-1, 501):
         assert lib.lyap_ctx_set_option(gpu_ctx._h, b"order_probe", bad) == L.ERR_INVALID
     assert lib.lyap_ctx_set_option(gpu_ctx._h, b"order_probe", 0) == L.SUCCESS
+    for bad in (-1, 9):
+        assert lib.lyap_ctx_set_option(gpu_ctx._h, b"order_radius", bad) == L.ERR_INVALID
+    assert lib.lyap_ctx_set_option(gpu_ctx._h, b"order_radius", 2) == L.SUCCESS
