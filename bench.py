@@ -239,9 +239,12 @@ def bench_sweep(args, ctx, spec, wl_name):
     tuned = tuned_lanes = None
     if args.no_tune:
         # fixed shapes (profiling runs): the surrogate rerun's gather for the
-        # 3-D flows, the default Lorenz-96 kernel
+        # 3-D flows, the default Lorenz-96 kernel, heavy-first order by
+        # --order-probe for adaptive grids
         if desc.dim == 3:
             ctx.set_option("lane_gather", 4)
+        if desc.adaptive:
+            ctx.set_option("order_probe", args.order_probe)
     elif desc.model == L.MODEL_L96:
         tuned = A.tune_l96(ctx, desc, cache=A.Cache(None), time_scale=0.1)  # sets the tile shape on ctx
     elif desc.dim == 3:
@@ -284,6 +287,7 @@ def bench_sweep(args, ctx, spec, wl_name):
         # profiling runs: the chosen form's launches only
         rec = base_record(args, n / (ms_per_step * 1e-3), ms_per_step, desc)
         rec["config"] = {"workload": wl_name, "points": n, "profiling_run": True,
+                         "order_probe": args.order_probe if args.no_tune and desc.adaptive else None,
                          "mean_steps_per_point": float(na_c.mean())}
         rec["roofline"] = roofline(f64_c, f32_c, kern_avg, kname + " (mixed) over the grid", args.probe,
                                    measured_traffic(wl_name))
@@ -750,8 +754,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=80)
     ap.add_argument("--fourier-sigma", type=float, default=6.0)
     ap.add_argument("--order-probe", type=int, default=10,
-                    help="surrogate workload: heavy-first order of the refinement's launch and of the "
-                         "comparison sweeps, by a probe over this many thousandths of the times (0: off)")
+                    help="heavy-first order by a probe over this many thousandths of the times (0: off): "
+                         "the surrogate workload's launch and comparison sweeps, and the single-grid "
+                         "workloads under --no-tune (the tuner picks it otherwise)")
     ap.add_argument("--split-schedule", action="store_true",
                     help="surrogate workload: the split schedule (separate first launches, then a rerun)")
     ap.add_argument("--batch-points", type=int, default=1 << 20)

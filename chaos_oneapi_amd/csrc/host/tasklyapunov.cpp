@@ -130,6 +130,12 @@ int TaskLyapunov::allocate() {
             printf("error: %s\n", lyap_last_error());
             return -1;
         }
+        if (mParameters->order_probe > 0 &&
+            lyap_ctx_set_option(s.ctx, "order_probe", mParameters->order_probe) != LYAP_SUCCESS) {
+            printf("error: %s\n", lyap_last_error());
+            mShards.push_back(s);
+            return -1;
+        }
         const size_t cnt = (size_t)(s.count > 0 ? s.count : 1);
         if (!hip_ok(hipMalloc(&s.d_lambda, cnt * mDesc.n_exps * sizeof(double)), "hipMalloc") ||
             !hip_ok(hipMalloc(&s.d_status, cnt * sizeof(uint32_t)), "hipMalloc") ||

@@ -142,6 +142,7 @@ LyapunovParameters::LyapunovParameters(const json& jsonConfig, const std::string
             if (const json* v = find_key(c, &tool, "max_steps")) max_steps = v->as_int();
             if (const json* v = find_key(c, &tool, "escape_radius")) escape_radius = v->as_double();
             if (const json* v = find_key(c, &tool, "seed")) seed = (uint64_t)v->as_int();
+            if (const json* v = find_key(c, &tool, "order_probe")) order_probe = (int32_t)v->as_int();
             if (const json* v = find_key(c, &tool, "init_perturbation")) {
                 const std::string& kind = v->at("kind").as_string();
                 if (kind == "uniform") init_kind = LYAP_INIT_UNIFORM;

@@ -10,7 +10,9 @@
 //   "integrator": "dverk" | "dopri5", "adaptive": bool, "rtol", "atol",
 //   "h_min", "h_max", "max_steps", "escape_radius",
 //   "precision": "fp64" | "mixed",
-//   "seed", "init_perturbation": {"kind": "uniform"|"normal", "scale": s, "add_param": i}
+//   "seed", "init_perturbation": {"kind": "uniform"|"normal", "scale": s, "add_param": i},
+//   "order_probe": thousandths of the integration times of the heavy-first
+//   probe of adaptive sweeps (0: off; the lyap_ctx_set_option key of that name)
 #pragma once
 
 #include <cstdint>
@@ -71,6 +73,7 @@ public:
     int32_t init_kind = LYAP_INIT_NONE;
     double init_scale = 0.0;
     int32_t init_add_param = -1;
+    int32_t order_probe = 0;
 };
 
 struct ConfigFactory {

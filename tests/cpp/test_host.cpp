@@ -34,7 +34,7 @@ static const char* kConfig = R"({
      "initial_model_params": [10, 0, 0], "first_sweep_param": [1, 0, 200, 3],
      "second_sweep_param": [2, 0.5, 4, 4],
      "method": "var", "integrator": "dopri5", "adaptive": true, "rtol": 1e-9, "atol": 1e-9,
-     "precision": "mixed", "seed": 42,
+     "precision": "mixed", "seed": 42, "order_probe": 10,
      "init_perturbation": {"kind": "uniform", "scale": 0.001}},
     {"model_name": "shimizu", "lyapunov_exponent_num": 3, "epsilon": 1e-7,
      "integration_step": 1e-2, "skip_time_attractor": 1e+3, "skip_time_slave_trajectory": 1e+3,
@@ -93,9 +93,11 @@ static void test_params() {
     CHECK(p.lyapunov_exponent_num == 3 && p.total_time == 100.0);
     CHECK(p.method == LYAP_METHOD_VAR && p.integrator == LYAP_INT_DOPRI5 && p.adaptive);
     CHECK(p.precision == LYAP_PREC_MIXED && p.seed == 42 && p.init_kind == LYAP_INIT_UNIFORM);
+    CHECK(p.order_probe == 10);
     // reference defaults: using_var -> VAR_REF; a repeated key keeps the last value
     LyapunovParameters s(cfg, "shimizu");
     CHECK(s.method == LYAP_METHOD_VAR_REF && s.skip_time_normalization == 0.02);
+    CHECK(s.order_probe == 0);
     // tool_params.cpp:108-121: missing model and a 3-element sweep array throw
     int thrown = 0;
     try {
