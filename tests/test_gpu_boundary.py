@@ -142,3 +142,47 @@ def test_chaos_lyap_driver_on_shimizu_fd(tmp_path):
     assert lines[-1] == "" and len(lines) == 121
     for row, line in zip(rows, lines):
         assert line == " ".join(f"{v:f}" for v in row)
+
+
+ROSSLER_ORDERED_CONFIG = {
+    "hardware": {"num_GPUs": 1},
+    "tool": {"lyapunov": {"samples_num": 1, "using_qr": False, "using_var": True}},
+    "systems": [{
+        "model_name": "rossler", "lyapunov_exponent_num": 3, "epsilon": 1e-7,
+        "integration_step": 0.01, "skip_time_attractor": 5, "skip_time_slave_trajectory": 0,
+        "skip_time_normalization": 0.5, "total_time": 20, "initial_state": [0, 0, 0],
+        "initial_model_params": [0, 0, 0.2], "first_sweep_param": [0, 0.05, 0.4, 12],
+        "second_sweep_param": [1, 2, 20, 10],
+        "method": "var_trace", "integrator": "dverk", "adaptive": True, "rtol": 1e-9, "atol": 1e-9,
+        "precision": "mixed", "seed": 11, "init_perturbation": {"kind": "uniform", "scale": 1.0},
+        "order_probe": 50}],
+}
+
+
+def test_chaos_lyap_driver_heavy_first_rossler(tmp_path):
+    """The driver with this build's optional keys (the sweep's method, adaptive
+    DVERK, mixed form, trace closure) and the heavy-first order: the rows are the
+    oracle's values bit for bit, in the reference's p1-outer order."""
+    print("case chaos_lyap rossler var_trace order_probe 12x10", flush=True)
+    (tmp_path / "config.json").write_text(json.dumps(ROSSLER_ORDERED_CONFIG))
+    r = subprocess.run([CLI, "rossler"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    rows = np.load(tmp_path / "Data" / "Lyapunov rossler.npy")
+    assert rows.shape == (120, 5)
+    s = ROSSLER_ORDERED_CONFIG["systems"][0]
+    f, g = s["first_sweep_param"], s["second_sweep_param"]
+    spec = dict(model=L.MODEL_ROSSLER, dim=3, n_exps=3, n_params=3, method=L.METHOD_VAR_TRACE,
+                integrator=L.INT_DVERK, adaptive=1, precision=L.PREC_MIXED, h=s["integration_step"],
+                rtol=s["rtol"], atol=s["atol"], eps=s["epsilon"], t_transient=s["skip_time_attractor"],
+                t_slave=0.0, t_average=s["total_time"], t_renorm=s["skip_time_normalization"],
+                base_params=s["initial_model_params"], init_state=s["initial_state"],
+                init_kind=L.INIT_UNIFORM, init_scale=1.0, init_param_offset=-1, seed=s["seed"],
+                sweep_idx1=0, start1=f[1], delta1=abs(f[2] - f[1]) / (f[3] - 1), n1=f[3],
+                sweep_idx2=1, start2=g[1], delta2=abs(g[2] - g[1]) / (g[3] - 1), n2=g[3])
+    desc = L.finalize(L.make_desc(spec))
+    lam_o, st_o, _, _ = O.sweep(O.make_desc(desc.to_dict()), n_threads=8)
+    ok = st_o == 0
+    assert ok.sum() > 60
+    np.testing.assert_array_equal(rows[ok, 2:], lam_o[ok])
+    assert np.all(np.isnan(rows[~ok, 2:]))
