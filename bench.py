@@ -176,6 +176,8 @@ def chosen_form_name(desc):
     from chaos_oneapi_amd import lyap as L
 
     name = "mixed" if desc.precision == L.PREC_MIXED else "fp64"
+    if desc.precision == L.PREC_MIXED and desc.adaptive and desc.integrator == L.INT_DVERK and desc.dim == 3:
+        name += " (binary32 tangent on DVERK's fifth-order weights)"
     if desc.method == L.METHOD_VAR_TRACE:
         name += (f", {desc.dim - 1} deviation vectors + trace closure (lambda_{desc.dim} = "
                  "<tr J> - the others)")

@@ -201,6 +201,16 @@ struct Tab<LYAP_INT_DVERK> {
     }
     // stage 6 feeds only the error estimate
     static constexpr bool for_solution(int s) { return s != 5; }
+    // the embedded fifth-order weights b - e, which use stages 1-6 only: the
+    // binary32 tangent of the mixed form with adaptive steps takes these (its
+    // rounding, ~6e-8 a step, is far above the fifth-order truncation at steps
+    // whose sixth-order error estimate is held near rtol)
+    static constexpr int S_LOW = 6;
+    static constexpr double bh(int j) {
+        constexpr double BH[8] = {13.0 / 160.0, 0.0, 2375.0 / 5984.0, 5.0 / 16.0,
+                                  12.0 / 85.0,  3.0 / 44.0, 0.0,    0.0};
+        return BH[j];
+    }
 };
 
 // Dormand-Prince 5(4), cuda/tools/dopri.dp.cpp:14-45; stage 7 is FSAL
@@ -253,6 +263,14 @@ struct RowB {
     }
 };
 template <class TB, class T, int LD>
+struct RowBh {
+    T c[LD];
+    LYAP_D RowBh() {
+#pragma unroll
+        for (int j = 0; j < LD; ++j) c[j] = TB::bh(j) != 0.0 ? sconst((T)TB::bh(j)) : T(0);
+    }
+};
+template <class TB, class T, int LD>
 struct RowE {
     T c[LD];
     LYAP_D RowE() {
@@ -288,6 +306,23 @@ LYAP_D T coef_sum_b(const RowB<TB, T, LD>& row, const T (&k)[LD][N], int i) {
 #pragma unroll
     for (int j = 0; j < LD; ++j) {
         if (TB::b(j) == 0.0) continue;
+        if (!any) {
+            acc = row.c[j] * k[j][i];
+            any = true;
+        } else {
+            acc = fma(row.c[j], k[j][i], acc);
+        }
+    }
+    return acc;
+}
+
+template <class TB, class T, int LD, int N>
+LYAP_D T coef_sum_bh(const RowBh<TB, T, LD>& row, const T (&k)[LD][N], int i) {
+    T acc = T(0);
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < LD; ++j) {
+        if (TB::bh(j) == 0.0) continue;
         if (!any) {
             acc = row.c[j] * k[j][i];
             any = true;

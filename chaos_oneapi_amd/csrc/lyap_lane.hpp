@@ -72,14 +72,32 @@ LYAP_D void base_step(const double* x, const double* p, double h, double* xn, do
     if constexpr (TRACE) *tr_step = M::kTraceVar ? h * (M::trace_const(p) + trq) : h * M::trace_const(p);
 }
 
+// stages of the embedded fifth-order weights (DVERK); other tableaux have none
+template <class TB>
+struct LowStages {
+    static constexpr int value = TB::S;
+};
+template <>
+struct LowStages<Tab<LYAP_INT_DVERK>> {
+    static constexpr int value = Tab<LYAP_INT_DVERK>::S_LOW;
+};
+
+// adaptive steps take the low-order tangent in the mixed DVERK form
+template <int INTEG, bool MIXED>
+inline constexpr bool kLowTangent = MIXED && INTEG == LYAP_INT_DVERK;
+
 // coupled tangent step for one deviation vector v (FP64 storage).
 // FP64 form: stages in double.  Mixed form: stages in binary32, increment
-// h*sum(b_j K_j) formed in binary32 and added to the FP64 vector.
-template <class M, int INTEG, bool MIXED, class TX>
+// h*sum(b_j K_j) formed in binary32 and added to the FP64 vector.  LOW (the
+// mixed form with adaptive DVERK): the embedded fifth-order weights b - e on
+// stages 1-6 (the base step computes all eight stage states for its error
+// estimate), one stage fewer than the sixth-order solution's seven.
+template <class M, int INTEG, bool MIXED, class TX, bool LOW = false>
 LYAP_D void tangent_step(const typename M::template J<TX> (&X)[Tab<INTEG>::S], const double* p,
                          const float* pf, double h, double* v) {
     using TB = Tab<INTEG>;
     constexpr int S = TB::S, N = M::N;
+    static_assert(!LOW || (MIXED && INTEG == LYAP_INT_DVERK), "the low-order tangent is the mixed DVERK form's");
     using T = typename std::conditional<MIXED, float, double>::type;
     const T hT = (T)h;
     T vt[N];
@@ -88,7 +106,7 @@ LYAP_D void tangent_step(const typename M::template J<TX> (&X)[Tab<INTEG>::S], c
     T kv[S][N];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
-        if (!TB::for_solution(s)) {
+        if (LOW ? s >= LowStages<TB>::value : !TB::for_solution(s)) {
 #pragma unroll
             for (int i = 0; i < N; ++i) kv[s][i] = T(0);
             continue;
@@ -104,14 +122,23 @@ LYAP_D void tangent_step(const typename M::template J<TX> (&X)[Tab<INTEG>::S], c
         if constexpr (MIXED) M::template jvp<float>(X[s], pf, arg, kv[s]);
         else M::template jvp<double>(X[s], p, arg, kv[s]);
     }
-    const RowB<TB, T, S> rb;
+    if constexpr (LOW) {
+        const RowBh<TB, T, S> rbh;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        if constexpr (MIXED) {
-            const float inc = hT * coef_sum_b(rb, kv, i);
+        for (int i = 0; i < N; ++i) {
+            const float inc = hT * coef_sum_bh(rbh, kv, i);
             v[i] = v[i] + (double)inc;
-        } else {
-            v[i] = fma(h, coef_sum_b(rb, kv, i), v[i]);
+        }
+    } else {
+        const RowB<TB, T, S> rb;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            if constexpr (MIXED) {
+                const float inc = hT * coef_sum_b(rb, kv, i);
+                v[i] = v[i] + (double)inc;
+            } else {
+                v[i] = fma(h, coef_sum_b(rb, kv, i), v[i]);
+            }
         }
     }
 }
@@ -442,7 +469,8 @@ LYAP_D void lane_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                         } else {
 #pragma unroll
                             for (int r = 0; r < K; ++r)
-                                tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_try, V[r]);
+                                tangent_step<M, INTEG, MIXED, TX, kLowTangent<INTEG, MIXED>>(X, p, pf, h_try,
+                                                                                             V[r]);
                             if constexpr (TRACE) tri = tri + tr_step;
 #pragma unroll
                             for (int i = 0; i < N; ++i) {
@@ -689,7 +717,7 @@ LYAP_D void quad_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO& io
                         if (!acc) {
                             ++nrej;
                         } else {
-                            if (ql < K) tangent_step<M, INTEG, MIXED, TX>(X, p, pf, h_try, v);
+                            if (ql < K) tangent_step<M, INTEG, MIXED, TX, kLowTangent<INTEG, MIXED>>(X, p, pf, h_try, v);
                             if constexpr (TRACE) tri = tri + tr_step;
 #pragma unroll
                             for (int i = 0; i < N; ++i) {

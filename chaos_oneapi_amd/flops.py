@@ -21,7 +21,10 @@ the binary64 deviation vector (N flops per vector and step).  The Lorenz-96
 mixed form folds h into its binary32 Jacobian data (two binary64 products per
 component and stage, h*d once per step): its stage arguments are fma chains
 started on v (2*nnz per stage and component) and its increment sum(b_j K'_j)
-has no multiply by h (2*nnz(b) - 1).
+has no multiply by h (2*nnz(b) - 1).  The 3-D flows' mixed form with adaptive
+DVERK steps integrates its binary32 tangent with the embedded fifth-order
+weights b - e (5 nonzeros) on stages 1-6 (a-row nonzeros 0..5), one stage
+fewer than the sixth-order solution's.
 
 Peaks (MI355X, published): binary64 vector 78.6 TFLOP/s, binary32 vector
 157.3 TFLOP/s (the latter counts packed two-wide FMAs, which this build does
@@ -46,6 +49,7 @@ MODEL_COST = {
 # nonzero counts of the tableaux
 DVERK_A_ROWS = [0, 1, 2, 3, 4, 5, 5, 6]  # stage 6 (index 5) feeds the error only
 DVERK_NNZ_B, DVERK_NNZ_E = 6, 7
+DVERK_LOW_ROWS, DVERK_NNZ_BH = [0, 1, 2, 3, 4, 5], 5  # the fifth-order weights' stages 1-6
 DOPRI_A_ROWS = [0, 1, 2, 3, 4, 5, 5]  # stage 7 (index 6) is the FSAL stage
 DOPRI_NNZ_B, DOPRI_NNZ_E = 5, 6
 
@@ -89,6 +93,8 @@ def _step_parts(model: int, n: int, k: int, integ: int, adaptive: bool, mixed: b
     if mixed and model == L.MODEL_L96:
         vec32 = len(tan_st) * jv_cost + n * sum(2 * z for z in tan_st if z > 0) + n * (2 * nnz_b - 1)
         return base, k * vec32, k * n + 2 * n * len(tan_st) + 1
+    if mixed and adaptive and integ == L.INT_DVERK:
+        tan_st, nnz_b = DVERK_LOW_ROWS, DVERK_NNZ_BH
     tangent = len(tan_st) * jv_cost + n * sum(2 * z + 1 for z in tan_st if z > 0) + n * (2 * nnz_b + 1)
     if mixed:
         return base, k * (tangent - n), k * n

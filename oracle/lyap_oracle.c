@@ -338,6 +338,11 @@ static const double DV_B[8] = {3.0 / 40.0, 0.0, 875.0 / 2244.0, 23.0 / 72.0, 264
                                125.0 / 11592.0, 43.0 / 616.0};
 static const double DV_E[8] = {-1.0 / 160.0, 0.0, -125.0 / 17952.0, 1.0 / 144.0, -12.0 / 1955.0,
                                -3.0 / 44.0, 125.0 / 11592.0, 43.0 / 616.0};
+/* the embedded fifth-order weights b - e (stages 1-6 only): the binary32
+   tangent of the mixed form with adaptive DVERK steps takes these
+   (lyap_lane.hpp tangent_step LOW) */
+static const double DV_BH[8] = {13.0 / 160.0, 0.0, 2375.0 / 5984.0, 5.0 / 16.0, 12.0 / 85.0,
+                                3.0 / 44.0, 0.0, 0.0};
 
 /* Dormand-Prince 5(4), cuda/tools/dopri.dp.cpp:14-45 */
 static const double DP_A[7][7] = {
@@ -481,15 +486,18 @@ void orc_tangent_step(int model, int integ, int n, const double* stage_x, const 
 
 /* mixed form: the deviation vector stays FP64; the stage arguments, the
    Jacobian products and the stage sums are binary32; the step increment
-   h*sum(b_j K_j) is formed in binary32 and added to the FP64 vector. */
-static void tangent_step_f32(int model, int integ, int n, const double* stage_x, const double* p,
-                             double h, double* v) {
+   h*sum(b_j K_j) is formed in binary32 and added to the FP64 vector.  low
+   (adaptive DVERK steps): the embedded fifth-order weights DV_BH on stages
+   1-6, whose states the error-controlled base step has computed. */
+void orc_tangent_step_f32(int model, int integ, int n, const double* stage_x, const double* p,
+                          double h, double* v, int low) {
     float kv[8 * ORC_MAX_DIM], arg[ORC_MAX_DIM], vf[ORC_MAX_DIM];
     const int S = n_stages(integ);
     const float hf = (float)h;
+    low = low && integ == ORC_INT_DVERK;
     for (int i = 0; i < n; ++i) vf[i] = (float)v[i];
     for (int s = 0; s < S; ++s) {
-        if (!stage_needed_for_solution(integ, s)) {
+        if (low ? s >= 6 : !stage_needed_for_solution(integ, s)) {
             for (int i = 0; i < n; ++i) kv[s * ORC_MAX_DIM + i] = 0.0f;
             continue;
         }
@@ -497,7 +505,7 @@ static void tangent_step_f32(int model, int integ, int n, const double* stage_x,
         for (int i = 0; i < n; ++i) arg[i] = comb_f(a, s, kv, ORC_MAX_DIM, i, hf, vf[i]);
         orc_jvp_f32(model, n, stage_x + s * n, p, arg, kv + s * ORC_MAX_DIM);
     }
-    const double* b = tab_b(integ);
+    const double* b = low ? DV_BH : tab_b(integ);
     for (int i = 0; i < n; ++i) {
         float inc = hf * sum_f(b, S, kv, ORC_MAX_DIM, i);
         v[i] = v[i] + (double)inc;
@@ -994,7 +1002,7 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
                 orc_rk_step(d->model, integ, n, x, p, h, xn, NULL, sx);
                 for (int r = 0; r < k; ++r) {
                     if (mixed && d->model == ORC_MODEL_L96) l96_tangent_step_f32(integ, n, sx, p, h, V + r * n);
-                    else if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h, V + r * n);
+                    else if (mixed) orc_tangent_step_f32(d->model, integ, n, sx, p, h, V + r * n, 0);
                     else orc_tangent_step(d->model, integ, n, sx, p, h, V + r * n);
                 }
                 if (trace) tri = tri + orc_trace_step(d->model, integ, n, sx, p, h);
@@ -1053,7 +1061,7 @@ static void point_var(const orc_desc* d, const double* p, double* x, double* lam
             if (!acc) { ++n_rej; continue; }
             for (int r = 0; r < k; ++r) {
                 if (mixed && d->model == ORC_MODEL_L96) l96_tangent_step_f32(integ, n, sx, p, h_try, V + r * n);
-                else if (mixed) tangent_step_f32(d->model, integ, n, sx, p, h_try, V + r * n);
+                else if (mixed) orc_tangent_step_f32(d->model, integ, n, sx, p, h_try, V + r * n, 1);
                 else orc_tangent_step(d->model, integ, n, sx, p, h_try, V + r * n);
             }
             if (trace) tri = tri + orc_trace_step(d->model, integ, n, sx, p, h_try);

@@ -113,6 +113,10 @@ int orc_rk_step(int model, int integ, int n, const double* x, const double* p, d
 /* coupled tangent step (Jacobian at the stage states returned by orc_rk_step) */
 void orc_tangent_step(int model, int integ, int n, const double* stage_x, const double* p,
                       double h, double* v);
+/* the mixed form's coupled tangent step (binary32 stages, FP64 vector); low:
+   adaptive DVERK's fifth-order weights b - e on stages 1-6 */
+void orc_tangent_step_f32(int model, int integ, int n, const double* stage_x, const double* p,
+                          double h, double* v, int low);
 /* reference frozen-Jacobian tangent step (dverkStepVarMat) */
 void orc_frozen_tangent_step(int model, int n, const double* x_frozen, const double* p,
                              double h, double* v);

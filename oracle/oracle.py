@@ -94,6 +94,7 @@ def lib():
         L.orc_rk_step.restype = i
         L.orc_rk_step.argtypes = [i, i, i, P, P, d, P, P, P]
         L.orc_tangent_step.argtypes = [i, i, i, P, P, d, P]
+        L.orc_tangent_step_f32.argtypes = [i, i, i, P, P, d, P, i]
         L.orc_frozen_tangent_step.argtypes = [i, i, P, P, d, P]
         L.orc_mgs.argtypes = [i, i, i, P, P]
         L.orc_split_exponent.restype = d
@@ -200,6 +201,15 @@ def tangent_step(model, integ, stage_x, p, h, v):
     v = f64(v).copy()
     sx = f64(stage_x)
     lib().orc_tangent_step(model, integ, len(v), _ptr(sx), _ptr(p), float(h), _ptr(v))
+    return v
+
+
+def tangent_step_f32(model, integ, stage_x, p, h, v, low=False):
+    """The mixed form's tangent step; low: adaptive DVERK's fifth-order weights."""
+    p = f64(np.pad(p, (0, MAX_PARAMS - len(p))))
+    v = f64(v).copy()
+    sx = f64(stage_x)
+    lib().orc_tangent_step_f32(model, integ, len(v), _ptr(sx), _ptr(p), float(h), _ptr(v), int(low))
     return v
 
 

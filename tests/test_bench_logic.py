@@ -10,7 +10,7 @@ from chaos_oneapi_amd import workloads as W
 
 
 def test_precision_split_of_a_rossler_dverk_step():
-    acc, rej = F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True)
+    acc, rej = F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, True)
     f32 = F.per_step_split(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, True)
     # the tangent part is binary32 but the last add of each of the 3 vectors'
     # increments (3 components each) to the binary64 deviation vector
@@ -36,9 +36,25 @@ def test_l96_mixed_step_counts_the_folded_tangent():
     f32 = F.per_step_split(L.MODEL_L96, n, k, L.INT_DOPRI5, True, True)
     # binary32: 6 Jacobian products (7n) + stage chains (2*15n) + b sum (9n) per vector
     assert f32 == k * (6 * 7 * n + 30 * n + 9 * n)
-    # the 3-D flows' mixed form counts as the FP64 form (only the precision moves)
-    assert F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True, True) == \
-        F.per_step(L.MODEL_ROSSLER, 3, 3, L.INT_DVERK, True)
+    # the 3-D flows' mixed form counts as the FP64 form (only the precision
+    # moves) where its tangent keeps the solution's weights
+    for integ, adaptive in ((L.INT_DOPRI5, True), (L.INT_DVERK, False)):
+        assert F.per_step(L.MODEL_ROSSLER, 3, 3, integ, adaptive, True) == \
+            F.per_step(L.MODEL_ROSSLER, 3, 3, integ, adaptive)
+
+
+def test_low_order_tangent_of_adaptive_dverk_counts_six_stages():
+    # sixth-order tangent: stages 1-5, 7, 8 (a-row nonzeros 0+1+2+3+4+5+6 = 21),
+    # 6 nonzero b; fifth-order: stages 1-6 (0+1+2+3+4+5 = 15), 5 nonzero b - e.
+    # Per vector: Rossler J v costs 7 a stage; stage sums 2*nnz + 1 per component
+    # with a nonzero row; the increment 2*nnz + 1.
+    n = 3
+    full = 7 * 7 + n * sum(2 * z + 1 for z in (1, 2, 3, 4, 5, 6)) + n * (2 * 6 + 1)
+    low = 6 * 7 + n * sum(2 * z + 1 for z in (1, 2, 3, 4, 5)) + n * (2 * 5 + 1)
+    acc64, rej64 = F.per_step(L.MODEL_ROSSLER, n, 2, L.INT_DVERK, True)
+    acc32, rej32 = F.per_step(L.MODEL_ROSSLER, n, 2, L.INT_DVERK, True, True)
+    assert rej32 == rej64
+    assert acc64 - rej64 == 2 * full and acc32 - rej32 == 2 * low
 
 
 def test_trace_closure_step_counts_two_vectors_and_the_trace():

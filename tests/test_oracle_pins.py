@@ -241,6 +241,61 @@ def test_coupled_tangent_is_derivative_of_step(integ, model, p, n, f):
     np.testing.assert_allclose(tv, fd, rtol=1e-7, atol=1e-8)
 
 
+@pytest.mark.parametrize("model,p,x0,h", [(O.MODEL_LORENZ, [10.0, 28.0, 8 / 3], [5.0, 8.0, 20.0], 0.1),
+                                          (O.MODEL_ROSSLER, [0.2, 5.7, 0.2], [1.0, 2.0, 3.0], 0.3),
+                                          (O.MODEL_SHIMIZU, [0.75, 0.45], [1.0, 2.0, 3.0], 0.5)])
+def test_low_order_tangent_is_derivative_of_the_fifth_order_step(model, p, x0, h):
+    # the mixed form's tangent with adaptive DVERK steps takes the embedded
+    # fifth-order weights: it is the derivative of x5 = x_new - err (b - e),
+    # not of the sixth-order x_new; at a step large enough for the two
+    # solutions' derivatives to differ far beyond binary32 rounding, only the
+    # right one agrees with central differences
+    x0 = np.array(x0)
+    v = np.array([0.3, -0.7, 0.5])
+    _, _, sx = O.rk_step(model, O.INT_DVERK, x0, p, h)
+
+    def fd(g):
+        d = 1e-6
+        return (g(x0 + d * v) - g(x0 - d * v)) / (2 * d)
+
+    def x6(x):
+        return O.rk_step(model, O.INT_DVERK, x, p, h)[0]
+
+    def x5(x):
+        xn, ev, _ = O.rk_step(model, O.INT_DVERK, x, p, h)
+        return xn - ev
+
+    d5, d6 = fd(x5), fd(x6)
+    scale = np.abs(d6).max()
+    tol = 1e-6 * scale
+    assert np.abs(d5 - d6).max() > 100 * tol  # the test can tell the two apart
+    low = O.tangent_step_f32(model, O.INT_DVERK, sx, p, h, v, low=True)
+    full = O.tangent_step_f32(model, O.INT_DVERK, sx, p, h, v, low=False)
+    assert np.abs(low - d5).max() < tol
+    assert np.abs(full - d6).max() < tol
+    # the FP64 tangent keeps the sixth-order weights
+    np.testing.assert_allclose(O.tangent_step(model, O.INT_DVERK, sx, p, h, v), d6, rtol=1e-7,
+                               atol=1e-8 * scale)
+
+
+def test_fifth_order_weights_are_b_minus_e_and_fifth_order():
+    from fractions import Fraction as Fr
+
+    A = [[0], [Fr(1, 6)], [Fr(4, 75), Fr(16, 75)], [Fr(5, 6), Fr(-8, 3), Fr(5, 2)],
+         [Fr(-165, 64), Fr(55, 6), Fr(-425, 64), Fr(85, 96)],
+         [Fr(12, 5), Fr(-8), Fr(4015, 612), Fr(-11, 36), Fr(88, 255)]]
+    b = [Fr(3, 40), 0, Fr(875, 2244), Fr(23, 72), Fr(264, 1955), 0, Fr(125, 11592), Fr(43, 616)]
+    e = [Fr(-1, 160), 0, Fr(-125, 17952), Fr(1, 144), Fr(-12, 1955), Fr(-3, 44), Fr(125, 11592),
+         Fr(43, 616)]
+    bh = [bj - ej for bj, ej in zip(b, e)]
+    assert bh[6] == 0 and bh[7] == 0  # stages 7 and 8 drop out
+    assert bh[:6] == [Fr(13, 160), 0, Fr(2375, 5984), Fr(5, 16), Fr(12, 85), Fr(3, 44)]
+    c = [sum(row, Fr(0)) for row in A]
+    for q in range(1, 6):
+        assert sum(bh[i] * c[i] ** (q - 1) for i in range(6)) == Fr(1, q)
+    assert sum(bh[i] * c[i] ** 5 for i in range(6)) != Fr(1, 6)  # fifth order, not sixth
+
+
 @pytest.mark.parametrize("model,p,n,f", MODELS)
 def test_frozen_tangent_is_dverk_on_linear_system(model, p, n, f):
     # cuda/tools/dverk.dp.cpp:46-80 applies DVERK to v' = J(x*) v: exp(hJ) v to O(h^7)
