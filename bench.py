@@ -249,16 +249,26 @@ def bench_sweep(args, ctx, spec, wl_name):
             ctx.set_option("order_probe", args.order_probe)
     elif desc.model == L.MODEL_L96:
         tuned = A.tune_l96(ctx, desc, cache=A.Cache(None), time_scale=0.1)  # sets the tile shape on ctx
-    elif desc.dim == 3:
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    # an adaptive 3-D grid of at most a few points per lane: the heavy-first
+    # order and the launch shape set its length together, so the order is
+    # tuned first and the shape on full runs in that order
+    few_rounds = (not args.no_tune and desc.dim == 3 and desc.n1 * desc.n2 <= 4 * A.resident_points(desc, cus)
+                  and len(A.order_candidates(desc, n, A.resident_points(desc, cus))) > 1)
+    tuned_order = None
+    if few_rounds:
+        tuned_order = A.tune_order(ctx, desc, A.resident_points(desc, cus), cache=A.Cache(None))
+        tuned_lanes = A.tune_lanes(ctx, desc, cache=A.Cache(None), time_scale=1.0)
+    elif desc.dim == 3 and not args.no_tune:
         # persistent grid cap and renormalisation gather, set on ctx for the
         # chosen form and the plain FP64 sweep alike
         tuned_lanes = A.tune_lanes(ctx, desc, cache=A.Cache(None), time_scale=0.1)
     # heavy-first order of adaptive grids that start their points in a few
     # rounds of the launch, timed on full runs (probe, sort and scatter included)
-    cus = torch.cuda.get_device_properties(0).multi_processor_count
     resident = A.resident_points(desc, cus, int(tuned.best[1]) if tuned is not None and tuned.best[0] == "ppw"
                                  else 0)
-    tuned_order = None if args.no_tune else A.tune_order(ctx, desc, resident, cache=A.Cache(None))
+    if tuned_order is None and not args.no_tune:
+        tuned_order = A.tune_order(ctx, desc, resident, cache=A.Cache(None))
     stream = torch.cuda.current_stream()
     dev = torch.device("cuda", 0)
     bufs = (torch.empty((n, desc.n_exps), dtype=torch.float64, device=dev),
@@ -297,7 +307,10 @@ def bench_sweep(args, ctx, spec, wl_name):
     # the plain FP64 sweep with a launch shape tuned for it (its own method and
     # precision), not the chosen form's
     tuned_plain = None
-    if tuned_lanes is not None:
+    if tuned_lanes is not None and few_rounds:
+        A.tune_order(ctx, plain_desc_of(desc), resident, cache=A.Cache(None))
+        tuned_plain = A.tune_lanes(ctx, plain_desc_of(desc), cache=A.Cache(None), time_scale=1.0, apply=False)
+    elif tuned_lanes is not None:
         tuned_plain = A.tune_lanes(ctx, plain_desc_of(desc), cache=A.Cache(None), time_scale=0.1,
                                    apply=False)
     lam_p, st_p, plain_ms, flops_p, order_plain = plain_fp64(
@@ -329,6 +342,7 @@ def bench_sweep(args, ctx, spec, wl_name):
                                        "lane_gather": int(tuned_lanes.best[1]),
                                        "lane_quad": int(tuned_lanes.best[2])}
         rec["config"]["lane_tuning_ms_scaled"] = {str(k): round(v, 1) for k, v in tuned_lanes.timings.items()}
+        rec["config"]["lane_tuning_time_scale"] = 1.0 if few_rounds else 0.1
         rec["config"]["plain_fp64_lane_shape"] = {"lane_max_blocks": int(tuned_plain.best[0]),
                                                   "lane_gather": int(tuned_plain.best[1]),
                                                   "lane_quad": int(tuned_plain.best[2])}

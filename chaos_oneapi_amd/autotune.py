@@ -277,14 +277,17 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
     lanes of a wave gather before renormalising together, and whether a point
     takes four lanes (small grids, whose launches last as long as their slowest
     point).  With time_scale the candidates run over the whole grid at that
-    fraction of the integration times; without it, on a sample of the points."""
+    fraction of the integration times (1.0: the full run, in the heavy-first
+    order set on ctx, for grids of a few points per lane, whose launch length
+    the order and the shape set together); without it, on a sample of the
+    points."""
     import torch
 
     cache = cache if cache is not None else Cache()
     n_total = desc.n1 * desc.n2
     name = device_name(ctx.device) if timer is None else "test"
     cus = torch.cuda.get_device_properties(ctx.device).multi_processor_count if timer is None else 256
-    key = f"lanes3|{name}|{desc_signature(desc)}x{_bucket(n_total)}"
+    key = f"lanes4|{name}|{desc_signature(desc)}x{_bucket(n_total)}|t{time_scale}"
     hit = cache.get(key)
     if hit and not force:
         res = TuneResult(best=tuple(hit["best"]), timings=hit["timings"], cached=True)
@@ -295,7 +298,7 @@ def tune_lanes(ctx, desc, frac: float = 1 / 16, min_points: int = 1 << 16,
             run_desc = desc
             idx = torch.as_tensor(sample_indices(n_total, frac, min_points, 4), dtype=torch.int64)
         timings = {}
-        for cand in ((0, 0, 0), (0, 8, 0), (0, 16, 0), (2 * cus, 0, 0), (2 * cus, 8, 0), (0, 0, 1)):
+        for cand in ((0, 0, 0), (0, 4, 0), (0, 8, 0), (0, 16, 0), (2 * cus, 0, 0), (2 * cus, 8, 0), (0, 0, 1)):
             if timer:
                 timings[cand] = timer(cand, idx)
                 continue

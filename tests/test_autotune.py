@@ -104,7 +104,7 @@ def test_tune_lanes_grid_caps():
     r = A.tune_lanes(None, desc, cache=A.Cache(None),
                      timer=lambda c, idx: 2.0 - 0.01 * c[1] + (c[0] > 0) + 0.5 * c[2])
     assert r.best == (0, 16, 0)
-    assert set(r.timings) == {(0, 0, 0), (0, 8, 0), (0, 16, 0), (512, 0, 0), (512, 8, 0), (0, 0, 1)}
+    assert set(r.timings) == {(0, 0, 0), (0, 4, 0), (0, 8, 0), (0, 16, 0), (512, 0, 0), (512, 8, 0), (0, 0, 1)}
     # a small grid whose launch is its slowest point: four lanes per point wins
     r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=lambda c, idx: 2.0 - 1.5 * c[2])
     assert r.best == (0, 0, 1)
@@ -167,7 +167,7 @@ def test_tune_lanes_time_scaled_runs_the_whole_grid():
 
     r = A.tune_lanes(None, desc, cache=A.Cache(None), timer=timer, time_scale=0.1)
     assert r.best == (512, 8, 0)
-    assert len(seen) == 6 and all(i is None for i in seen)
+    assert len(seen) == 7 and all(i is None for i in seen)
 
 
 def test_order_candidates_need_an_adaptive_grid_of_few_rounds():

@@ -136,3 +136,19 @@ def test_order_probe_range(gpu_ctx):
     for bad in (-1, 9):
         assert lib.lyap_ctx_set_option(gpu_ctx._h, b"order_radius", bad) == L.ERR_INVALID
     assert lib.lyap_ctx_set_option(gpu_ctx._h, b"order_radius", 2) == L.SUCCESS
+
+
+def test_heavy_first_with_parameter_rows(gpu_ctx):
+    """Caller-supplied parameter rows (no grid): the points are ordered by their
+    own probe counts, and every output is the unordered run's bit for bit."""
+    spec, opts = CASES["rossler-trace-lane"]
+    desc = L.finalize(L.make_desc(spec))
+    n = 400
+    g = torch.Generator().manual_seed(3)
+    a = 0.05 + 0.35 * torch.rand(n, generator=g, dtype=torch.float64)
+    c = 2.0 + 18.0 * torch.rand(n, generator=g, dtype=torch.float64)
+    params = torch.stack([a, c, torch.full((n,), 0.2, dtype=torch.float64)], dim=1).contiguous().cuda()
+    print("case order with 400 parameter rows", flush=True)
+    base = run_opts(gpu_ctx, desc, opts, count=n, params=params)
+    got = run_opts(gpu_ctx, desc, dict(opts, order_probe=30), count=n, params=params)
+    assert_same(got, base)
