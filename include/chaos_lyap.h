@@ -130,7 +130,10 @@ typedef struct lyap_ctx lyap_ctx;
 /* message of the last failing call on this thread ("" if none) */
 const char* lyap_last_error(void);
 
-/* a context per (host thread, device); owns the work counters */
+/* a context per (host thread, device); owns the work counters, the staging
+   buffers of heavy-first runs and the watch counters, so runs on one context
+   follow each other on a stream: two runs of one context in flight at once on
+   different streams share them and race (use a context per concurrent run) */
 int lyap_ctx_create(int device, lyap_ctx** out);
 int lyap_ctx_destroy(lyap_ctx* ctx);
 
