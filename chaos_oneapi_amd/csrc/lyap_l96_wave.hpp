@@ -2,15 +2,16 @@
 // (lyap_l96.hip) and the wave-level drain used by the batched launch.
 //
 // Wave-level drain layout: a wave holds P points ("slots") on LP = 64/P lanes
-// each, as G = LP/N groups of N lanes (WaveMap: consecutive lanes for N >= 16,
-// two groups interleaved per 16-lane row for N = 8).  Lane (group vg of the
+// each, as G = LP/N groups of N lanes (WaveMap: consecutive lanes for N = 16,
+// two groups interleaved per 16-lane row for N = 8, and for N = 32 a row pair
+// whose first row holds the even components and second row the odd ones).  Lane (group vg of the
 // slot, component i) owns component i of the base state and of E = N/G
 // deviation vectors, r = vg + G*m, m < E; u = vg*N + i is its position in the
 // slot.  The slot's base state is replicated in each of its G groups (G = 1 at
 // P = 64/N: no replication).
 //   - neighbour couplings are moves inside the N-lane group (Ring: DPP row
-//     rotations for N = 8 and 16; for N = 32, row shifts whose edge lanes
-//     take the other row of the pair through v_permlane16_swap);
+//     rotations for N = 8 and 16; for N = 32, the other row of the pair
+//     through v_permlane16_swap and row-masked DPP rotations);
 //   - the error norm and the Jacobian-norm cap are butterflies over the group
 //     (Ring::sum / Ring::max), bit-identical in every lane of the group;
 //   - renormalisation stages the slot's N x N deviation matrix in the wave's
@@ -83,6 +84,26 @@ LYAP_D double dpp_shift(double old, double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
+// Rotation by R lanes within the 16-lane rows selected by ROWS (bit r: row r);
+// the other rows keep `old`.
+template <int R, int ROWS>
+LYAP_D uint32_t dpp_ror_rows_u32(uint32_t old, uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, 0x120 + R, ROWS, 0xF, false);
+}
+// Neighbours of component i = 2m + r on the interleaved row pair (row r of the
+// pair holds the components of parity r, lane m of the row component 2m + r):
+// one v_permlane16_swap of v gives the pair's even and odd components in both
+// rows, and i+1 is the odd row's value (even rows) or the even row's rotated by
+// one lane (odd rows), i-1 the odd row's rotated (even rows) or the even row's
+// value (odd rows), each one row-masked DPP move; i-2 is the own row rotated.
+LYAP_D void pair_il_moves_u32(uint32_t v, uint32_t& p1, uint32_t& m1, uint32_t& m2) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    const uint32_t ev = r[0], od = r[1];
+    p1 = dpp_ror_rows_u32<15, 0xA>(od, ev);
+    m1 = dpp_ror_rows_u32<1, 0x5>(ev, od);
+    m2 = (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x121, 0xF, 0xF, true);
+}
+
 // Couplings x_{i+1}, x_{i-1}, x_{i-2} of component i inside its N-lane group
 // (periodic).  CS is the lane stride between consecutive components: 1 for
 // groups of consecutive lanes, 2 for two N = 8 groups interleaved in one
@@ -91,12 +112,17 @@ LYAP_D double dpp_shift(double old, double v) {
 // span a row pair: a component's neighbours are the row rotation of its own
 // row, except at the row's edge lanes, which take the rotation of the other
 // row of the pair (v_permlane16_swap) — all on the VALU.  Other groups move by
-// lane shuffles (ds_bpermute).  Every lane of a group is active whenever one
-// is (the callers' control is group-uniform).
-template <int N, int CS = 1>
+// lane shuffles (ds_bpermute).  IL (N = 32): the group's components are
+// interleaved over its row pair, even components in the first row and odd
+// ones in the second (WaveMap), so that the neighbours are the other row's
+// value or a rotation of it (pair_il_moves_u32): four moves where the
+// consecutive layout takes eight.  Every lane of a group is active whenever
+// one is (the callers' control is group-uniform).
+template <int N, int CS = 1, bool IL = false>
 struct Ring {
+    static constexpr bool kIL = IL && N == 32 && CS == 1;
     static constexpr bool kDpp = N * CS == 16;
-    static constexpr bool kPair = N == 32 && CS == 1;
+    static constexpr bool kPair = N == 32 && CS == 1 && !kIL;
     int src_p1, src_m1, src_m2;
     bool odd_row;  // rows 1 and 3 of the wave
     LYAP_D explicit Ring(int lane) {
@@ -113,7 +139,23 @@ struct Ring {
     // edge lanes keep the other row's value rotated into their place
     template <class T>
     LYAP_D void moves(T v, T& vp1, T& vm1, T& vm2) const {
-        if constexpr (kPair) {
+        if constexpr (kIL) {
+            if constexpr (sizeof(T) == 4) {
+                uint32_t a, b, c;
+                pair_il_moves_u32(__builtin_bit_cast(uint32_t, v), a, b, c);
+                vp1 = __builtin_bit_cast(T, a);
+                vm1 = __builtin_bit_cast(T, b);
+                vm2 = __builtin_bit_cast(T, c);
+            } else {
+                const uint64_t w = __builtin_bit_cast(uint64_t, v);
+                uint32_t a0, b0, c0, a1, b1, c1;
+                pair_il_moves_u32((uint32_t)w, a0, b0, c0);
+                pair_il_moves_u32((uint32_t)(w >> 32), a1, b1, c1);
+                vp1 = __builtin_bit_cast(T, ((uint64_t)a1 << 32) | (uint64_t)a0);
+                vm1 = __builtin_bit_cast(T, ((uint64_t)b1 << 32) | (uint64_t)b0);
+                vm2 = __builtin_bit_cast(T, ((uint64_t)c1 << 32) | (uint64_t)c0);
+            }
+        } else if constexpr (kPair) {
             const T pv = row_partner(v, odd_row);
             vp1 = dpp_shift<-1>(dpp_ror<15>(pv), v);
             vm1 = dpp_shift<1>(dpp_ror<1>(pv), v);
@@ -126,19 +168,28 @@ struct Ring {
     }
     template <class T>
     LYAP_D T p1(T v) const {
-        if constexpr (kDpp) return dpp_ror<16 - CS>(v);
+        if constexpr (kIL) {
+            T a, b, c;
+            moves(v, a, b, c);
+            return a;
+        } else if constexpr (kDpp) return dpp_ror<16 - CS>(v);
         else if constexpr (kPair) return dpp_shift<-1>(dpp_ror<15>(row_partner(v, odd_row)), v);
         else return __shfl(v, src_p1);
     }
     template <class T>
     LYAP_D T m1(T v) const {
-        if constexpr (kDpp) return dpp_ror<CS>(v);
+        if constexpr (kIL) {
+            T a, b, c;
+            moves(v, a, b, c);
+            return b;
+        } else if constexpr (kDpp) return dpp_ror<CS>(v);
         else if constexpr (kPair) return dpp_shift<1>(dpp_ror<1>(row_partner(v, odd_row)), v);
         else return __shfl(v, src_m1);
     }
     template <class T>
     LYAP_D T m2(T v) const {
-        if constexpr (kDpp) return dpp_ror<2 * CS>(v);
+        if constexpr (kIL) return dpp_ror<1>(v);
+        else if constexpr (kDpp) return dpp_ror<2 * CS>(v);
         else if constexpr (kPair) return dpp_shift<2>(dpp_ror<2>(row_partner(v, odd_row)), v);
         else return __shfl(v, src_m2);
     }
@@ -148,7 +199,16 @@ struct Ring {
     // (in components), so the value o components away by rotation is the xor
     // partner's value, and a + b == b + a.
     LYAP_D double sum(double v) const {
-        if constexpr (kDpp || kPair) {
+        if constexpr (kIL) {
+            // component offsets 16, 8, 4, 2 are lane offsets 8, 4, 2, 1 of the
+            // row; offset 1 is the other row of the pair
+            v = v + dpp_ror<8>(v);
+            v = v + dpp_ror<4>(v);
+            v = v + dpp_ror<2>(v);
+            v = v + dpp_ror<1>(v);
+            v = v + row_partner(v, odd_row);
+            return v;
+        } else if constexpr (kDpp || kPair) {
             if constexpr (kPair) v = v + row_partner(v, odd_row);  // offset 16: the other row
             if constexpr (N >= 16) v = v + dpp_ror<8 * CS>(v);
             v = v + dpp_ror<4 * CS>(v);
@@ -163,7 +223,14 @@ struct Ring {
     }
     LYAP_D double max(double v) const {
         auto mx = [](double a, double b) { return a > b ? a : b; };
-        if constexpr (kDpp || kPair) {
+        if constexpr (kIL) {
+            v = mx(v, dpp_ror<8>(v));
+            v = mx(v, dpp_ror<4>(v));
+            v = mx(v, dpp_ror<2>(v));
+            v = mx(v, dpp_ror<1>(v));
+            v = mx(v, row_partner(v, odd_row));
+            return v;
+        } else if constexpr (kDpp || kPair) {
             if constexpr (kPair) v = mx(v, row_partner(v, odd_row));
             if constexpr (N >= 16) v = mx(v, dpp_ror<8 * CS>(v));
             v = mx(v, dpp_ror<4 * CS>(v));
@@ -179,15 +246,24 @@ struct Ring {
 };
 
 // Lanes of the wave-level drain: lane -> (group q, component c).  N >= 16:
-// group q holds lanes q*N .. q*N+N-1.  N = 8: the two groups of each 16-lane
-// row are interleaved (lane = 16*row + 2*c + h, q = 2*row + h), so that the
-// component rotations above are row rotations.
-template <int N>
+// group q holds lanes q*N .. q*N+N-1; with IL and N = 32 component c sits at
+// lane q*32 + 16*(c & 1) + c/2 (even components in the first row of the pair,
+// odd ones in the second).  N = 8: the two groups of each 16-lane row are
+// interleaved (lane = 16*row + 2*c + h, q = 2*row + h), so that the component
+// rotations above are row rotations.
+template <int N, bool IL = false>
 struct WaveMap {
     static constexpr int CS = N >= 16 ? 1 : 16 / N;
-    LYAP_D static int comp(int lane) { return CS == 1 ? lane % N : (lane & 15) / CS; }
+    static constexpr bool kIL = IL && N == 32;
+    LYAP_D static int comp(int lane) {
+        if constexpr (kIL) return 2 * (lane & 15) + ((lane >> 4) & 1);
+        else return CS == 1 ? lane % N : (lane & 15) / CS;
+    }
     LYAP_D static int group(int lane) { return CS == 1 ? lane / N : (lane >> 4) * CS + (lane & 15) % CS; }
-    LYAP_D static int lane_of(int q, int c) { return CS == 1 ? q * N + c : 16 * (q / CS) + c * CS + q % CS; }
+    LYAP_D static int lane_of(int q, int c) {
+        if constexpr (kIL) return q * N + (c & 1) * 16 + (c >> 1);
+        else return CS == 1 ? q * N + c : 16 * (q / CS) + c * CS + q % CS;
+    }
     // ballot bits of the lanes of group q
     LYAP_D static uint64_t group_bits(int q) {
         if constexpr (CS == 1) return (N == 64 ? ~0ULL : ((1ULL << N) - 1ULL)) << (q * N);
@@ -219,7 +295,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
     constexpr int S = TB::S;
     static_assert(64 % P == 0 && LP % N == 0 && K % G == 0 && E >= 1, "layout");
 
-    using Map = WaveMap<N>;
+    using Map = WaveMap<N, true>;  // N = 32: components interleaved over each row pair
     constexpr int CS = Map::CS;
     const int lane = (int)__lane_id();
     const int q = Map::group(lane);   // N-lane group of the wave
@@ -231,7 +307,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
     uint64_t slot_bits = 0;
 #pragma unroll
     for (int g = 0; g < G; ++g) slot_bits |= Map::group_bits(slot * G + g);
-    const Ring<N, CS> ring(lane);
+    const Ring<N, CS, true> ring(lane);
     constexpr int RS = N + 1;  // LDS row stride (doubles)
     double* sV = lds + slot * (N * RS);
 
