@@ -131,3 +131,26 @@ def test_batch_rejects_bad_classes(gpu_ctx):
     nogrid[0]["params"] = None
     with pytest.raises(L.LyapError, match="beyond the"):
         L.Batch(gpu_ctx, nogrid)
+
+
+@pytest.mark.parametrize("trace", [False, True])
+def test_dverk_batch_matches_oracle(gpu_ctx, trace):
+    """The batch built for adaptive DVERK: its 3-D classes in the mixed form
+    take the fifth-order binary32 tangent (as the single-class sweep does), the
+    Lorenz-96 classes the sixth-order one; every class bit for bit against the
+    oracle."""
+    mix = W.batch_mix(600, seed=19, time_scale=0.005)
+    classes = device_classes(mix, L.PREC_MIXED, trace)
+    for c in classes:
+        d = L.SweepDesc.from_buffer_copy(c["desc"])
+        d.integrator = L.INT_DVERK
+        c["desc"] = L.finalize(d)
+    print("case dverk batch-vs-oracle trace", trace, [(c["name"], c["count"]) for c in mix], flush=True)
+    b = L.Batch(gpu_ctx, classes)
+    outs = to_host(b.launch())
+    torch.cuda.synchronize()
+    b.close()
+    for c, cl, g in zip(mix, classes, outs):
+        od = O.make_desc(cl["desc"].to_dict())
+        o = O.sweep(od, count=c["count"], params_in=c["params"])
+        assert_same(g, o, c["name"])
