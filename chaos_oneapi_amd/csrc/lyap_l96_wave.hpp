@@ -84,6 +84,18 @@ LYAP_D double dpp_shift(double old, double v) {
     return __builtin_bit_cast(double, ((uint64_t)hi << 32) | (uint64_t)lo);
 }
 
+// acc += (lane i - R of v within its 16-lane row) * w: one v_fmac_f32 with the
+// row rotation as its DPP source (the compiler folds a rotation into a VOP2
+// multiply but not into an fma).  The s_nop gives the two wait states a DPP
+// read of a VGPR written by the previous VALU instruction needs.
+template <int R>
+LYAP_D void fmac_ror_f32(float& acc, float v, float w) {
+    static_assert(R >= 1 && R <= 15, "row rotation");
+    __asm__("s_nop 1\n\tv_fmac_f32_dpp %0, %1, %2 row_ror:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+            : "+v"(acc)
+            : "v"(v), "v"(w), "i"(R));
+}
+
 // Rotation by R lanes within the 16-lane rows selected by ROWS (bit r: row r);
 // the other rows keep `old`.
 template <int R, int ROWS>
@@ -532,6 +544,17 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
 #pragma unroll
                         for (int q2 = 0; q2 < NV; ++q2) {
                             T vp1, vm1, vm2;
+                            if constexpr (MIXED && Ring<N, CS, true>::kDpp) {
+                                // v(i+1) - v(i-2) is d(i+1) for d(j) = v(j) - v(j-3): one DPP
+                                // subtraction, and its move to lane i rides on the
+                                // multiply-add (the same operations as the general form)
+                                const T dlt = varg[q2] - dpp_ror<3 * CS>(varg[q2]);
+                                vm1 = dpp_ror<CS>(varg[q2]);
+                                T acc = fma(-dh, varg[q2], jd[s] * vm1);
+                                fmac_ror_f32<16 - CS>(acc, dlt, jm1[s]);
+                                kv[q2][s] = acc;
+                                continue;
+                            }
                             ring.moves(varg[q2], vp1, vm1, vm2);
                             // binary32: the jd*v(i-1) product first, so its lane move folds
                             // into a DPP multiply (the FP64 form keeps the fused order)
