@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64 * W) void k_l96(const lyap_sweep_desc d, const S
                     }
                     const double ev = h_try * acce;
                     const double ax = fabs(x), an = fabs(xn);
-                    const double sk = fma(rtol, ax > an ? ax : an, atol);
+                    const double sk = fma(rtol, fmax(ax, an), atol);
                     const double q = ev / sk;
                     const double err = sqrt(ring.sum(q * q) / (double)N);
                     if (wave_uniform(!(err < kInf))) {

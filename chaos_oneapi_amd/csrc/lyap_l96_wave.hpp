@@ -478,7 +478,7 @@ LYAP_D void l96_wave_drain(const lyap_sweep_desc& d, GetDesc getd, const SweepIO
                 }
                 const double ev = h_try * acce;
                 const double ax = fabs(x), an = fabs(xn);
-                const double sk = fma(rtol, ax > an ? ax : an, atol);
+                const double sk = fma(rtol, fmax(ax, an), atol);
                 const double qe = ev / sk;
                 const double err = sqrt(ring.sum(qe * qe) / (double)N);
                 if (slot_uniform(!(err < kInf))) {

@@ -173,12 +173,16 @@ LYAP_D void mgs_rows(double (&V)[K][N], double* nrm_out) {
 // left out).  Three divisions, a square root and a division by N become one
 // division; the 3-D flows only (products of N - 1 squared scales in range).
 template <int N>
+// fmax(|x|, |x_new|): one v_max_f64 with abs modifiers.  It differs from a
+// compare-and-select only when |x_new| is NaN, and then e is not finite either
+// (every stage that feeds x_new also feeds the error estimate), so the norm is
+// not finite and the point ends as non-finite all the same.
 LYAP_D double err2_seq(const double* ev, const double* x, const double* xn, double rtol, double atol) {
     double A[N], pre[N], suf[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const double ax = fabs(x[i]), an = fabs(xn[i]);
-        const double sk = fma(rtol, ax > an ? ax : an, atol);
+        const double sk = fma(rtol, fmax(ax, an), atol);
         A[i] = sk * sk;
     }
     pre[0] = 1.0;

@@ -724,7 +724,7 @@ double orc_err_norm(int n, int tree, const double* ev, const double* x, const do
     if (n <= 0) return 0.0;
     for (int i = 0; i < n; ++i) {
         double ax = fabs(x[i]), an = fabs(xn[i]);
-        double sk = fma(rtol, ax > an ? ax : an, atol);
+        double sk = fma(rtol, fmax(ax, an), atol);
         q[i] = ev[i] / sk;
     }
     double s;
@@ -749,7 +749,7 @@ double orc_err2_seq(int n, const double* ev, const double* x, const double* xn, 
     if (n <= 0) return 0.0;
     for (int i = 0; i < n; ++i) {
         double ax = fabs(x[i]), an = fabs(xn[i]);
-        double sk = fma(rtol, ax > an ? ax : an, atol);
+        double sk = fma(rtol, fmax(ax, an), atol);
         A[i] = sk * sk;
     }
     pre[0] = 1.0;
