@@ -519,6 +519,7 @@ def bench_surrogate(args, ctx, spec, wl_name):
         # per-exponent error maps of the mixed form against FP64 over the plane
         "dlambda_mixed_vs_plain_fp64_per_exponent": error_map_stats(lam_m, lam_p, (st_m == 0) & okp, desc.n1, desc.n2),
         "rerun_lane_gather": st.get("rerun_gather"),
+        "launch_free_blocks": st.get("train_blocks"),
         "one_launch": st.get("one_launch"),
         "computed_points": st.get("computed_points"),
         "speculative_points": st.get("speculative_points"),

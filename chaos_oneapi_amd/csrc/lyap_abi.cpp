@@ -490,7 +490,8 @@ int lyap_ctx_set_option(lyap_ctx* c, const char* key, int64_t value) {
             return LYAP_SUCCESS;
         }
     if (k == "lane_max_blocks") {
-        if (value < 0) return fail(LYAP_ERR_INVALID, "lane_max_blocks must be >= 0");
+        if (value < -65536 || value > INT32_MAX)
+            return fail(LYAP_ERR_INVALID, "lane_max_blocks must be in [-65536, 2^31 - 1]");
         c->lane_max_blocks = (int)value;
         return LYAP_SUCCESS;
     }

@@ -524,7 +524,9 @@ __global__ void k_reset_watch(unsigned long long* done) {
 // ---------------------------------------------------------------------------
 // launch table
 // ---------------------------------------------------------------------------
-// lanes_per_point: 1 (k_var_lane and the reference kernels) or 4 (k_var_quad)
+// lanes_per_point: 1 (k_var_lane and the reference kernels) or 4 (k_var_quad);
+// max_blocks_hint > 0 caps the grid, < 0 leaves that many blocks of the
+// occupancy-sized grid free for kernels on other streams (at least one block runs)
 template <class K, class... Extra>
 static hipError_t launch_persistent(K kern, const lyap_sweep_desc& d, const SweepIO& io,
                                     hipStream_t stream, int num_cus, int max_blocks_hint,
@@ -534,6 +536,7 @@ static hipError_t launch_persistent(K kern, const lyap_sweep_desc& d, const Swee
     if (e != hipSuccess) return e;
     if (per_cu < 1) per_cu = 1;
     int64_t blocks = (int64_t)per_cu * num_cus;
+    if (max_blocks_hint < 0) blocks += max_blocks_hint;
     const int64_t need = (io.count * lanes_per_point + 255) / 256;
     if (blocks > need) blocks = need;
     if (max_blocks_hint > 0 && blocks > max_blocks_hint) blocks = max_blocks_hint;

@@ -169,7 +169,9 @@ int lyap_sweep_run(lyap_ctx* ctx, const lyap_sweep_desc* d, int64_t first, int64
  *   "l96_ppw_8", "l96_ppw_16", "l96_ppw_32": points per wave64 of the
  *       wave-level Lorenz-96 drain (N=8: 1, 2, 4, 8; N=16: 1, 2, 4; N=32: 1, 2);
  *       0 (default) = the block-per-point kernel above
- *   "lane_max_blocks": cap on the persistent grid of the 3-D kernels (0 = none)
+ *   "lane_max_blocks": cap on the persistent grid of the 3-D kernels (0 = none;
+ *       -k = the occupancy-sized grid less k blocks, whose slots stay free for
+ *       kernels on other streams; at least one block runs; -65536 and above)
  *   "l96_max_blocks": cap on the persistent grid of the Lorenz-96 kernels (0 = none)
  *   "lane_gather": 3-D kernels: lanes at a renormalisation wait until this many
  *       lanes of the wave wait, then renormalise together (0 or 1 = at once;

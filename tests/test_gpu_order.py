@@ -152,3 +152,15 @@ def test_heavy_first_with_parameter_rows(gpu_ctx):
     base = run_opts(gpu_ctx, desc, opts, count=n, params=params)
     got = run_opts(gpu_ctx, desc, dict(opts, order_probe=30), count=n, params=params)
     assert_same(got, base)
+
+
+def test_lane_grid_less_free_blocks(gpu_ctx):
+    """lane_max_blocks < 0 leaves that many blocks of the occupancy-sized grid
+    free (at least one block runs): the same bits as the full grid."""
+    lib = L.load()
+    assert lib.lyap_ctx_set_option(gpu_ctx._h, b"lane_max_blocks", -65537) == L.ERR_INVALID
+    spec, _ = CASES["rossler-trace-lane"]
+    desc = L.finalize(L.make_desc(spec))
+    base = run_opts(gpu_ctx, desc, {})
+    for free in (-32, -65536):
+        assert_same(run_opts(gpu_ctx, desc, {"lane_max_blocks": free}), base)
