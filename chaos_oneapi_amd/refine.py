@@ -32,11 +32,13 @@ training, inference and the choice of the rule follow on the main stream, and
 the points the rule answers are skipped from the moment their skip words are
 written.  The FP64 judge runs on the main stream first, on a few blocks
 (judge_blocks), beside the launch, and the launch's grid is the occupancy-sized
-one less train_blocks blocks (lane_max_blocks = -train_blocks), so that the
-judge's slots pass to training and inference when the judge ends rather than
-to launch blocks that would hold them to the end (2048^2 Rossler plane:
-training done ~3.6 s into a step instead of after the launch, 6.68 s a step
-against 6.74 s at 0 and 6.78 s at 16; 6.84 s at 64).  So no wave idles on the slowest point of a short
+one less train_blocks blocks (lane_max_blocks = -train_blocks; by default
+judge_blocks), so that the judge's slots pass to training and inference when
+the judge ends rather than to launch blocks that would hold them to the end
+(2048^2 Rossler plane: training done ~3.7 s into a step instead of after the
+launch; 6.68-6.73 s a step at 32 against 6.74 s at 0, 6.78 s at 16,
+6.83-6.84 s at 24, 6.74-6.76 s at 40, 6.79-6.80 s at 48, 6.84 s at 64).
+So no wave idles on the slowest point of a short
 launch: on the 2048^2 Rossler plane the lattice, calibration and verification
 points as launches of their own (one point per lane) kept their waves mostly
 masked for ~0.8 s, and a launch beside them could only use the issue slots
@@ -384,7 +386,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
            infer_batch: int = 65536, judge: bool = True, hidden: int = 256,
            fourier_sigma: float = 6.0, calib_frac: float = 0.01,
            calib_target: float = 0.9995, dlam_cal: float = 0.5 * DLAMBDA1_GATE,
-           rerun_gather: int = 4, one_launch: bool = True, judge_blocks: int = 32, train_blocks: int = 32,
+           rerun_gather: int = 4, one_launch: bool = True, judge_blocks: int = 32, train_blocks: int | None = None,
            split_blocks_per_cu: int = 1, watch_timeout_s: float = 600.0,
            order_probe: int = 10, order_radius: int = 2) -> RefineResult:
     import torch
@@ -475,6 +477,8 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
             lam_j, st_j, _, _ = judge_ctx.run(_judge_desc(fd), idx_list=verify, stream=main)
         ctx.set_option("item_order", 1)
         ctx.set_option("lane_gather", rerun_gather)
+        if train_blocks is None:
+            train_blocks = judge_blocks
         ctx.set_option("lane_max_blocks", -train_blocks)
         ctx.set_watch(n_t, n_t + n_cal, skip)
         try:
@@ -612,7 +616,7 @@ def refine(ctx, fine: dict, train_n: int = 256, epochs: int = 80, batch: int = 2
                  train_on_lattice=bool(lattice), one_launch=launches[0]["name"] == "one launch",
                  rerun_fraction=float(n_rerun) / n, losses=[float(x.item()) for x in losses],
                  computed_points=int(computed.sum().item()), speculative_points=speculative,
-                 rerun_gather=int(rerun_gather), train_blocks=int(train_blocks), launches=launches)
+                 rerun_gather=int(rerun_gather), train_blocks=int(train_blocks or 0), launches=launches)
     if judge:
         stats.update(verify_gates(flagged[verify].cpu().numpy(), pred[verify].cpu().numpy(),
                                   lam_v.cpu().numpy(), st_v.cpu().numpy(), lam_j.cpu().numpy(),
